@@ -1,0 +1,200 @@
+/*
+ * lepic_oracle.h -- CPU FP64 restatement of the LePIC per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (jd_lepic_amd, liblepic_hip.so) never links or calls it.
+ *
+ * Every routine restates the algorithm of a reference Fortran routine (cited
+ * per function, paths relative to the reference checkout) in plain C, with
+ * the same expression order, strict IEEE double arithmetic (built with
+ * -ffp-contract=off) and the same integer conversions, so that it can be
+ * compared bit for bit with the reference compiled by flang (oracle/_ref) and
+ * with the GPU FP64 kernels.
+ *
+ * Pinning status per function (what checks it besides itself):
+ *   lo_ran2            Park-Miller published known answer; oracle/_ref ran2.
+ *   lo_load_gauss      oracle/_ref load_gauss; Rayleigh/uniform closed form.
+ *   lo_part_mover      oracle/_ref part_mover (bit-exact); Boris closed forms.
+ *   lo_charge_deposition oracle/_ref charge_deposition (bit-exact); charge sum.
+ *   lo_calc_efield     oracle/_ref calc_Efield (bit-exact); quadratic phi.
+ *   lo_part_sorting    oracle/_ref part_sorting (bit-exact); stable sort.
+ *   lo_calc_rhs, lo_dens_red   closed forms (sums / maxima).
+ *   lo_sor_rb, lo_mg, lo_iterative_solver, lo_restriction, lo_prolongation:
+ *                      converged phi pinned by a direct sparse solve; single
+ *                      red/black sweep pinned by an independent Gauss-Seidel
+ *                      statement; restriction/prolongation by their linear
+ *                      maps.  Per-level sweep counts: parity unpinned (only
+ *                      the compiled reference could pin them and mg.f90 /
+ *                      sors.f90 need MPI, which the image lacks).
+ *   lo_ordering        closed form (union grid, linear interpolation).
+ *   lo_collisions      rates pinned statistically by constant cross sections;
+ *                      the exact event sequence is parity unpinned
+ *                      (collisions.f90 needs MPI, so it cannot be compiled).
+ */
+#ifndef LEPIC_ORACLE_H
+#define LEPIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LO_MAX_LEVELS 12
+#define LO_MAX_REAC 64       /* reactions per table in this restatement */
+#define LO_MAX_TYPES 32      /* npart, particle_info.h:14 */
+#define LO_MAX_RG 5          /* nm_rg, particle_info.h:45 */
+
+/* ---- domain: grid, walls and static magnetic field ---- */
+typedef struct {
+  int n1, n2;            /* cells; node arrays are (n1+3)*(n2+3), ix fastest */
+  double h1, h2;         /* cell size (m) */
+  double xmax, ymax, zmax;
+  double dt;
+  double qe;             /* FP64 value of the single-precision literal */
+  double amu;
+  double eps0;           /* k_eps0 * eps0 */
+  const int32_t *bcnd;   /* node array: -1 interior, >=1 Dirichlet wall label */
+  int ngrid;             /* number of wall labels (p_mac second index 0..ngrid) */
+  /* magnetic field (Bi(4,0:nB1+2,0:nB2+2), part_expmover.f90:29) */
+  int flag_B;            /* 1 when a field is present */
+  int nB1, nB2;          /* nB1 == 1: constant field Bi(1:3,1,1) */
+  double hB1, hB2;
+  int flag_gridB;        /* 1 when the B map grid differs from the E grid */
+  int flag_B_pos;        /* 1: positive ions are not magnetised */
+  const double *Bi;      /* 4 doubles per B node */
+} lo_domain;
+
+typedef struct {
+  double charge;         /* C */
+  double mass;           /* kg; neutrals negative (read_reactions.f90:204) */
+  double Nm;             /* macroparticle weight */
+  int tag_neg;           /* 1 if this species is the tagged negative ion */
+} lo_species;
+
+/* Tallies of part_mover (part_expmover.f90:42-44, 314-446). */
+typedef struct {
+  double P_loss[4];      /* P_loss(1:4,ptype) */
+  double cnt_dead;
+  long long np_lost;
+} lo_mover_tally;
+
+/* ---- RNG and Maxwellian draw ---- */
+double lo_ran2(int32_t *irand);
+void lo_load_gauss(double *vx, double *vy, double vt, const double rnd[2], double pi);
+
+/* ---- particle kernels ----
+ * vxp: AoS, 6 doubles per particle [x,y,z,vx,vy,vz] (vxp(6,nmax) column-major).
+ * flag_dead: one byte per particle.  p_mac: 2*(ngrid+1) doubles,
+ * p_mac[k*(ngrid+1)+igrid], k=0 count, k=1 energy.  ss2D_sink: node array or NULL.
+ * Returns 0, or a negative code for an unsupported option. */
+int lo_part_mover(const lo_domain *d, const lo_species *sp, const double *Ei,
+                  double *vxp, int8_t *flag_dead, long long *np_tot,
+                  lo_mover_tally *t, double *p_mac, double *ss2D_sink);
+void lo_charge_deposition(const lo_domain *d, const lo_species *sp,
+                          const double *vxp, long long np_tot, const double *kq,
+                          double *np_grid);
+void lo_kq(const lo_domain *d, double *kq);
+void lo_calc_rhs(int n1, int n2, int ntype, const double *charge,
+                 const double *const *np, double *rhs);
+void lo_dens_red(int n1, int n2, int ntype, const int32_t *bcnd,
+                 const double *const *np, double *const *np_red, double *np_mx);
+void lo_calc_efield(int n1, int n2, const double h[2], const double *phi,
+                    const int32_t *bcnd, double *Ei);
+void lo_part_sorting(int n1, int n2, const double h[2], double *vxp,
+                     long long np_tot, int32_t *Plist);
+
+/* ---- Poisson solve ---- */
+typedef struct {
+  int n1, n2, ng;
+  double h1, h2, eps0;
+  const int32_t *bcnd;
+} lo_poisson;
+
+/* Sweep log: for each V-cycle, 2*ng-1 entries (ksor per ig). */
+typedef struct {
+  int *ksor;             /* capacity cap entries, may be NULL */
+  int cap;
+  int count;
+} lo_sweep_log;
+
+void lo_sor_rb(double *u, const double *b, const double h[2], double eps0,
+               const int32_t *bcnd, int n1, int n2, double omega, double eps,
+               int kmg, int dig, double *res, int *ksor);
+void lo_restriction(const double *u, const double *b, const double h[2],
+                    double eps0, const int32_t *bcnd, int32_t *bcnd2h,
+                    double *r2h, double *e2h, int n1, int n2);
+void lo_prolongation(double *u, const double *e2h, const int32_t *bcnd,
+                     int n1, int n2);
+int lo_mg(const lo_poisson *P, double *u, const double *b, double omega,
+          double eps, int k, double *ktot, double *res, lo_sweep_log *log);
+int lo_iterative_solver(const lo_poisson *P, double *u, const double *b,
+                        int ncycl, double eps, int *k_out, double *ktot,
+                        double *res, lo_sweep_log *log);
+
+/* ---- cross-section tables (read_reactions.f90:511-697) ---- */
+typedef struct {
+  int ncol;                         /* reactions */
+  int col_info[LO_MAX_REAC][10];    /* col_info(icol,1:10), 1-based contents */
+  double Eth[LO_MAX_REAC];          /* sig_Eex(icol,1) (already scaled) */
+  int npt[LO_MAX_REAC];
+  const double *E[LO_MAX_REAC];     /* per-reaction energies (scaled) */
+  const double *S[LO_MAX_REAC];     /* per-reaction cross sections (scaled) */
+} lo_reaction_input;
+
+typedef struct {
+  int ncol, npt_mx;                 /* npt_mx = sig_npt_mx */
+  double *sig_Er;                   /* [npt_mx] union energy grid */
+  double *sig;                      /* [ncol][npt_mx] interpolated */
+  int col_info[LO_MAX_REAC][10];
+  double Eth[LO_MAX_REAC];
+  int p_ncol[LO_MAX_TYPES + 1];     /* 1-based ptype */
+  int sig_list[LO_MAX_TYPES + 1][LO_MAX_REAC];   /* 1-based icol -> 1-based reaction */
+  double sigv_mx[LO_MAX_TYPES + 1][LO_MAX_REAC]; /* [ptype][reaction-1] */
+} lo_reaction_table;
+
+/* Builds the union grid into caller buffers sig_Er (>= sum npt) and sig
+ * (>= ncol * sum npt).  mass: 1-based per ptype.  Returns 0 or <0 on error. */
+int lo_ordering(const lo_reaction_input *in, const double *mass, double qe,
+                int ntype_all, double *sig_Er, double *sig, lo_reaction_table *t);
+
+/* ---- null-collision MCC, check mode (collisions.f90:1-207, 209-1040) ---- */
+typedef struct {
+  int ntype;                 /* charged species 1..ntype carry particles */
+  int ntype_all;             /* including neutrals */
+  double charge[LO_MAX_TYPES + 1], mass[LO_MAX_TYPES + 1], Nm[LO_MAX_TYPES + 1];
+  double vt0[LO_MAX_TYPES + 1], np_mx[LO_MAX_TYPES + 1], nu_uplim[LO_MAX_TYPES + 1];
+  int ns_coll;
+  int plt_src;
+  int n_cell;
+  int n_rg;
+  int ixl_rg[LO_MAX_RG], ixr_rg[LO_MAX_RG], iyl_rg, iyr_rg;
+  int flag_ionrep;
+} lo_mcc_setup;
+
+typedef struct {
+  double *vxp[LO_MAX_TYPES + 1];     /* per charged species, capacity cap[] */
+  int8_t *flag_dead[LO_MAX_TYPES + 1];
+  long long np_tot[LO_MAX_TYPES + 1];
+  long long cap[LO_MAX_TYPES + 1];
+} lo_particles;
+
+typedef struct {
+  double *cnt_col;           /* [3][ncol][n_rg]: index (k*ncol + (c-1))*n_rg + rg */
+  double P_loss3[LO_MAX_TYPES + 1];
+  long long Nc[LO_MAX_TYPES + 1];
+  double nu_max[LO_MAX_TYPES + 1];
+  long long events[LO_MAX_REAC];     /* per reaction, number of real events */
+  int32_t *event_log;        /* optional: per candidate (ip, c_ind) pairs */
+  long long log_cap, log_len;
+} lo_mcc_out;
+
+int lo_collisions(const lo_domain *d, const lo_mcc_setup *s,
+                  const lo_reaction_table *t, lo_particles *p,
+                  const double *const *np_red, int32_t *iseed, lo_mcc_out *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
