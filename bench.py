@@ -1,0 +1,276 @@
+"""Benchmark of the per-timestep hot path on one MI355X.
+
+Workload (one GPU): 512 x 512 cells, dx = dy = 5e-5 m, grounded box, e- and
+Ar+ (39.95 u) at 100 macroparticles per cell each (52,428,800 particles),
+n0 = 1e16 m^-3, Maxwellian Te = 3 eV and Ti = 0.03 eV, Bz(x) Gaussian filter
+of 10 mT peak at L/2 and width L/8, background Ar at 0.3 Pa / 300 K with
+analytic cross sections (constant elastic 1e-19 m^2, excitation at 11.5 eV,
+Lotz ionization at 15.76 eV), dt = 5e-12 s, MG Poisson (9 levels) to 1e-10,
+sort and null-collision MCC every 10 steps.  Synthetic inputs from the
+device loader (counter-based RNG, recorded seed).
+
+One step = rhs -> Poisson -> E -> [sort] -> [MCC] -> push+deposit of both
+species (main.f90:689-1090 order).  The headline value is particles/s of
+push + deposit + MCC in the mixed-precision form (FP32 push/deposit with
+FP64 grid sums, FP32 MG smoothing in FP64 refinement); the plain FP64 GPU
+form is measured on the same workload beside it.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import jd_lepic_amd as L  # noqa: E402
+from jd_lepic_amd import reactions as RX, tables  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """HIP events on the library's stream (kernel timing) via libamdhip64."""
+
+    def __init__(self):
+        self.lib = C.CDLL("libamdhip64.so")
+        for n in ("hipEventCreate", "hipEventRecord", "hipEventSynchronize", "hipEventElapsedTime",
+                  "hipEventDestroy", "hipDeviceSynchronize"):
+            getattr(self.lib, n).restype = C.c_int
+        self.lib.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.lib.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, stream) == 0
+
+    def elapsed(self, a, b):
+        self.lib.hipEventSynchronize(b)
+        ms = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.lib.hipDeviceSynchronize() == 0
+
+
+def workload(n=512, ppc=100):
+    h = 5.0e-5
+    L1 = n * h
+    x = (np.arange(n + 3) - 1) * h
+    Bz = 0.01 * np.exp(-(x - L1 / 2) ** 2 / (2 * (L1 / 8) ** 2))   # Bfield.f90:54-57
+    Bi = np.zeros((n + 3, n + 3, 4))
+    Bi[..., 2] = Bz[None, :]
+    Bi[..., 3] = np.abs(Bi[..., 2])
+    bcnd = -np.ones((n + 3, n + 3), np.int32)
+    bcnd[:2, :] = bcnd[n + 1:, :] = bcnd[:, :2] = bcnd[:, n + 1:] = 1
+    ngas = 0.3 / (1.380649e-23 * 300.0)
+    chem = RX.argon(ngas)
+    n0, Vc = 1.0e16, h * h * 1.0
+    Nm = n0 * Vc / ppc
+    vte = np.sqrt(2.0 * L.QE * 3.0 / L.ME)
+    vti = np.sqrt(2.0 * L.QE * 0.03 / (39.95 * L.AMU))
+    vtg = np.sqrt(2.0 * L.QE * 0.02585 / (39.95 * L.AMU))
+    return dict(n=n, h=h, Bi=Bi, bcnd=bcnd, chem=chem, Nm=Nm, ppc=ppc, vte=vte, vti=vti, vtg=vtg,
+                dt=5.0e-12, ng=int(np.log2(n)), ngas=ngas)
+
+
+def build_ctx(W, precision, seed=20261202):
+    n = W["n"]
+    cap = int(n * n * W["ppc"] * 1.05) + 4096
+    sp = [L.Species(-L.QE, L.ME, W["Nm"], cap), L.Species(L.QE, 39.95 * L.AMU, W["Nm"], cap)]
+    ctx = L.Context(n, n, W["h"], W["h"], W["dt"], sp, ng=W["ng"], precision=precision, track_z=0)
+    ctx.set_boundary(W["bcnd"], np.zeros((n + 3, n + 3)))
+    ctx.set_bfield(W["Bi"], (n, n), (W["h"], W["h"]))
+    L1 = n * W["h"]
+    ctx.load_uniform(1, W["ppc"], W["vte"], 0.0, L1, seed)
+    ctx.load_uniform(2, W["ppc"], W["vti"], 0.0, L1, seed)
+    ctx.set_reactions(W["chem"], tables.ordering(W["chem"]),
+                      vt0=[W["vte"], W["vti"], W["vtg"]])
+    for p in (1, 2):
+        ctx.charge_deposition(p)
+    return ctx
+
+
+def run_mode(W, precision, steps, warmup, hip, seed=7):
+    ctx = build_ctx(W, precision)
+    stream = C.c_void_p()
+    ctx.lib.lepic_stream(ctx._ctx, C.byref(stream))
+    nsp = 2
+    ev = [hip.event() for _ in range(12)]
+    acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[], []])
+    # the first solve starts cold and converges from zero; later ones warm-start
+    for it in range(1, warmup + steps + 1):
+        timed = it > warmup
+        if it == warmup + 1:
+            hip.sync()
+            t0 = time.perf_counter()
+        hip.record(ev[0], stream)
+        ctx.calc_rhs()
+        info = ctx.iterative_solver(eps=1e-10, ncycl=20000,
+                                    start=L.START_COLD if it == 1 else L.START_PREV)
+        ctx.calc_efield()
+        hip.record(ev[1], stream)
+        if it % 10 == 1:
+            for p in (1, 2):
+                ctx.part_sorting(p)
+        hip.record(ev[2], stream)
+        if it > 1 and it % 10 == 1:
+            ctx.dens_red()
+            ctx.collisions(it, mode=L.MCC_PROD, seed=seed)
+        hip.record(ev[3], stream)
+        for p in (1, 2):
+            hip.record(ev[4 + 2 * (p - 1)], stream)
+            ctx.part_mover(p, deposit=True)
+            hip.record(ev[5 + 2 * (p - 1)], stream)
+        hip.record(ev[8], stream)
+        if timed:
+            acc["field"] += hip.elapsed(ev[0], ev[1])
+            acc["sort"] += hip.elapsed(ev[1], ev[2])
+            acc["mcc"] += hip.elapsed(ev[2], ev[3])
+            acc["push"] += hip.elapsed(ev[3], ev[8])
+            acc["vcycles"] += info["vcycles"]
+            for p in (0, 1):
+                acc["push_kernel_ms"][p].append(hip.elapsed(ev[4 + 2 * p], ev[5 + 2 * p]))
+    hip.sync()
+    wall = time.perf_counter() - t0
+    counts = [ctx.count(p) for p in (1, 2)]
+    ctx.close()
+    return acc, wall, counts
+
+
+def cpu_baseline(W, budget_s=10.0):
+    """The oracle (scalar C restatement, one core) on a bounded sample of the
+    same workload: 64 x 64 cells of it at 100 ppc for both species."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n, ppc, h = 64, W["ppc"], W["h"]
+    rng = np.random.default_rng(3)
+    N = n * n * ppc
+    pos = np.column_stack([rng.uniform(0, n * h, N), rng.uniform(0, n * h, N), np.zeros(N)])
+    ve = rng.normal(0, W["vte"] / np.sqrt(2), (N, 3))
+    vi = rng.normal(0, W["vti"] / np.sqrt(2), (N, 3))
+    bc = O.box_bcnd(n, n)
+    x = (np.arange(n + 3) - 1) * h
+    Bi = np.zeros((n + 3, n + 3, 4))
+    Bi[..., 2] = 0.01 * np.exp(-(x - n * h / 2) ** 2 / (2 * (n * h / 8) ** 2))[None, :]
+    Bi[..., 3] = np.abs(Bi[..., 2])
+    dom = O.Domain(n1=n, n2=n, h1=h, h2=h, dt=W["dt"], bcnd=bc, Bi=Bi, nB=(n, n), hB=(h, h))
+    Ei = np.random.default_rng(4).normal(0, 1e3, (n + 3, n + 3, 2))
+    chem = W["chem"]
+    tab = O.ordering([dict(col_info=r.col_info(), Eth=r.Eth, E=r.E, S=r.S) for r in chem.reactions],
+                     chem.masses(), chem.ntype_all)
+    setup = dict(ntype=2, ntype_all=3, ns_coll=10, plt_src=0, n_cell=n * n, n_rg=1, ixl_rg=[1],
+                 ixr_rg=[n + 1], iyl_rg=1, iyr_rg=n + 1, flag_ionrep=0,
+                 charge={1: -L.QE, 2: L.QE, 3: 0.0}, mass=chem.masses(),
+                 Nm={1: W["Nm"], 2: W["Nm"], 3: W["Nm"]},
+                 vt0={1: W["vte"], 2: W["vti"], 3: W["vtg"]},
+                 np_mx={1: 0.0, 2: 0.0, 3: W["ngas"]}, nu_uplim={1: 6e7, 2: 5e6, 3: 5e6})
+    t_push = t_mcc = 0.0
+    reps = 0
+    start = time.perf_counter()
+    while time.perf_counter() - start < budget_s or reps < 1:
+        e = np.column_stack([pos, ve])
+        i = np.column_stack([pos, vi])
+        t_push += O.time_push_deposit(dom, -L.QE, L.ME, W["Nm"], Ei, e)
+        t_push += O.time_push_deposit(dom, L.QE, 39.95 * L.AMU, W["Nm"], Ei, i)
+        cap = N + N // 10
+        pe = np.zeros((cap, 6)); pe[:N] = e
+        pi_ = np.zeros((cap, 6)); pi_[:N] = i
+        parts = {1: dict(vxp=pe, np_tot=N, flag_dead=np.zeros(cap, np.int8)),
+                 2: dict(vxp=pi_, np_tot=N, flag_dead=np.zeros(cap, np.int8))}
+        np_red = [np.full((n + 3, n + 3), 1e16), np.full((n + 3, n + 3), 1e16)]
+        t = time.perf_counter()
+        O.collisions(dom, setup, tab, parts, np_red, 123456)
+        t_mcc += time.perf_counter() - t
+        reps += 1
+    per_step = (t_push + t_mcc / 10.0) / reps     # MCC runs every 10 steps
+    return dict(value=2 * N / per_step, unit="particles/s", cores=1, kind="port",
+                sample=f"oracle C restatement, 1 core: {n}x{n} cells x 100 ppc x 2 species "
+                       f"({2 * N} particles), push+deposit each step + MCC/10, {reps} reps")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
+              "split is a design only)", file=sys.stderr)
+        sys.exit(2)
+    hip = Hip()
+    W = workload(args.n)
+    N = 2 * args.n * args.n * W["ppc"]
+    K = args.steps
+    print(f"[bench] mixed precision, {N} particles, {args.warmup}+{K} steps", file=sys.stderr, flush=True)
+    mx, wall_mx, cnt_mx = run_mode(W, L.MIXED, K, args.warmup, hip)
+    fp = None
+    if not args.skip_fp64:
+        print("[bench] FP64", file=sys.stderr, flush=True)
+        fp, wall_fp, cnt_fp = run_mode(W, L.FP64, K, args.warmup, hip)
+
+    def summary(a, wall, cnt):
+        pdm = (a["push"] + a["mcc"]) / K            # push + deposit + MCC per step (ms)
+        step = (a["field"] + a["sort"] + a["mcc"] + a["push"]) / K
+        return dict(ms_per_step=step, ms_push_deposit=a["push"] / K, ms_mcc=a["mcc"] / K,
+                    ms_sort=a["sort"] / K, ms_field=a["field"] / K, vcycles_per_step=a["vcycles"] / K,
+                    particles_per_s=sum(cnt) / (pdm * 1e-3), wall_s=wall, particles=sum(cnt))
+
+    smx = summary(mx, wall_mx, cnt_mx)
+    # roofline of the dominant kernel (the electron mover, mixed precision):
+    # algorithmic bytes = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities,
+    # read and written) per particle and launch
+    bpp = 48.0
+    ke = np.array(mx["push_kernel_ms"][0])
+    alg = cnt_mx[0] * bpp
+    achieved = alg / (ke.mean() * 1e-3) / 1e9
+    cpu = None if args.skip_cpu else cpu_baseline(W, args.cpu_budget)
+    out = {
+        "metric": "ms/step & particles/s (push+deposit+MCC), mixed+learned vs FP64 GPU",
+        "value": smx["particles_per_s"],
+        "unit": "particles/s",
+        "n_gpus": 1, "steps": K, "warmup": args.warmup,
+        "ms_per_step": smx["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.n}x{args.n} cells, e-/Ar+ 100 ppc each ({N} particles), "
+                               "B filter 10 mT, Ar 0.3 Pa analytic MCC, dt 5e-12, MG 1e-10, "
+                               "mixed precision (learned warm start not in this build)",
+                   "global_batch": N, "seq_len": 1, "parallelism": "single GPU"},
+        "mixed": smx,
+        "fp64": summary(fp, wall_fp, cnt_fp) if fp else None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_mover_mixed (electrons)", "alg_bytes_per_launch": alg,
+                     "avg_launch_ms": float(ke.mean())},
+        "cpu_baseline": cpu,
+    }
+    if fp:
+        kf = np.array(fp["push_kernel_ms"][0])
+        out["fp64"]["roofline_push_GBs"] = cnt_fp[0] * 96.0 / (kf.mean() * 1e-3) / 1e9
+        out["speedup_mixed_vs_fp64_step"] = out["fp64"]["ms_per_step"] / smx["ms_per_step"]
+        out["speedup_mixed_vs_fp64_pdm"] = smx["particles_per_s"] / out["fp64"]["particles_per_s"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

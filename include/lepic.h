@@ -1,0 +1,216 @@
+/*
+ * lepic.h -- C ABI of the MI355X (gfx950) implementation of LePIC's
+ * per-timestep particle/field hot path.
+ *
+ * The reference calls its hot-path routines as implicit-interface Fortran
+ * externals (all arguments by reference, arrays by base address, globals in
+ * COMMON blocks).  This library keeps the same routine split, one entry point
+ * per reference routine, but state lives on the GPU inside an opaque context:
+ * the caller uploads particles and static fields once, then calls the phase
+ * entry points each step in the reference's order (main.f90:689-1090), and
+ * downloads only for diagnostics or checks.  Every function returns 0 or a
+ * negative LEPIC_ERR_* code; lepic_last_error() gives the message (the
+ * reference instead stops through stop_calculation, restart.f90:263-279).
+ *
+ * Array layouts are the reference's Fortran layouts (column-major):
+ *   node arrays   A(0:n1+2,0:n2+2)      -> A[iy*(n1+3)+ix]
+ *   rhs(n1+1,n2+1)                       -> rhs[(iy-1)*(n1+1)+(ix-1)]
+ *   Ei(2,0:n1+2,0:n2+2)                  -> Ei[2*node+c]
+ *   Bi(4,0:nB1+2,0:nB2+2)                -> Bi[4*bnode+c]
+ *   vxp(6,nmax) for one species          -> vxp[6*i+c]  (x,y,z,vx,vy,vz)
+ * Species indices are 1-based like ptype.  Not re-entrant per context; one
+ * host thread per context.  No OpenMP inside.
+ */
+#ifndef LEPIC_H
+#define LEPIC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lepic_ctx lepic_ctx;
+
+enum {
+  LEPIC_OK = 0,
+  LEPIC_ERR_ARG = -1,          /* bad argument or size */
+  LEPIC_ERR_UNSUPPORTED = -2,  /* option outside this build's hot path */
+  LEPIC_ERR_HIP = -3,          /* HIP runtime error */
+  LEPIC_ERR_CAPACITY = -4,     /* particle capacity exceeded (nmax, collisions.f90:187-191) */
+  LEPIC_ERR_DIVERGED = -5,     /* MG divergence (iterative_solver.f90:39-51) */
+  LEPIC_ERR_MCC = -6,          /* a collisions.f90 abort (Pmax>1, Ee<0, energy above table) */
+  LEPIC_ERR_STATE = -7         /* call out of order (e.g. no reactions set) */
+};
+
+enum { LEPIC_FP64 = 0, LEPIC_MIXED = 1 };
+enum { LEPIC_MCC_CHECK = 0, LEPIC_MCC_PROD = 1 };
+enum { LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2 };
+enum {
+  LEPIC_GRID_PHI = 0,   /* phi(0:n1+2,0:n2+2) */
+  LEPIC_GRID_RHS = 1,   /* rhs(n1+1,n2+1) */
+  LEPIC_GRID_E = 2,     /* Ei(2,0:n1+2,0:n2+2) */
+  LEPIC_GRID_NP = 3,    /* np(0:n1+2,0:n2+2,ptype): deposited density */
+  LEPIC_GRID_NPRED = 4, /* np_red(0:n1+2,0:n2+2,ptype) */
+  LEPIC_GRID_SINK = 5   /* ss2D(2,:,:,ptype): wall-loss sink counts */
+};
+
+typedef struct {
+  int32_t n1, n2;         /* cells (n(1), n(2)) */
+  double h1, h2;          /* cell size h(1), h(2) in m */
+  double zmax;            /* third-dimension length (1 m when hz<0) */
+  double dt;              /* time step (s) */
+  double eps0;            /* k_eps0 * eps0 (main.f90:161) */
+  double qe;              /* elementary charge as the reference holds it */
+  int32_t ng;             /* MG levels; 1 = red-black SOR only */
+  int32_t ngrid;          /* number of wall labels (p_mac(:,:,0:ngrid)) */
+  int32_t nspecies;       /* charged species carried as particles (ntype) */
+  int32_t precision;      /* LEPIC_FP64 or LEPIC_MIXED */
+  int32_t flag_B_pos;     /* positive ions not magnetised (part_expmover.f90:185) */
+  int32_t tag_neg;        /* tagged negative-ion species (1-based) or 0 */
+  int32_t track_z;        /* mixed precision only: keep the virtual z coordinate */
+  int32_t reserved;
+} lepic_grid_desc;
+
+typedef struct {
+  double charge;          /* C */
+  double mass;            /* kg */
+  double Nm;              /* macroparticle weight */
+  int64_t capacity;       /* maximum particle count (nmax) */
+} lepic_species_desc;
+
+typedef struct {
+  int32_t vcycles;        /* V-cycles done (the reference's niter-1) */
+  int32_t status;         /* 0 converged, 1 hit ncycl */
+  double res;             /* final residual / alpha (iterative_solver.f90:36) */
+  double ktot;            /* SOR-equivalent work (mg.f90:227-229) */
+  double res0;            /* residual / alpha of the starting guess (-1 if not computed) */
+  int32_t start_used;     /* LEPIC_START_* actually used */
+  int32_t nlog;           /* entries written to the sweep log */
+} lepic_solve_info;
+
+typedef struct {
+  int64_t Nc[8];          /* candidates per species (index ptype-1) */
+  double nu_max[8];       /* null collision frequency per species */
+  int64_t events[64];     /* real events per reaction (index icol-1) */
+  int64_t created[8];     /* particles appended per species */
+  int64_t missed;         /* production mode: candidates dropped after claim retries */
+  int32_t seed;           /* check mode: ran2 state after the call */
+  int32_t reserved;
+} lepic_mcc_info;
+
+/* ---- lifecycle ---- */
+/* replaces the allocations of main.f90:192-214 */
+int lepic_create(const lepic_grid_desc *g, const lepic_species_desc *sp, lepic_ctx **out);
+int lepic_destroy(lepic_ctx *ctx);
+const char *lepic_last_error(const lepic_ctx *ctx);
+int lepic_sync(lepic_ctx *ctx);
+/* HIP device index of the context and its stream handle (hipStream_t) */
+int lepic_stream(lepic_ctx *ctx, void **stream);
+
+/* ---- static inputs ---- */
+/* bcnd and the Dirichlet values of generate_boundary.f90 (phi at walls). */
+int lepic_set_boundary(lepic_ctx *ctx, const int32_t *bcnd, const double *phi);
+/* Bi(4,0:nB1+2,0:nB2+2) with |B| in component 4 (main.f90:361-422); nB1 == 1
+ * means a constant field Bi(1:3,1,1). */
+int lepic_set_bfield(lepic_ctx *ctx, int32_t nB1, int32_t nB2, double hB1, double hB2,
+                     const double *Bi);
+
+/* ---- particle transfer (reference vxp(6,N) layout) ---- */
+int lepic_upload_particles(lepic_ctx *ctx, int32_t ptype, const double *vxp, int64_t np);
+int lepic_download_particles(lepic_ctx *ctx, int32_t ptype, double *vxp, int64_t cap,
+                             int64_t *np);
+int lepic_get_count(lepic_ctx *ctx, int32_t ptype, int64_t *np);
+/* flag_dead(nmax,ptype) as int8, for the first np particles */
+int lepic_set_flag_dead(lepic_ctx *ctx, int32_t ptype, const int8_t *flags, int64_t np);
+int lepic_get_flag_dead(lepic_ctx *ctx, int32_t ptype, int8_t *flags, int64_t np);
+
+/* ---- grid transfer ---- */
+int lepic_get_grid(lepic_ctx *ctx, int32_t which, int32_t ptype, double *out);
+int lepic_set_grid(lepic_ctx *ctx, int32_t which, int32_t ptype, const double *in);
+
+/* ---- phases ---- */
+/* part_mover (part_expmover.f90:1-517): Boris/leap-frog push, wall loss,
+ * order-preserving compaction, removal of particles flagged dead; with
+ * deposit=1 the survivors' charge is also deposited (charge_deposition,
+ * part_expmover.f90:520-610) into a zeroed np(:,:,ptype). */
+int lepic_part_mover(lepic_ctx *ctx, int32_t ptype, int32_t deposit);
+/* charge_deposition (part_expmover.f90:520-610): zero np(:,:,ptype), deposit */
+int lepic_charge_deposition(lepic_ctx *ctx, int32_t ptype);
+/* mover tallies since the last reset: P_loss(1:4), p_mac(1:2,0:ngrid), cnt_dead */
+int lepic_get_mover_tally(lepic_ctx *ctx, int32_t ptype, double *P_loss, double *p_mac,
+                          double *cnt_dead);
+int lepic_reset_tallies(lepic_ctx *ctx);
+/* calc_rhs (calc_rho.f90:37-77): rhs = -sum_ptype charge*np */
+int lepic_calc_rhs(lepic_ctx *ctx);
+/* calc_Efield (Efield.f90:1-128) */
+int lepic_calc_efield(lepic_ctx *ctx);
+/* iterative_solver (iterative_solver.f90:1-69) with mg/sor_rb; start selects
+ * the initial phi (previous, zero, or the grid set by lepic_set_grid).
+ * sweep_log (optional, cap entries) receives ksor per level visit. */
+int lepic_iterative_solver(lepic_ctx *ctx, double eps, int32_t ncycl, int32_t start,
+                           lepic_solve_info *info, int32_t *sweep_log, int32_t log_cap);
+/* part_sorting (sorting.f90:92-251): stable counting sort by cell; Plist
+ * (optional, n1+n1*n2+2 int32) receives the cell-end list. */
+int lepic_part_sorting(lepic_ctx *ctx, int32_t ptype, int32_t *Plist);
+/* dens_red (calc_rho.f90:134-199): np_red and np_mx per charged species */
+int lepic_dens_red(lepic_ctx *ctx, double *np_mx);
+
+/* ---- collisions ---- */
+/* Cross-section tables already built on the union grid (read_reactions.f90:
+ * 511-697): sig_Er[npt], sig[ncol*npt] (reaction-major), col_info[ncol*10],
+ * Eth[ncol], sigv_mx[(ntype_all+1)*ncol] (row ptype), p_ncol[ntype_all+1],
+ * sig_list[(ntype_all+1)*ncol] (1-based reaction ids), and per species
+ * (1..ntype_all, index 0 unused): charge, mass (neutrals negative), Nm, vt0,
+ * np_mx (neutral densities; charged entries come from lepic_dens_red) and
+ * nu_uplim.  Regions: n_rg boxes [ixl,ixr] x [iyl,iyr] for the rate tallies. */
+typedef struct {
+  int32_t ntype_all, ncol, npt, ns_coll, n_cell, n_rg, iyl_rg, iyr_rg;
+  int32_t ixl_rg[5], ixr_rg[5];
+  int32_t flag_ionrep, reserved;
+  const double *sig_Er, *sig, *Eth, *sigv_mx;
+  const int32_t *col_info, *p_ncol, *sig_list;
+  const double *charge, *mass, *Nm, *vt0, *np_mx, *nu_uplim;
+} lepic_reactions;
+int lepic_set_reactions(lepic_ctx *ctx, const lepic_reactions *r);
+/* collisions (collisions.f90:1-207, collision_OMP :209-1040).  mode
+ * LEPIC_MCC_CHECK replays the reference's ran2 stream from *iseed in order
+ * (one GPU lane; identical events); LEPIC_MCC_PROD draws from a counter-based
+ * generator keyed by (seed, it) with one lane per candidate. */
+int lepic_collisions(lepic_ctx *ctx, int32_t it, int32_t mode, uint64_t seed, int32_t *iseed,
+                     lepic_mcc_info *info);
+/* cnt_col(3,ncol,n_rg) tallies since the last reset (collisions.f90:535-543, 1026-1033) */
+int lepic_get_cnt_col(lepic_ctx *ctx, double *cnt_col);
+/* P_loss(3,ptype) energy change of collisions since the last reset */
+int lepic_get_mcc_ploss(lepic_ctx *ctx, double *P_loss3);
+
+/* ---- synthetic input (load_part.f90:59-224 restated for a GPU) ----
+ * Fills [xl,xr) x [0,ymax) of a species with ppc particles per cell
+ * uniformly, velocities Maxwellian with thermal speed vt (vt0 of
+ * main.f90:468), from a counter-based generator keyed by seed; cells whose
+ * four corners are walls are skipped, like load_part.f90:115-116.  Appends. */
+int lepic_load_uniform(lepic_ctx *ctx, int32_t ptype, int32_t ppc, double vt, double xl,
+                       double xr, uint64_t seed);
+
+/* ---- one whole step (main.f90:689-1090 order) ----
+ * rhs -> Poisson -> E -> [sort if it%nsort==1] -> [dens_red+collisions if
+ * it>1 and it%ns_coll==1] -> push+deposit of every species. */
+typedef struct {
+  double eps;             /* Poisson tolerance */
+  int32_t ncycl;          /* V-cycle limit (20000 in main.f90:261) */
+  int32_t nsort;          /* 10 */
+  int32_t ns_coll;        /* 10 */
+  int32_t mcc_mode;       /* LEPIC_MCC_* or -1 for no collisions */
+  uint64_t seed;
+  int32_t iseed;          /* check-mode ran2 state (updated) */
+  int32_t start;          /* Poisson start */
+  /* outputs */
+  lepic_solve_info solve;
+  double ms_rhs_poisson_e, ms_sort, ms_mcc, ms_push;   /* phase times (HIP events) */
+} lepic_step_args;
+int lepic_step(lepic_ctx *ctx, int32_t it, lepic_step_args *a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LEPIC_H */

@@ -1,0 +1,259 @@
+// lepic_internal.h -- context state and device helpers shared by the HIP
+// translation units of liblepic_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/lepic.h"
+
+namespace lepic {
+
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxLevels = 12;
+constexpr int kMaxReac = 64;
+constexpr int kMaxTypes = 32;
+constexpr int kBlock = 256;          // threads per block of the particle kernels
+constexpr int kItems = 4;            // particles per thread in the mover
+constexpr int kChunk = kBlock * kItems;
+
+#define LEPIC_HIP(call)                                                              \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return ctx->fail(LEPIC_ERR_HIP, std::string(#call) + ": " + \
+                                           hipGetErrorString(e_));                   \
+  } while (0)
+
+// One MG level; level 1 aliases phi/rhs.
+struct Level {
+  int n1 = 0, n2 = 0;
+  double h1 = 0, h2 = 0;
+  double aw = 0, an = 0, ac = 0;     // mg.h coefficients (ae=aw, as=an)
+  double* u = nullptr;               // (n1+3)(n2+3)
+  double* b = nullptr;               // (n1+1)(n2+1)
+  float* uf = nullptr;               // FP32 copies for mixed smoothing
+  float* bf = nullptr;
+  int8_t* bc = nullptr;              // (n1+3)(n2+3)
+};
+
+// Per-level smoothing control kept on the device (one V-cycle visit).
+struct LevelCtrl {
+  double res;        // L1 residual of the last sweep
+  double res_s;      // previous sweep's residual
+  int32_t ksor;      // sweeps done in this visit
+  int32_t done;      // 1: stop sweeping
+  int32_t iter;      // sweeps of this level in this V-cycle (mg.f90:iter)
+  uint32_t arrive;   // last-block counter of the residual reduction
+};
+
+struct Species {
+  double charge = 0, mass = 0, Nm = 0;
+  int64_t cap = 0;
+  int64_t ub = 0;                    // host-side upper bound of the count
+  int cur = 0;                       // current particle buffer
+  void* mem[2] = {nullptr, nullptr};
+  double* d[2][6] = {};              // FP64: x y z vx vy vz
+  int32_t* cell[2] = {};             // mixed: (iy<<16)|ix, 1-based cell
+  float* f[2][6] = {};               // mixed: fx fy z vx vy vz
+  int8_t* flag_dead = nullptr;
+  double* np = nullptr;              // deposited density (nodes)
+  double* np_red = nullptr;
+  double* sink = nullptr;            // wall-loss sink counts (nodes)
+};
+
+// Tally block per species on the device.
+struct Tally {
+  double P_loss[4];
+  double cnt_dead;
+  double p_mac[2][65];               // (count, energy) x igrid 0..64
+};
+
+struct Reactions {
+  bool set = false;
+  int ntype_all = 0, ncol = 0, npt = 0, ns_coll = 10, n_cell = 0, n_rg = 0;
+  int iyl_rg = 0, iyr_rg = 0, ixl_rg[5] = {}, ixr_rg[5] = {};
+  int flag_ionrep = 0;
+  std::vector<int32_t> col_info, p_ncol, sig_list;
+  std::vector<double> sigv_mx, Eth, charge, mass, Nm, vt0, np_mx, nu_uplim;
+  double* d_sig_Er = nullptr;        // [npt]
+  double* d_sig = nullptr;           // [ncol*npt]
+  float* d_sig_Er_f = nullptr;
+  float* d_sig_f = nullptr;
+  double* d_cnt_col = nullptr;       // [3*ncol*n_rg]
+};
+
+}  // namespace lepic
+
+struct lepic_ctx {
+  lepic_grid_desc g{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int nsp = 0;
+  lepic::Species sp[lepic::kMaxSpecies];
+
+  // grids
+  int64_t nodes = 0, rhs_n = 0;
+  int8_t* bc = nullptr;              // level-1 bcnd (int8)
+  std::vector<int32_t> bc_host;      // int32 copy
+  double* phi = nullptr;
+  double* phi_given = nullptr;       // guess set through lepic_set_grid(PHI) for START_GIVEN
+  double* rhs = nullptr;
+  double* E = nullptr;               // 2 per node
+  float* Ef = nullptr;               // mixed: 2 per node
+  double* Bi = nullptr;              // 4 per B node
+  float* Bf = nullptr;               // mixed: 4 per B node
+  int nB1 = 1, nB2 = 1, flag_B = 0, flag_gridB = 0;
+  double hB1 = 0, hB2 = 0, Bc[3] = {0, 0, 0};
+  int efield_serial = 0;             // 1: wall extrapolation needs the reference's order
+  bool boundary_set = false;
+
+  // MG
+  int ng = 1;
+  lepic::Level lv[lepic::kMaxLevels + 1];
+  int Lc = 1;                        // first level handled by the single-workgroup kernel
+  lepic::LevelCtrl* d_ctrl = nullptr;   // [kMaxLevels+1]
+  double* d_partials = nullptr;      // residual partial sums
+  int64_t partials_cap = 0;
+  double* d_scalar = nullptr;        // small scratch (res0 etc.)
+  double* h_pinned = nullptr;        // pinned host scratch (64 doubles)
+  double* d_resid64 = nullptr;       // FP64 residual grid for mixed refinement
+  float* d_corr = nullptr;
+
+  // device scalars
+  int64_t* d_counts = nullptr;       // [kMaxSpecies]
+  lepic::Tally* d_tally = nullptr;   // [kMaxSpecies]
+  uint64_t* lb_status = nullptr;     // look-back status words
+  int64_t lb_cap = 0;
+  uint32_t lb_epoch = 0;
+  unsigned long long* d_ticket = nullptr;
+  unsigned long long ticket_base = 0;
+  int32_t* d_error = nullptr;        // device-side error flags
+
+  // sort scratch
+  uint32_t* s_key[2] = {nullptr, nullptr};
+  uint32_t* s_val[2] = {nullptr, nullptr};
+  uint32_t* s_hist = nullptr;
+  int64_t s_cap = 0, s_hist_cap = 0;
+  int32_t* d_plist = nullptr;        // cell-end list (pl_max+1)
+  uint32_t* d_scan_tmp = nullptr;
+  int64_t scan_tmp_cap = 0;
+
+  // collisions
+  lepic::Reactions rx;
+  uint32_t* d_claim = nullptr;       // candidate claim bitmap
+  int64_t claim_cap = 0;
+  void* d_mcc_stage = nullptr;       // staged new particles
+  int64_t stage_cap = 0;
+  int64_t* d_mcc_i64 = nullptr;      // [64] counters
+  double* d_mcc_f64 = nullptr;       // [64] sums: [0..8] P_loss(3,ptype), [40..44] 1/np_tot_rg
+  void* d_mcc_tab = nullptr;         // device copy of the reaction table
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+};
+
+namespace lepic {
+
+// ---- host helpers implemented in lepic_ctx.hip ----
+int ensure_lookback(lepic_ctx* ctx, int64_t nblocks);
+int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n);   // in place
+int zero_async(lepic_ctx* ctx, void* p, size_t bytes);
+
+// ---- phase entry points implemented per translation unit ----
+int mover_launch(lepic_ctx* ctx, int ptype, int deposit);
+int deposit_launch(lepic_ctx* ctx, int ptype);
+int rhs_launch(lepic_ctx* ctx);
+int efield_launch(lepic_ctx* ctx);
+int dens_red_launch(lepic_ctx* ctx, double* np_mx_host);
+int poisson_setup(lepic_ctx* ctx);
+int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_info* info,
+                  int32_t* log, int32_t log_cap);
+int sort_launch(lepic_ctx* ctx, int ptype, int32_t* Plist_host);
+int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed,
+            lepic_mcc_info* info);
+int load_uniform_launch(lepic_ctx* ctx, int ptype, int ppc, double vt, double xl, double xr,
+                        uint64_t seed);
+
+// ---- device helpers ----
+__device__ __forceinline__ int64_t node_index(int ix, int iy, int n1) {
+  return (int64_t)iy * (n1 + 3) + ix;
+}
+
+// Decoupled look-back status word: epoch (24 bits) | flag (2 bits) | value (38 bits).
+__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint64_t v) {
+  return ((uint64_t)(epoch & 0xFFFFFFu) << 40) | ((uint64_t)flag << 38) | (v & ((1ull << 38) - 1));
+}
+
+// Exclusive prefix of block `bid` given its aggregate; thread-0 only.
+__device__ inline int64_t lookback_exclusive(uint64_t* status, int64_t bid, uint64_t aggregate,
+                                             uint32_t epoch, int32_t* err) {
+  if (bid == 0) {
+    __hip_atomic_store(&status[0], lb_pack(epoch, 2, aggregate), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    return 0;
+  }
+  __hip_atomic_store(&status[bid], lb_pack(epoch, 1, aggregate), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  uint64_t excl = 0;
+  int64_t j = bid - 1;
+  uint32_t spins = 0;
+  while (j >= 0) {
+    uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t ep = (uint32_t)(s >> 40);
+    uint32_t fl = (uint32_t)(s >> 38) & 3u;
+    if (ep != (epoch & 0xFFFFFFu) || fl == 0) {
+      if (++spins > (1u << 22)) {  // bounded wait: report and give up
+        atomicOr(err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += s & ((1ull << 38) - 1);
+    if (fl == 2) break;
+    --j;
+  }
+  __hip_atomic_store(&status[bid], lb_pack(epoch, 2, excl + aggregate), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  return (int64_t)excl;
+}
+
+__device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+}  // namespace lepic
+
+namespace lepic {
+
+// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11): multipliers 0xD2511F53 and
+// 0xCD9E8D57, Weyl key increments 0x9E3779B9 and 0xBB67AE85, ten rounds.
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline uint32_t mulhi32(uint32_t a, uint32_t b) {
+  return (uint32_t)(((uint64_t)a * (uint64_t)b) >> 32);
+}
+
+__host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = mulhi32(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+    const uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+    c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// 53-bit uniform in [0,1) from two words
+__host__ __device__ inline double u01(uint32_t hi, uint32_t lo) {
+  return (double)((((uint64_t)hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+}  // namespace lepic

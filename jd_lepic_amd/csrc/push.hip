@@ -1,0 +1,656 @@
+// push.hip -- particle mover (part_expmover.f90:1-517) fused with charge
+// deposition (part_expmover.f90:520-610), FP64 and mixed precision.
+//
+// One block handles kChunk consecutive particles (thread t, item k ->
+// particle base + k*256 + t, so loads are coalesced).  Survivors are
+// compacted in particle order (part_expmover.f90:495-503) through a block
+// ballot scan and a decoupled look-back across blocks, writing to the other
+// particle buffer.  Deposition reduces equal-cell runs inside each wave
+// (particles are cell-sorted, so neighbours share a cell), accumulates the
+// runs in an LDS window of 4 rows x 128 columns of nodes around the block's
+// first cell, and flushes the window to the global density with FP64
+// atomics; particles outside the window go straight to global atomics.
+// The FP64 path computes exactly the reference's expressions (no
+// contraction: the library is built with -ffp-contract=off).
+#include <hip/hip_runtime.h>
+
+#include "lepic_internal.h"
+
+using namespace lepic;
+
+namespace {
+
+constexpr int kWinRows = 4;
+constexpr int kWinCols = 128;
+constexpr int kWin = kWinRows * kWinCols;
+
+struct MoverCommon {
+  const int8_t* bc;
+  int n1, n2;
+  double h1, h2, dt, k1;        // k1 = dt*q/(2m)
+  double Nm, mass, NmQ;         // NmQ = Nm*charge for cnt_dead
+  int tagneg;
+  int flagB, gridB, Bconst, magnetize;
+  int nB1, nB2;
+  double hB1, hB2, Bc0, Bc1, Bc2;
+  int deposit;
+  double* np;
+  double k1dep;                 // Nm/(h1*h2*zmax)
+  Tally* tally;
+  double* sink;
+  int ngrid;
+  const int64_t* n_in;
+  int64_t* n_out;
+  uint64_t* status;
+  uint32_t epoch;
+  unsigned long long* ticket;
+  unsigned long long ticket_base;
+  int64_t nblocks;
+  int32_t* err;
+};
+
+struct Soa64 { const double *x, *y, *z, *vx, *vy, *vz; };
+struct Soa64W { double *x, *y, *z, *vx, *vy, *vz; };
+struct SoaMx { const int32_t* cell; const float *fx, *fy, *z, *vx, *vy, *vz; };
+struct SoaMxW { int32_t* cell; float *fx, *fy, *z, *vx, *vy, *vz; };
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << (threadIdx.x & 63)) - 1ull;
+}
+
+// ---- deposition helpers ----------------------------------------------------
+// Wave-level segmented sum of w[0..3] over runs of equal key (adjacent lanes).
+// Returns true on the run head lane, which then holds the run sums.
+template <typename T>
+__device__ __forceinline__ bool run_reduce(int64_t key, T w[4]) {
+  const int lane = threadIdx.x & 63;
+  int64_t nxt = __shfl_down(key, 1, 64);
+  int64_t prv = __shfl_up(key, 1, 64);
+  uint64_t tails = __ballot(lane == 63 || nxt != key);
+  int seg_end = lane + __builtin_ctzll(tails >> lane);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T o0 = __shfl_down(w[0], off, 64), o1 = __shfl_down(w[1], off, 64);
+    T o2 = __shfl_down(w[2], off, 64), o3 = __shfl_down(w[3], off, 64);
+    if (lane + off <= seg_end) { w[0] += o0; w[1] += o1; w[2] += o2; w[3] += o3; }
+  }
+  return lane == 0 || prv != key;
+}
+
+// Deposit the four corner sums of cell (ix,iy) (corners 00,10,11,01 in the
+// order of part_expmover.f90:571-584).  kq = 2 on non-interior nodes
+// (main.f90:334-344) is applied here or at the flush (a factor 2 is exact).
+template <typename T>
+__device__ __forceinline__ void deposit_cell(const MoverCommon& a, T* win, int wx0, int wy0, int ix,
+                                             int iy, const T w[4], double scale) {
+  const int cx = ix - wx0, cy = iy - wy0;
+  if (cx >= 0 && cx + 1 < kWinCols && cy >= 0 && cy + 1 < kWinRows) {
+    atomicAdd(&win[cy * kWinCols + cx], w[0]);
+    atomicAdd(&win[cy * kWinCols + cx + 1], w[1]);
+    atomicAdd(&win[(cy + 1) * kWinCols + cx + 1], w[2]);
+    atomicAdd(&win[(cy + 1) * kWinCols + cx], w[3]);
+  } else {
+    const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
+    const int64_t nd[4] = {n00, n00 + 1, n01 + 1, n01};
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double kq = a.bc[nd[c]] != -1 ? 2.0 : 1.0;
+      atomic_add_f64(&a.np[nd[c]], kq * ((double)w[c] * scale));
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void flush_window(const MoverCommon& a, T* win, int wx0, int wy0,
+                                             double scale) {
+  for (int e = threadIdx.x; e < kWin; e += blockDim.x) {
+    T v = win[e];
+    if (v != (T)0) {
+      int ix = wx0 + e % kWinCols, iy = wy0 + e / kWinCols;
+      int64_t nd = node_index(ix, iy, a.n1);
+      double kq = a.bc[nd] != -1 ? 2.0 : 1.0;
+      atomic_add_f64(&a.np[nd], kq * ((double)v * scale));
+    }
+  }
+}
+
+__device__ __forceinline__ void clamp_cell(int& ix, int& iy, int n1, int n2) {
+  // check_outofbounds, part_expmover.f90:612-627
+  ix = ix < 0 ? 0 : (ix > n1 + 1 ? n1 + 1 : ix);
+  iy = iy < 0 ? 0 : (iy > n2 + 1 ? n2 + 1 : iy);
+}
+
+__device__ __forceinline__ bool all_walls(const int8_t* bc, int ix, int iy, int n1) {
+  const int64_t a = node_index(ix, iy, n1), b = node_index(ix, iy + 1, n1);
+  return bc[a] >= 1 && bc[a + 1] >= 1 && bc[b + 1] >= 1 && bc[b] >= 1;
+}
+
+// Lost-particle tallies, part_expmover.f90:314-332, 424-427.
+__device__ inline void tally_lost(const MoverCommon& a, double xp, double yp, int ix, int iy,
+                                  double Eki) {
+  Tally* t = a.tally;
+  atomic_add_f64(&t->P_loss[0], Eki);
+  int ixw = (int)llround(xp / a.h1) + 1, iyw = (int)llround(yp / a.h2) + 1;
+  clamp_cell(ixw, iyw, a.n1, a.n2);
+  atomic_add_f64(&a.sink[node_index(ixw, iyw, a.n1)], 1.0);
+  int igrid = a.bc[node_index(ix, iy, a.n1)];
+  atomic_add_f64(&t->p_mac[0][igrid], 1.0);
+  atomic_add_f64(&t->p_mac[1][igrid], Eki);
+}
+
+__device__ inline void tally_dead(const MoverCommon& a, double Eki) {
+  atomic_add_f64(&a.tally->P_loss[2], -Eki);
+  if (a.tagneg) atomic_add_f64(&a.tally->cnt_dead, a.NmQ);
+}
+
+// Block-wide order-preserving offsets of survivors; returns the global base
+// through *gbase and each item's local offset in off[].
+__device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const bool surv[kItems],
+                                       uint32_t off[kItems], int64_t* gbase) {
+  __shared__ uint32_t s_cnt[kItems][kBlock / 64];
+  __shared__ int64_t s_base;
+  const int w = threadIdx.x >> 6;
+  uint64_t mask[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    mask[k] = __ballot(surv[k]);
+    if ((threadIdx.x & 63) == 0) s_cnt[k][w] = (uint32_t)__popcll(mask[k]);
+  }
+  __syncthreads();
+  uint32_t run = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    uint32_t before = run;
+    for (int v = 0; v < kBlock / 64; ++v) {
+      if (v < w) before += s_cnt[k][v];
+      run += s_cnt[k][v];
+    }
+    off[k] = before + (uint32_t)__popcll(mask[k] & lanemask_lt());
+  }
+  if (threadIdx.x == 0) {
+    int64_t excl = bid < a.nblocks ? lookback_exclusive(a.status, bid, run, a.epoch, a.err) : 0;
+    s_base = excl;
+    if (bid == a.nblocks - 1) *a.n_out = excl + run;
+  }
+  __syncthreads();
+  *gbase = s_base;
+}
+
+// ---------------------------------------------------------------------------
+// FP64 mover
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_mover64(MoverCommon a, Soa64 in, Soa64W out,
+                                                    int8_t* flag_dead, const double* __restrict__ E,
+                                                    const double* __restrict__ B) {
+  __shared__ int64_t s_bid;
+  __shared__ double win[kWin];
+  __shared__ int s_wx0, s_wy0;
+  if (threadIdx.x == 0) s_bid = (int64_t)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
+  __syncthreads();
+  const int64_t bid = s_bid;
+  const int64_t N = *a.n_in;
+  const int64_t base = bid * kChunk;
+  const int t = threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2;
+  const double h1 = a.h1, h2 = a.h2, dt = a.dt, k1 = a.k1;
+
+  if (a.deposit) {
+    for (int e = t; e < kWin; e += kBlock) win[e] = 0.0;
+    if (t == 0) {
+      int ix0 = 1, iy0 = 1;
+      if (base < N) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
+      s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
+      s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+    }
+  }
+
+  double X[kItems], Y[kItems], Z[kItems], VX[kItems], VY[kItems], VZ[kItems];
+  bool surv[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + t;
+    surv[k] = false;
+    if (i >= N) continue;
+    double xp = in.x[i], yp = in.y[i], zp = in.z[i];
+    double vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+    if (flag_dead[i] == 1) {  // part_expmover.f90:84-98
+      flag_dead[i] = 0;
+      tally_dead(a, 0.5 * a.Nm * a.mass * (vx * vx + vy * vy + vz * vz));
+      continue;
+    }
+    int ix = (int)(xp / h1) + 1;  // INT(), lines 103-104
+    int iy = (int)(yp / h2) + 1;
+    clamp_cell(ix, iy, n1, n2);   // identity for positions inside the box
+    double px = (ix * h1 - xp) / h1;
+    double py = (iy * h2 - yp) / h2;
+    double k0 = px * py, kk1 = (1.0 - px) * py, kk2 = (1.0 - px) * (1.0 - py), kk3 = px * (1.0 - py);
+    const int64_t n00 = node_index(ix, iy, n1), n01 = node_index(ix, iy + 1, n1);
+    const double2 e00 = *(const double2*)(E + 2 * n00), e10 = *(const double2*)(E + 2 * (n00 + 1));
+    const double2 e11 = *(const double2*)(E + 2 * (n01 + 1)), e01 = *(const double2*)(E + 2 * n01);
+    double Exp = k0 * e00.x + kk1 * e10.x + kk2 * e11.x + kk3 * e01.x;
+    double Eyp = k0 * e00.y + kk1 * e10.y + kk2 * e11.y + kk3 * e01.y;
+    bool boris = false;
+    double Bx = 0, By = 0, Bz = 0;
+    if (a.flagB) {  // lines 131-182
+      if (a.Bconst) {
+        Bx = a.Bc0; By = a.Bc1; Bz = a.Bc2;
+        boris = true;
+      } else {
+        int bx = ix, by = iy;
+        if (a.gridB) {
+          bx = (int)(xp / a.hB1) + 1; by = (int)(yp / a.hB2) + 1;
+          bx = bx < 0 ? 0 : (bx > a.nB1 + 1 ? a.nB1 + 1 : bx);
+          by = by < 0 ? 0 : (by > a.nB2 + 1 ? a.nB2 + 1 : by);
+        }
+        const int64_t bs = a.nB1 + 3;
+        const double* B00 = B + 4 * ((int64_t)by * bs + bx);
+        if (B00[3] > 1.0e-5) {
+          if (a.gridB) {
+            px = (bx * a.hB1 - xp) / a.hB1;
+            py = (by * a.hB2 - yp) / a.hB2;
+            k0 = px * py; kk1 = (1.0 - px) * py; kk2 = (1.0 - px) * (1.0 - py); kk3 = px * (1.0 - py);
+          }
+          const double* B10 = B00 + 4;
+          const double* B01 = B + 4 * ((int64_t)(by + 1) * bs + bx);
+          const double* B11 = B01 + 4;
+          Bx = k0 * B00[0] + kk1 * B10[0] + kk2 * B11[0] + kk3 * B01[0];
+          By = k0 * B00[1] + kk1 * B10[1] + kk2 * B11[1] + kk3 * B01[1];
+          Bz = k0 * B00[2] + kk1 * B10[2] + kk2 * B11[2] + kk3 * B01[2];
+          boris = true;
+        }
+      }
+    }
+    if (!a.magnetize) boris = false;  // line 185
+    if (boris) {                       // lines 187-206
+      double Bt = sqrt(Bx * Bx + By * By + Bz * Bz);
+      double k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+      double v1x = vx + k1 * Exp, v1y = vy + k1 * Eyp, v1z = vz;
+      double v3x = v1x + k1 * (v1y * Bz - v1z * By);
+      double v3y = v1y + k1 * (v1z * Bx - v1x * Bz);
+      double v3z = v1z + k1 * (v1x * By - v1y * Bx);
+      double v2x = v1x + k2 * (v3y * Bz - v3z * By);
+      double v2y = v1y + k2 * (v3z * Bx - v3x * Bz);
+      double v2z = v1z + k2 * (v3x * By - v3y * Bx);
+      vx = v2x + k1 * Exp;
+      vy = v2y + k1 * Eyp;
+      vz = v2z;
+    } else {  // lines 210-211
+      vx = vx + 2.0 * k1 * Exp;
+      vy = vy + 2.0 * k1 * Eyp;
+    }
+    xp = xp + dt * vx;  // lines 216-219
+    yp = yp + dt * vy;
+    zp = zp + dt * vz;
+    ix = (int)floor(xp / h1) + 1;  // FLOOR(), lines 244-245
+    iy = (int)floor(yp / h2) + 1;
+    clamp_cell(ix, iy, n1, n2);
+    if (all_walls(a.bc, ix, iy, n1)) {  // lines 254-258, 314-446
+      tally_lost(a, xp, yp, ix, iy, 0.5 * a.Nm * a.mass * (vx * vx + vy * vy + vz * vz));
+      continue;
+    }
+    X[k] = xp; Y[k] = yp; Z[k] = zp; VX[k] = vx; VY[k] = vy; VZ[k] = vz;
+    surv[k] = true;
+  }
+
+  uint32_t off[kItems];
+  int64_t gbase;
+  compact_offsets(a, bid, surv, off, &gbase);  // contains __syncthreads
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (!surv[k]) continue;
+    const int64_t o = gbase + off[k];
+    out.x[o] = X[k]; out.y[o] = Y[k]; out.z[o] = Z[k];
+    out.vx[o] = VX[k]; out.vy[o] = VY[k]; out.vz[o] = VZ[k];
+  }
+  if (!a.deposit) return;
+  const int wx0 = s_wx0, wy0 = s_wy0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    // charge_deposition, part_expmover.f90:555-584, at the new position
+    int64_t key = -2 - (int64_t)t;
+    double w[4] = {0, 0, 0, 0};
+    int ix = 0, iy = 0;
+    if (surv[k]) {
+      ix = (int)(X[k] / h1) + 1;
+      iy = (int)(Y[k] / h2) + 1;
+      double px = (ix * h1 - X[k]) / h1;
+      double py = (iy * h2 - Y[k]) / h2;
+      w[0] = a.k1dep * px * py;
+      w[1] = a.k1dep * (1.0 - px) * py;
+      w[2] = a.k1dep * (1.0 - px) * (1.0 - py);
+      w[3] = a.k1dep * px * (1.0 - py);
+      key = node_index(ix, iy, n1);
+    }
+    bool head = run_reduce(key, w);
+    if (head && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+  }
+  __syncthreads();
+  flush_window(a, win, wx0, wy0, 1.0);
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision mover: int32 cell + FP32 in-cell offset, FP32 velocity,
+// FP32 E/B gathers; deposition of FP32 weights into an FP32 LDS window,
+// flushed to the FP64 density with the FP64 factor Nm/(h1 h2 zmax).
+// ---------------------------------------------------------------------------
+struct MixedParams {
+  float k1f, dtoh1, dtoh2, dtf;
+  float hB1r, hB2r;   // h/hB ratios for a B map on another grid
+};
+
+__global__ __launch_bounds__(kBlock) void k_mover_mixed(MoverCommon a, MixedParams m, SoaMx in,
+                                                        SoaMxW out, int8_t* flag_dead,
+                                                        const float* __restrict__ E,
+                                                        const float* __restrict__ B) {
+  __shared__ int64_t s_bid;
+  __shared__ float win[kWin];
+  __shared__ int s_wx0, s_wy0;
+  if (threadIdx.x == 0) s_bid = (int64_t)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
+  __syncthreads();
+  const int64_t bid = s_bid;
+  const int64_t N = *a.n_in;
+  const int64_t base = bid * kChunk;
+  const int t = threadIdx.x;
+  const int n1 = a.n1, n2 = a.n2;
+  const float k1 = m.k1f;
+  if (a.deposit) {
+    for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
+    if (t == 0) {
+      int c0 = base < N ? in.cell[base] : ((1 << 16) | 1);
+      int ix0 = c0 & 0xFFFF, iy0 = c0 >> 16;
+      s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
+      s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+    }
+  }
+  int C[kItems];
+  float FX[kItems], FY[kItems], Z[kItems], VX[kItems], VY[kItems], VZ[kItems];
+  bool surv[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + t;
+    surv[k] = false;
+    if (i >= N) continue;
+    int c = in.cell[i];
+    float fx = in.fx[i], fy = in.fy[i];
+    float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
+    float zp = in.z ? in.z[i] : 0.0f;
+    if (flag_dead[i] == 1) {
+      flag_dead[i] = 0;
+      tally_dead(a, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
+      continue;
+    }
+    int ix = c & 0xFFFF, iy = c >> 16;
+    clamp_cell(ix, iy, n1, n2);
+    const float px = 1.0f - fx, py = 1.0f - fy;
+    const float w0 = px * py, w1 = fx * py, w2 = fx * fy, w3 = px * fy;
+    const int64_t n00 = node_index(ix, iy, n1), n01 = node_index(ix, iy + 1, n1);
+    const float2 e00 = *(const float2*)(E + 2 * n00), e10 = *(const float2*)(E + 2 * (n00 + 1));
+    const float2 e11 = *(const float2*)(E + 2 * (n01 + 1)), e01 = *(const float2*)(E + 2 * n01);
+    float Ex = fmaf(w3, e01.x, fmaf(w2, e11.x, fmaf(w1, e10.x, w0 * e00.x)));
+    float Ey = fmaf(w3, e01.y, fmaf(w2, e11.y, fmaf(w1, e10.y, w0 * e00.y)));
+    bool boris = false;
+    float Bx = 0.f, By = 0.f, Bz = 0.f;
+    if (a.flagB) {
+      if (a.Bconst) {
+        Bx = (float)a.Bc0; By = (float)a.Bc1; Bz = (float)a.Bc2;
+        boris = true;
+      } else {
+        int bx = ix, by = iy;
+        float b0 = w0, b1 = w1, b2 = w2, b3 = w3;
+        if (a.gridB) {
+          float gx = ((float)(ix - 1) + fx) * m.hB1r, gy = ((float)(iy - 1) + fy) * m.hB2r;
+          float fbx = floorf(gx), fby = floorf(gy);
+          bx = (int)fbx + 1; by = (int)fby + 1;
+          bx = bx < 0 ? 0 : (bx > a.nB1 + 1 ? a.nB1 + 1 : bx);
+          by = by < 0 ? 0 : (by > a.nB2 + 1 ? a.nB2 + 1 : by);
+          float ox = gx - fbx, oy = gy - fby, qx = 1.0f - ox, qy = 1.0f - oy;
+          b0 = qx * qy; b1 = ox * qy; b2 = ox * oy; b3 = qx * oy;
+        }
+        const int64_t bs = a.nB1 + 3;
+        const float4 B00 = *(const float4*)(B + 4 * ((int64_t)by * bs + bx));
+        if (B00.w > 1.0e-5f) {
+          const float4 B10 = *(const float4*)(B + 4 * ((int64_t)by * bs + bx + 1));
+          const float4 B01 = *(const float4*)(B + 4 * ((int64_t)(by + 1) * bs + bx));
+          const float4 B11 = *(const float4*)(B + 4 * ((int64_t)(by + 1) * bs + bx + 1));
+          Bx = fmaf(b3, B01.x, fmaf(b2, B11.x, fmaf(b1, B10.x, b0 * B00.x)));
+          By = fmaf(b3, B01.y, fmaf(b2, B11.y, fmaf(b1, B10.y, b0 * B00.y)));
+          Bz = fmaf(b3, B01.z, fmaf(b2, B11.z, fmaf(b1, B10.z, b0 * B00.z)));
+          boris = true;
+        }
+      }
+    }
+    if (!a.magnetize) boris = false;
+    if (boris) {
+      float Bt2 = Bx * Bx + By * By + Bz * Bz;
+      float k2 = k1 * (2.0f / (1.0f + k1 * k1 * Bt2));
+      float v1x = fmaf(k1, Ex, vx), v1y = fmaf(k1, Ey, vy), v1z = vz;
+      float v3x = fmaf(k1, v1y * Bz - v1z * By, v1x);
+      float v3y = fmaf(k1, v1z * Bx - v1x * Bz, v1y);
+      float v3z = fmaf(k1, v1x * By - v1y * Bx, v1z);
+      vx = fmaf(k1, Ex, fmaf(k2, v3y * Bz - v3z * By, v1x));
+      vy = fmaf(k1, Ey, fmaf(k2, v3z * Bx - v3x * Bz, v1y));
+      vz = fmaf(k2, v3x * By - v3y * Bx, v1z);
+    } else {
+      vx = fmaf(2.0f * k1, Ex, vx);
+      vy = fmaf(2.0f * k1, Ey, vy);
+    }
+    fx = fmaf(m.dtoh1, vx, fx);
+    fy = fmaf(m.dtoh2, vy, fy);
+    zp = fmaf(m.dtf, vz, zp);
+    float sx = floorf(fx), sy = floorf(fy);
+    fx -= sx; fy -= sy;
+    if (fx >= 1.0f) { fx -= 1.0f; sx += 1.0f; }
+    if (fy >= 1.0f) { fy -= 1.0f; sy += 1.0f; }
+    // the shift is bounded by the CFL of the run; clamp before int conversion
+    sx = fminf(fmaxf(sx, -4096.f), 4096.f);
+    sy = fminf(fmaxf(sy, -4096.f), 4096.f);
+    ix += (int)sx;
+    iy += (int)sy;
+    int cx = ix, cy = iy;
+    clamp_cell(cx, cy, n1, n2);
+    if (all_walls(a.bc, cx, cy, n1)) {
+      double xd = ((double)(ix - 1) + fx) * a.h1, yd = ((double)(iy - 1) + fy) * a.h2;
+      tally_lost(a, xd, yd, cx, cy, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
+      continue;
+    }
+    C[k] = (iy << 16) | ix;
+    FX[k] = fx; FY[k] = fy; Z[k] = zp; VX[k] = vx; VY[k] = vy; VZ[k] = vz;
+    surv[k] = true;
+  }
+  uint32_t off[kItems];
+  int64_t gbase;
+  compact_offsets(a, bid, surv, off, &gbase);
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (!surv[k]) continue;
+    const int64_t o = gbase + off[k];
+    out.cell[o] = C[k]; out.fx[o] = FX[k]; out.fy[o] = FY[k];
+    if (out.z) out.z[o] = Z[k];
+    out.vx[o] = VX[k]; out.vy[o] = VY[k]; out.vz[o] = VZ[k];
+  }
+  if (!a.deposit) return;
+  const int wx0 = s_wx0, wy0 = s_wy0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    int64_t key = -2 - (int64_t)t;
+    float w[4] = {0.f, 0.f, 0.f, 0.f};
+    int ix = 0, iy = 0;
+    if (surv[k]) {
+      ix = C[k] & 0xFFFF; iy = C[k] >> 16;
+      float px = 1.0f - FX[k], py = 1.0f - FY[k];
+      w[0] = px * py; w[1] = FX[k] * py; w[2] = FX[k] * FY[k]; w[3] = px * FY[k];
+      key = node_index(ix, iy, n1);
+    }
+    bool head = run_reduce(key, w);
+    if (head && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, a.k1dep);
+  }
+  __syncthreads();
+  flush_window(a, win, wx0, wy0, a.k1dep);
+}
+
+// ---------------------------------------------------------------------------
+// Deposition only (charge_deposition without a push)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
+  __shared__ double win[kWin];
+  __shared__ int s_wx0, s_wy0;
+  const int64_t N = *a.n_in;
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+  const int t = threadIdx.x;
+  for (int e = t; e < kWin; e += kBlock) win[e] = 0.0;
+  if (t == 0) {
+    int ix0 = 1, iy0 = 1;
+    if (base < N) { ix0 = (int)(in.x[base] / a.h1) + 1; iy0 = (int)(in.y[base] / a.h2) + 1; }
+    s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
+    s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+  }
+  __syncthreads();
+  const int wx0 = s_wx0, wy0 = s_wy0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + t;
+    int64_t key = -2 - (int64_t)t;
+    double w[4] = {0, 0, 0, 0};
+    int ix = 0, iy = 0;
+    bool ok = i < N;
+    if (ok) {
+      double xp = in.x[i], yp = in.y[i];
+      ix = (int)(xp / a.h1) + 1;
+      iy = (int)(yp / a.h2) + 1;
+      clamp_cell(ix, iy, a.n1, a.n2);
+      double px = (ix * a.h1 - xp) / a.h1;
+      double py = (iy * a.h2 - yp) / a.h2;
+      w[0] = a.k1dep * px * py;
+      w[1] = a.k1dep * (1.0 - px) * py;
+      w[2] = a.k1dep * (1.0 - px) * (1.0 - py);
+      w[3] = a.k1dep * px * (1.0 - py);
+      key = node_index(ix, iy, a.n1);
+    }
+    bool head = run_reduce(key, w);
+    if (head && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+  }
+  __syncthreads();
+  flush_window(a, win, wx0, wy0, 1.0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx in) {
+  __shared__ float win[kWin];
+  __shared__ int s_wx0, s_wy0;
+  const int64_t N = *a.n_in;
+  const int64_t base = (int64_t)blockIdx.x * kChunk;
+  const int t = threadIdx.x;
+  for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
+  if (t == 0) {
+    int c0 = base < N ? in.cell[base] : ((1 << 16) | 1);
+    s_wx0 = (c0 & 0xFFFF) - 2 < 0 ? 0 : (c0 & 0xFFFF) - 2;
+    s_wy0 = (c0 >> 16) - 1 < 0 ? 0 : (c0 >> 16) - 1;
+  }
+  __syncthreads();
+  const int wx0 = s_wx0, wy0 = s_wy0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock + t;
+    int64_t key = -2 - (int64_t)t;
+    float w[4] = {0.f, 0.f, 0.f, 0.f};
+    int ix = 0, iy = 0;
+    bool ok = i < N;
+    if (ok) {
+      int c = in.cell[i];
+      ix = c & 0xFFFF; iy = c >> 16;
+      clamp_cell(ix, iy, a.n1, a.n2);
+      float fx = in.fx[i], fy = in.fy[i], px = 1.0f - fx, py = 1.0f - fy;
+      w[0] = px * py; w[1] = fx * py; w[2] = fx * fy; w[3] = px * fy;
+      key = node_index(ix, iy, a.n1);
+    }
+    bool head = run_reduce(key, w);
+    if (head && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, a.k1dep);
+  }
+  __syncthreads();
+  flush_window(a, win, wx0, wy0, a.k1dep);
+}
+
+MoverCommon common_args(lepic_ctx* ctx, int p) {
+  Species& s = ctx->sp[p];
+  MoverCommon a{};
+  a.bc = ctx->bc;
+  a.n1 = ctx->g.n1; a.n2 = ctx->g.n2;
+  a.h1 = ctx->g.h1; a.h2 = ctx->g.h2; a.dt = ctx->g.dt;
+  a.k1 = ctx->g.dt * s.charge / (2.0 * s.mass);  // part_expmover.f90:61
+  a.Nm = s.Nm; a.mass = s.mass; a.NmQ = s.Nm * s.charge;
+  a.tagneg = (ctx->g.tag_neg == p + 1);
+  a.flagB = ctx->flag_B; a.gridB = ctx->flag_gridB; a.Bconst = (ctx->nB1 == 1);
+  a.magnetize = !(ctx->g.flag_B_pos == 1 && s.charge > 0.0);
+  a.nB1 = ctx->nB1; a.nB2 = ctx->nB2; a.hB1 = ctx->hB1; a.hB2 = ctx->hB2;
+  a.Bc0 = ctx->Bc[0]; a.Bc1 = ctx->Bc[1]; a.Bc2 = ctx->Bc[2];
+  a.np = s.np;
+  a.k1dep = s.Nm / (ctx->g.h1 * ctx->g.h2 * ctx->g.zmax);  // part_expmover.f90:564
+  a.tally = ctx->d_tally + p;
+  a.sink = s.sink;
+  a.ngrid = ctx->g.ngrid;
+  a.n_in = ctx->d_counts + p;
+  a.n_out = ctx->d_counts + p;
+  a.err = ctx->d_error;
+  return a;
+}
+
+}  // namespace
+
+namespace lepic {
+
+int mover_launch(lepic_ctx* ctx, int p, int deposit) {
+  Species& s = ctx->sp[p];
+  MoverCommon a = common_args(ctx, p);
+  a.deposit = deposit ? 1 : 0;
+  const int64_t nb = std::max<int64_t>((s.ub + kChunk - 1) / kChunk, 1);
+  if (int rc = ensure_lookback(ctx, nb)) return rc;
+  a.status = ctx->lb_status;
+  a.epoch = ctx->lb_epoch;
+  if (int rc = zero_async(ctx, ctx->d_ticket, 8)) return rc;
+  a.ticket = ctx->d_ticket;
+  a.ticket_base = 0;
+  a.nblocks = nb;
+  if (deposit)
+    if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
+  const int b = s.cur, o = 1 - s.cur;
+  if (ctx->g.precision == LEPIC_MIXED) {
+    MixedParams m;
+    m.k1f = (float)a.k1;
+    m.dtoh1 = (float)(ctx->g.dt / ctx->g.h1);
+    m.dtoh2 = (float)(ctx->g.dt / ctx->g.h2);
+    m.dtf = (float)ctx->g.dt;
+    m.hB1r = ctx->flag_gridB ? (float)(ctx->g.h1 / ctx->hB1) : 1.0f;
+    m.hB2r = ctx->flag_gridB ? (float)(ctx->g.h2 / ctx->hB2) : 1.0f;
+    SoaMx in{s.cell[b], s.f[b][0], s.f[b][1], s.f[b][2], s.f[b][3], s.f[b][4], s.f[b][5]};
+    SoaMxW out{s.cell[o], s.f[o][0], s.f[o][1], s.f[o][2], s.f[o][3], s.f[o][4], s.f[o][5]};
+    hipLaunchKernelGGL(k_mover_mixed, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, m, in, out,
+                       s.flag_dead, (const float*)ctx->Ef, (const float*)(ctx->Bf ? ctx->Bf : (float*)ctx->Ef));
+  } else {
+    Soa64 in{s.d[b][0], s.d[b][1], s.d[b][2], s.d[b][3], s.d[b][4], s.d[b][5]};
+    Soa64W out{s.d[o][0], s.d[o][1], s.d[o][2], s.d[o][3], s.d[o][4], s.d[o][5]};
+    hipLaunchKernelGGL(k_mover64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, in, out,
+                       s.flag_dead, (const double*)ctx->E, (const double*)ctx->Bi);
+  }
+  LEPIC_HIP(hipGetLastError());
+  s.cur = o;
+  return 0;
+}
+
+int deposit_launch(lepic_ctx* ctx, int p) {
+  Species& s = ctx->sp[p];
+  MoverCommon a = common_args(ctx, p);
+  a.deposit = 1;
+  if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
+  const int64_t nb = std::max<int64_t>((s.ub + kChunk - 1) / kChunk, 1);
+  const int b = s.cur;
+  if (ctx->g.precision == LEPIC_MIXED) {
+    SoaMx in{s.cell[b], s.f[b][0], s.f[b][1], s.f[b][2], s.f[b][3], s.f[b][4], s.f[b][5]};
+    hipLaunchKernelGGL(k_deposit_mixed, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, in);
+  } else {
+    Soa64 in{s.d[b][0], s.d[b][1], s.d[b][2], s.d[b][3], s.d[b][4], s.d[b][5]};
+    hipLaunchKernelGGL(k_deposit64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, in);
+  }
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace lepic

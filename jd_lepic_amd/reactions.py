@@ -155,14 +155,14 @@ def write(chem: Chemistry, ngas: float) -> str:
         out += [title, "name   mass  charge  temp.(eV)  n/n0", "-" * 37]
         for s in group:
             ni0 = s.ni0 / ngas if s.charge == 0.0 else s.ni0
-            out.append(f"{s.name:<8s} {abs(s.mass) / AMU!r} {s.charge / QE!r} {s.T!r} {ni0!r}")
+            out.append(f"{s.name:<8s} {float(abs(s.mass) / AMU)!r} {float(s.charge / QE)!r} {float(s.T)!r} {float(ni0)!r}")
         out += ["-" * 37, ""]
     words = {1: "COLLISION", 2: "IONIZATION", 3: "EXCITATION", 4: "CHARGEEXCHANGE",
              5: "DISSOCIATION"}
     for r in chem.reactions:
-        out += ["REACTION", r.label, f" {r.Eth!r}  {r.dE!r}", " 1.  1.", words[r.rtype],
+        out += ["REACTION", r.label, f" {float(r.Eth)!r}  {float(r.dE)!r}", " 1.  1.", words[r.rtype],
                 "-" * 60]
-        out += [f"  {e!r}  {s!r}" for e, s in zip(r.E, r.S)]
+        out += [f"  {float(e)!r}  {float(s)!r}" for e, s in zip(r.E, r.S)]
         out += ["-" * 60, ""]
     return "\n".join(out) + "\n"
 

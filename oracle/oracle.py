@@ -410,3 +410,31 @@ def collisions(dom: Domain, setup: dict, table: dict, parts: dict, np_red, iseed
     return dict(seed=seed.value, cnt_col=cnt.reshape(3, ncol, max(setup["n_rg"], 1)),
                 P_loss3=np.array(o.P_loss3[:]), Nc=np.array(o.Nc[:]), nu_max=np.array(o.nu_max[:]),
                 events=np.array(o.events[:ncol]), log=log[:2 * o.log_len].reshape(-1, 2).copy())
+
+
+def time_push_deposit(dom: Domain, charge, mass, Nm, Ei, vxp, reps=1):
+    """Seconds spent in lo_part_mover + lo_charge_deposition alone (no
+    wrapper copies), for the CPU baseline; vxp is left unchanged."""
+    import time
+    Ei = np.ascontiguousarray(Ei, dtype=np.float64)
+    src = np.ascontiguousarray(vxp, dtype=np.float64).reshape(-1, 6)
+    work = src.copy()
+    N = src.shape[0]
+    fd = np.zeros(N, np.int8)
+    d = dom.struct()
+    sp = LoSpecies(charge, mass, Nm, 0)
+    k = kq(dom)
+    rho = np.zeros((dom.n2 + 3, dom.n1 + 3))
+    p_mac = np.zeros(2 * (dom.ngrid + 1))
+    ss2D = np.zeros((dom.n2 + 3, dom.n1 + 3))
+    total = 0.0
+    for _ in range(reps):
+        work[:] = src
+        t = LoMoverTally()
+        npt = C.c_longlong(N)
+        t0 = time.perf_counter()
+        lib().lo_part_mover(C.byref(d), C.byref(sp), _p(Ei), _p(work), _p(fd, _bp), C.byref(npt),
+                            C.byref(t), _p(p_mac), _p(ss2D))
+        lib().lo_charge_deposition(C.byref(d), C.byref(sp), _p(work), npt.value, _p(k), _p(rho))
+        total += time.perf_counter() - t0
+    return total / reps
