@@ -51,7 +51,23 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_fortran_driver(objdir)
     return LIB
+
+
+def build_fortran_driver(objdir: str) -> str | None:
+    """The Fortran time loop over the C ABI (iso_c_binding), when amdflang is
+    present; links liblepic_hip.so from the package directory."""
+    flang = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+    if not os.path.exists(flang):
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    exe = os.path.join(fdir, "lepic_driver")
+    cmd = [flang, "-O2", "-module-dir", objdir, os.path.join(fdir, "lepic_capi.f90"),
+           os.path.join(fdir, "lepic_driver.f90"), "-L" + HERE, "-llepic_hip",
+           "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    return exe
 
 
 if __name__ == "__main__":

@@ -84,6 +84,30 @@ module lepic_capi
        real(c_double), value :: vt, xl, xr
        integer(c_int64_t), value :: seed
      end function lepic_load_uniform
+     integer(c_int) function lepic_upload_particles(ctx, ptype, vxp, np) &
+          bind(C, name='lepic_upload_particles')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ptype
+       real(c_double), intent(in) :: vxp(*)
+       integer(c_int64_t), value :: np
+     end function lepic_upload_particles
+     integer(c_int) function lepic_download_particles(ctx, ptype, vxp, cap, np) &
+          bind(C, name='lepic_download_particles')
+       import :: c_int, c_ptr, c_int32_t, c_double, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ptype
+       real(c_double), intent(out) :: vxp(*)
+       integer(c_int64_t), value :: cap
+       integer(c_int64_t), intent(out) :: np
+     end function lepic_download_particles
+     integer(c_int) function lepic_get_mover_tally(ctx, ptype, P_loss, p_mac, cnt_dead) &
+          bind(C, name='lepic_get_mover_tally')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: ptype
+       real(c_double), intent(out) :: P_loss(*), p_mac(*), cnt_dead
+     end function lepic_get_mover_tally
      integer(c_int) function lepic_get_count(ctx, ptype, np) bind(C, name='lepic_get_count')
        import :: c_int, c_ptr, c_int32_t, c_int64_t
        type(c_ptr), value :: ctx
