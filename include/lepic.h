@@ -69,7 +69,8 @@ typedef struct {
   int32_t flag_B_pos;     /* positive ions not magnetised (part_expmover.f90:185) */
   int32_t tag_neg;        /* tagged negative-ion species (1-based) or 0 */
   int32_t track_z;        /* mixed precision only: keep the virtual z coordinate */
-  int32_t reserved;
+  int32_t mg_tail_nodes;  /* MG levels with at most this many nodes ((n1+1)(n2+1)) run in
+                             one single-workgroup kernel; 0 = library default (20000) */
 } lepic_grid_desc;
 
 typedef struct {

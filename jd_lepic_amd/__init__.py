@@ -55,14 +55,14 @@ class Context:
     """GPU-resident state of one simulation (grid, fields, particles)."""
 
     def __init__(self, n1, n2, h1, h2, dt, species, zmax=1.0, eps0=EPS0, ng=1, ngrid=1,
-                 precision=FP64, flag_B_pos=0, tag_neg=0, track_z=1, qe=QE):
+                 precision=FP64, flag_B_pos=0, tag_neg=0, track_z=1, qe=QE, mg_tail_nodes=0):
         self.lib = capi.load()
         self.n1, self.n2, self.h = n1, n2, (h1, h2)
         self.nsp = len(species)
         self.species = list(species)
         self.precision = precision
         g = capi.GridDesc(n1, n2, h1, h2, zmax, dt, eps0, qe, ng, ngrid, len(species), precision,
-                          flag_B_pos, tag_neg, track_z, 0)
+                          flag_B_pos, tag_neg, track_z, mg_tail_nodes)
         sp = (capi.SpeciesDesc * len(species))(*[capi.SpeciesDesc(s.charge, s.mass, s.Nm,
                                                                    int(s.capacity)) for s in species])
         self._ctx = C.c_void_p()

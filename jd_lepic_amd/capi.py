@@ -28,7 +28,7 @@ class GridDesc(C.Structure):
                 ("zmax", C.c_double), ("dt", C.c_double), ("eps0", C.c_double), ("qe", C.c_double),
                 ("ng", C.c_int32), ("ngrid", C.c_int32), ("nspecies", C.c_int32),
                 ("precision", C.c_int32), ("flag_B_pos", C.c_int32), ("tag_neg", C.c_int32),
-                ("track_z", C.c_int32), ("reserved", C.c_int32)]
+                ("track_z", C.c_int32), ("mg_tail_nodes", C.c_int32)]
 
 
 class SpeciesDesc(C.Structure):

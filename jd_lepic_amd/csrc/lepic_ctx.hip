@@ -298,7 +298,7 @@ int lepic_create(const lepic_grid_desc* g, const lepic_species_desc* sp, lepic_c
   H(hipMemset(ctx->d_mcc_i64, 0, 64 * 8), "mcc");
   H(hipMalloc(&ctx->d_mcc_f64, 64 * 8), "mcc");
   H(hipMemset(ctx->d_mcc_f64, 0, 64 * 8), "mcc");
-  H(hipHostMalloc((void**)&ctx->h_pinned, 64 * 8, hipHostMallocDefault), "pinned");
+  H(hipHostMalloc((void**)&ctx->h_pinned, kPinnedDoubles * 8, hipHostMallocDefault), "pinned");
   ctx->bc_host.assign(ctx->nodes, 1);
   if (!rc) rc = poisson_setup(ctx);
   *out = ctx;
@@ -309,6 +309,7 @@ int lepic_create(const lepic_grid_desc* g, const lepic_species_desc* sp, lepic_c
 int lepic_destroy(lepic_ctx* ctx) {
   if (!ctx) return 0;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  poisson_teardown(ctx);
   for (int p = 0; p < ctx->nsp; ++p) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);

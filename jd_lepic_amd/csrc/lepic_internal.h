@@ -19,6 +19,10 @@ constexpr int kMaxTypes = 32;
 constexpr int kBlock = 256;          // threads per block of the particle kernels
 constexpr int kItems = 4;            // particles per thread in the mover
 constexpr int kChunk = kBlock * kItems;
+constexpr int kSweepsPerLaunch = 5;  // red-black sweeps per launch on multi-block MG levels
+constexpr int kRot = kSweepsPerLaunch + 1;  // rotating state buffers of a multi-block level
+constexpr int kRing = 16;            // residual partial-sum slots (sweep index mod kRing)
+constexpr int kPinnedDoubles = 1024; // pinned host scratch (doubles)
 
 #define LEPIC_HIP(call)                                                              \
   do {                                                                               \
@@ -37,18 +41,26 @@ struct Level {
   float* uf = nullptr;               // FP32 copies for mixed smoothing
   float* bf = nullptr;
   int8_t* bc = nullptr;              // (n1+3)(n2+3)
+  // multi-block levels: rotating states (index 0 is u or uf) and the
+  // residual partial sums of the sweep kernels (kRing slots x blocks)
+  bool mb = false;
+  void* rot[kRot] = {};
+  int nbx = 0, nby = 0;
+  double* partials = nullptr;
 };
 
-// Per-level smoothing control kept on the device (one V-cycle visit).
+// Device-side state of a multi-block MG level across its visits.  A visit
+// (down or up leg of one V-cycle) carries an epoch; visits of a level have
+// consecutive epochs.  The sweep kernels record an early stop
+// (done_epoch/ksor/res); the kernel consuming the visit's result resolves it
+// and stores the buffer holding the result in cur[epoch & 1], which the
+// level's next visit reads as cur[(epoch - 1) & 1].
 struct LevelCtrl {
-  double res;        // L1 residual of the last sweep
-  double res_s;      // previous sweep's residual
-  int32_t ksor;      // sweeps done in this visit
-  int32_t done;      // 1: stop sweeping
-  int32_t iter;      // sweeps of this level in this V-cycle (mg.f90:iter)
-  uint32_t arrive;   // last-block counter of the residual reduction
+  int32_t done_epoch;
+  int32_t ksor;
+  int32_t cur[2];
+  double res;
 };
-
 struct Species {
   double charge = 0, mass = 0, Nm = 0;
   int64_t cap = 0;
@@ -115,11 +127,13 @@ struct lepic_ctx {
   int ng = 1;
   lepic::Level lv[lepic::kMaxLevels + 1];
   int Lc = 1;                        // first level handled by the single-workgroup kernel
+  int32_t mg_epoch = 0;              // epoch of the last multi-block level visit
   lepic::LevelCtrl* d_ctrl = nullptr;   // [kMaxLevels+1]
-  double* d_partials = nullptr;      // residual partial sums
+  double* d_partials = nullptr;      // FP64 residual partial sums of the finest level
+  void* d_erot = nullptr;            // mixed: device array of the level-1 FP32 correction buffers
   int64_t partials_cap = 0;
   double* d_scalar = nullptr;        // small scratch (res0 etc.)
-  double* h_pinned = nullptr;        // pinned host scratch (64 doubles)
+  double* h_pinned = nullptr;        // pinned host scratch (kPinnedDoubles)
   double* d_resid64 = nullptr;       // FP64 residual grid for mixed refinement
   float* d_corr = nullptr;
 
@@ -172,6 +186,7 @@ int rhs_launch(lepic_ctx* ctx);
 int efield_launch(lepic_ctx* ctx);
 int dens_red_launch(lepic_ctx* ctx, double* np_mx_host);
 int poisson_setup(lepic_ctx* ctx);
+void poisson_teardown(lepic_ctx* ctx);
 int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_info* info,
                   int32_t* log, int32_t log_cap);
 int sort_launch(lepic_ctx* ctx, int ptype, int32_t* Plist_host);
@@ -225,6 +240,29 @@ __device__ inline int64_t lookback_exclusive(uint64_t* status, int64_t bid, uint
 }
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }
+
+// ---- wave reductions in a fixed order through DPP lane moves ----
+// (VALU data-parallel primitives: row_shr:1,2,4,8 build a scan inside each
+// row of 16 lanes, row_bcast:15 and row_bcast:31 carry it across rows; lane
+// 63 ends with the total.)  Every lane must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_move<0x111, 0xf>(v);
+  v += dpp_move<0x112, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
+  v += dpp_move<0x142, 0xa>(v);
+  v += dpp_move<0x143, 0xc>(v);
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+  return __hiloint2double(hi, lo);
+}
 
 }  // namespace lepic
 

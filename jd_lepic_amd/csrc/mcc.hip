@@ -591,13 +591,27 @@ __global__ void k_add_count(int64_t* counts, int b, const uint32_t* offs, const 
 
 // np_tot_rg of collisions.f90:144-154: INT(k * SUM(np_red over the region)),
 // summed in the reference's column-major order by one lane per region.
-__global__ void k_region_sum(const double* __restrict__ np_red, int n1, int ixl, int ixr, int iyl,
-                             int iyr, double kk, double* __restrict__ out) {
-  if (threadIdx.x || blockIdx.x) return;
+// Particles of one region (collisions.f90:144-154) for production mode: a
+// fixed-order block reduction (thread partials over a strided walk, then the
+// 16 wave sums in order), so the result is reproducible run to run; check
+// mode keeps the reference's sequential sum inside k_mcc_check.
+__global__ __launch_bounds__(1024) void k_region_sum(const double* __restrict__ np_red, int n1, int ixl,
+                                                     int ixr, int iyl, int iyr, double kk,
+                                                     double* __restrict__ out) {
+  __shared__ double ws[16];
+  const int nx = ixr - ixl + 1, ny = iyr - iyl + 1;
+  const int64_t cnt = (int64_t)nx * ny;
   double s = 0.0;
-  for (int iy = iyl; iy <= iyr; ++iy)
-    for (int ix = ixl; ix <= ixr; ++ix) s += np_red[node_index(ix, iy, n1)];
-  *out = 1.0 / (double)(int64_t)(kk * s);
+  for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x)
+    s += np_red[node_index(ixl + (int)(q % nx), iyl + (int)(q / nx), n1)];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
+    *out = 1.0 / (double)(int64_t)(kk * t);
+  }
 }
 
 }  // namespace
@@ -734,7 +748,7 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
       double* d_rg = ctx->d_mcc_f64 + 40;
       const double kk = ctx->g.h1 * ctx->g.h2 * ctx->g.zmax / R.Nm[pt];
       for (int rg = 0; rg < R.n_rg; ++rg)
-        hipLaunchKernelGGL(k_region_sum, dim3(1), dim3(64), 0, ctx->stream, (const double*)ctx->sp[q].np_red,
+        hipLaunchKernelGGL(k_region_sum, dim3(1), dim3(1024), 0, ctx->stream, (const double*)ctx->sp[q].np_red,
                            ctx->g.n1, R.ixl_rg[rg], R.ixr_rg[rg], R.iyl_rg, R.iyr_rg, kk, d_rg + rg);
       LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, d_rg, 5 * 8, hipMemcpyDeviceToHost, ctx->stream));
       // scratch: claim bitmap, staging

@@ -2,14 +2,23 @@
 // V-cycle (mg.f90:1-232) and red-black SOR with adaptive sweep control
 // (sors.f90:1-435).
 //
-// Levels with many nodes run node-parallel kernels (one red, one black and
-// one residual kernel per sweep); the residual kernel's last block sums the
-// block partials in a fixed order and takes the reference's exit decision
-// on the device, so later sweeps of the same level visit exit at once and
-// the host never waits inside a V-cycle.  The coarse levels whose arrays fit
-// in 64 KB of LDS run in a single workgroup kernel that performs the whole
-// lower part of the V-cycle (down-leg sweeps and restrictions, the coarsest
-// solve, prolongations and up-leg sweeps) without leaving the CU.
+// A kernel launch costs about 3 us on MI355X whatever it does, and a grid-wide
+// software barrier costs more, so the solver is organised to launch few
+// kernels and to need no grid synchronisation:
+//  * Fine ("multi-block") levels run k_sweeps: each block owns a 64-wide tile,
+//    stages the tile plus a halo of 2S+1 nodes in LDS and performs S red-black
+//    sweeps on it, recomputing the halo redundantly instead of exchanging it.
+//    After every sweep it writes the tile to that sweep's state buffer (S+1
+//    rotating buffers per level) and its share of the L1 residual.  The exit
+//    test of sors.f90:371-377 for those sweeps is taken at the start of the
+//    next launch (or by the kernel that consumes the level), by every block
+//    from the same partial sums in the same order, so all blocks agree and no
+//    atomics or host round trips are needed.  A visit with the reference's
+//    limit of 10 sweeps takes two launches.
+//  * Coarse levels (at most mg_tail_nodes nodes) run in k_tail, one workgroup
+//    of 1024 threads that performs the whole lower part of the V-cycle (down
+//    sweeps, restrictions, the coarsest solve, prolongations and up sweeps),
+//    with the levels that fit in 64 KB of LDS held there.
 // FP64 mode computes the reference's expressions in their order (no
 // contraction), so a sweep is bit-identical to sors.f90; only the order of
 // the L1 residual sum differs.  Mixed mode runs the same V-cycle in FP32 on
@@ -28,30 +37,21 @@ using namespace lepic;
 
 namespace {
 
-constexpr int kVisitLog = 2 * kMaxLevels;    // ig = 1 .. 2ng-1
-constexpr size_t kCoarseLds = 64 * 1024;
-constexpr int kCoarseThreads = 1024;
+constexpr int kVisitLog = 2 * kMaxLevels;     // ig = 1 .. 2ng-1
+constexpr size_t kTailLds = 64 * 1024;
+constexpr int kTailThreads = 1024;
+constexpr int kS = kSweepsPerLaunch;
+constexpr int kHalo = 2 * kS + 1;             // validity shrinks by one node per colour pass
+constexpr int kTX = 64;
+constexpr int kSweepThreads = 1024;
+constexpr int kDefaultTailNodes = 20000;
+constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
+constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
+constexpr int kRowBlocks = 128;               // row-loop blocks of the grid-transfer kernels
 
-template <typename T>
-struct Lv {
-  T* u;
-  const T* b;
-  const int8_t* bc;
-  int n1, n2;
-  T aw, an, ac;
-};
-
-template <typename T>
-Lv<T> lv_of(lepic_ctx* ctx, int l, bool f32) {
-  Level& L = ctx->lv[l];
-  Lv<T> r;
-  r.u = f32 ? (T*)L.uf : (T*)L.u;
-  r.b = f32 ? (const T*)L.bf : (const T*)L.b;
-  r.bc = L.bc;
-  r.n1 = L.n1; r.n2 = L.n2;
-  r.aw = (T)L.aw; r.an = (T)L.an; r.ac = (T)L.ac;
-  return r;
-}
+template <typename T> struct TileY;
+template <> struct TileY<float> { static constexpr int v = 32; };
+template <> struct TileY<double> { static constexpr int v = 16; };
 
 // ---- SOR update of one node (sors.f90:108-110 / 238-240) ----
 template <typename T>
@@ -75,128 +75,31 @@ __device__ __forceinline__ T getres(const T* u, const T* b, int ir, int jr, int 
   return urhs - (aw * uw + aw * ue + an * un + an * us + ac * uij);
 }
 
-// ---------------------------------------------------------------------------
-// multi-block kernels
-// ---------------------------------------------------------------------------
+// full-weighted residual of coarse node (i,j), mg.f90:386-414
 template <typename T>
-__global__ void k_sor_color(Lv<T> L, T omega, int color, const LevelCtrl* __restrict__ ctrl) {
-  if (__hip_atomic_load(&ctrl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  const int j = blockIdx.y + 1;
-  const int is = color ? ((j & 1) ? 2 : 1) : ((j & 1) ? 1 : 2);  // red: i+j odd
-  const int i = is + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (i > L.n1 + 1) return;
-  const int64_t W = L.n1 + 3, c = (int64_t)j * W + i;
-  if (L.bc[c] > 0) return;
-  T* u = L.u;
-  u[c] = sor_update(u[c], L.b[(int64_t)(j - 1) * (L.n1 + 1) + (i - 1)], u[c - 1], u[c + W], u[c - W],
-                    u[c + 1], L.aw, L.an, L.ac, omega);
-}
-
-// L1 residual over interior nodes and the exit test of sors.f90:338-379.
-template <typename T>
-__global__ __launch_bounds__(256) void k_residual(Lv<T> L, LevelCtrl* __restrict__ ctrl,
-                                                  double* __restrict__ partials, int dig, double eps,
-                                                  int nr, int* __restrict__ visit, int ig,
-                                                  double* __restrict__ vres) {
-  if (__hip_atomic_load(&ctrl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  __shared__ double ws[4];
-  __shared__ bool last;
-  const int j = blockIdx.y + 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  double r = 0.0;
-  if (i <= L.n1 + 1) {
-    const int64_t W = L.n1 + 3, c = (int64_t)j * W + i;
-    if (L.bc[c] == -1) {
-      const T* u = L.u;
-      r = fabs(sor_resid(u[c], L.b[(int64_t)(j - 1) * (L.n1 + 1) + (i - 1)], u[c - 1], u[c + W], u[c - W],
-                         u[c + 1], L.aw, L.an, L.ac));
-    }
-  }
-  for (int o = 32; o > 0; o >>= 1) r += __shfl_down(r, o, 64);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = r;
-  __syncthreads();
-  const unsigned nb = gridDim.x * gridDim.y;
-  const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-  if (threadIdx.x == 0) {
-    double s = ((ws[0] + ws[1]) + ws[2]) + ws[3];
-    __hip_atomic_store(&partials[bid], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = atomicAdd(&ctrl->arrive, 1u);
-    last = (prev == nb - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double s = 0.0;
-  for (unsigned k = threadIdx.x; k < nb; k += blockDim.x)
-    s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double res = ((ws[0] + ws[1]) + ws[2]) + ws[3];
-    ctrl->arrive = 0;
-    ctrl->ksor += 1;
-    ctrl->iter += 1;
-    ctrl->res = res;
-    bool stop;
-    if (dig == 0) stop = res <= eps;
-    else stop = ctrl->res_s > 0.0 && (res / ctrl->res_s) <= 0.6;
-    if (ctrl->ksor >= nr) stop = true;
-    ctrl->res_s = res;
-    if (stop) {
-      visit[ig] = ctrl->ksor;
-      *vres = res;
-      __hip_atomic_store(&ctrl->done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
-}
-
-__global__ void k_ctrl_reset(LevelCtrl* c) {
-  c->res = 0.0; c->res_s = 0.0; c->ksor = 0; c->done = 0; c->arrive = 0;
-}
-
-// restriction, mg.f90:335-444: full-weighted residual onto the coarse grid
-// and a zero coarse correction (the coarse mask is precomputed).
-template <typename T>
-__global__ void k_restrict(Lv<T> F, T* __restrict__ r2h, T* __restrict__ e2h,
-                           const int8_t* __restrict__ bc2h) {
-  const int m1 = F.n1 / 2, m2 = F.n2 / 2;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i > m1 + 2) return;
-  e2h[(int64_t)j * (m1 + 3) + i] = (T)0;
-  if (i < 1 || i > m1 + 1 || j < 1 || j > m2 + 1) return;
-  const int ir = 2 * i - 1, jr = 2 * j - 1, n1 = F.n1, n2 = F.n2;
+__device__ __forceinline__ T restrict_node(const T* u, const T* b, const int8_t* bc, int i, int j, int n1,
+                                           int n2, T aw, T an, T ac) {
+  const int ir = 2 * i - 1, jr = 2 * j - 1;
   const int64_t W = n1 + 3;
-  const int8_t* bc = F.bc;
-  T out = (T)0;
-  if (bc2h[(int64_t)j * (m1 + 3) + i] <= 0) {
-    auto B = [&](int a, int b) { return bc[(int64_t)b * W + a]; };
-    T r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0, r8 = 0;
-    T r9 = getres(F.u, F.b, ir, jr, n1, F.aw, F.an, F.ac);
-    if (ir != n1 + 1 && jr != n2 + 1 && B(ir + 1, jr + 1) <= 0) r1 = getres(F.u, F.b, ir + 1, jr + 1, n1, F.aw, F.an, F.ac);
-    if (ir != n1 + 1 && B(ir + 1, jr) <= 0) r2 = getres(F.u, F.b, ir + 1, jr, n1, F.aw, F.an, F.ac);
-    if (B(ir + 1, jr - 1) <= 0 && ir != n1 + 1 && jr != 1) r3 = getres(F.u, F.b, ir + 1, jr - 1, n1, F.aw, F.an, F.ac);
-    if (B(ir, jr - 1) <= 0 && jr != 1) r4 = getres(F.u, F.b, ir, jr - 1, n1, F.aw, F.an, F.ac);
-    if (B(ir - 1, jr - 1) <= 0 && ir != 1 && jr != 1) r5 = getres(F.u, F.b, ir - 1, jr - 1, n1, F.aw, F.an, F.ac);
-    if (B(ir - 1, jr) <= 0 && ir != 1) r6 = getres(F.u, F.b, ir - 1, jr, n1, F.aw, F.an, F.ac);
-    if (B(ir - 1, jr + 1) <= 0 && ir != 1 && jr != n2 + 1) r7 = getres(F.u, F.b, ir - 1, jr + 1, n1, F.aw, F.an, F.ac);
-    if (B(ir, jr + 1) <= 0 && jr != n2 + 1) r8 = getres(F.u, F.b, ir, jr + 1, n1, F.aw, F.an, F.ac);
-    out = ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
-  }
-  r2h[(int64_t)(j - 1) * (m1 + 1) + (i - 1)] = out;
+  auto B = [&](int a, int c) { return bc[(int64_t)c * W + a]; };
+  T r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0, r8 = 0;
+  T r9 = getres(u, b, ir, jr, n1, aw, an, ac);
+  if (ir != n1 + 1 && jr != n2 + 1 && B(ir + 1, jr + 1) <= 0) r1 = getres(u, b, ir + 1, jr + 1, n1, aw, an, ac);
+  if (ir != n1 + 1 && B(ir + 1, jr) <= 0) r2 = getres(u, b, ir + 1, jr, n1, aw, an, ac);
+  if (B(ir + 1, jr - 1) <= 0 && ir != n1 + 1 && jr != 1) r3 = getres(u, b, ir + 1, jr - 1, n1, aw, an, ac);
+  if (B(ir, jr - 1) <= 0 && jr != 1) r4 = getres(u, b, ir, jr - 1, n1, aw, an, ac);
+  if (B(ir - 1, jr - 1) <= 0 && ir != 1 && jr != 1) r5 = getres(u, b, ir - 1, jr - 1, n1, aw, an, ac);
+  if (B(ir - 1, jr) <= 0 && ir != 1) r6 = getres(u, b, ir - 1, jr, n1, aw, an, ac);
+  if (B(ir - 1, jr + 1) <= 0 && ir != 1 && jr != n2 + 1) r7 = getres(u, b, ir - 1, jr + 1, n1, aw, an, ac);
+  if (B(ir, jr + 1) <= 0 && jr != n2 + 1) r8 = getres(u, b, ir, jr + 1, n1, aw, an, ac);
+  return ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
 }
 
-// prolongation, mg.f90:585-611: bilinear add of the coarse correction
+// bilinear add of coarse node (i,j)'s correction to its four fine nodes,
+// mg.f90:585-611
 template <typename T>
-__global__ void k_prolong(T* __restrict__ u, const int8_t* __restrict__ bc, int n1, int n2,
-                          const T* __restrict__ e2h) {
-  const int m1 = n1 / 2, m2 = n2 / 2;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  const int j = blockIdx.y + 1;
-  if (i > m1 + 1 || j > m2 + 1) return;
+__device__ __forceinline__ void prolong_node(T* u, const int8_t* bc, int n1, const T* e2h, int m1, int i,
+                                             int j) {
   const int64_t Wc = m1 + 3, W = n1 + 3;
   T r1 = e2h[(int64_t)j * Wc + i], r2 = e2h[(int64_t)(j - 1) * Wc + i];
   T r3 = e2h[(int64_t)(j - 1) * Wc + i - 1], r4 = e2h[(int64_t)j * Wc + i - 1];
@@ -209,230 +112,400 @@ __global__ void k_prolong(T* __restrict__ u, const int8_t* __restrict__ bc, int 
 }
 
 // ---------------------------------------------------------------------------
-// single-workgroup kernel for the coarse levels Lc..ng
+// visit control of multi-block levels
 // ---------------------------------------------------------------------------
-template <typename T>
-struct CLevel {
-  int n1, n2;
-  T aw, an, ac;
-  uint32_t ou, ob, obc;       // byte offsets in LDS
-  const int8_t* gbc;
-};
-
-template <typename T>
-struct CoarseArgs {
-  int Lc, ng, k;
-  T omega;
+struct VisitRef {
+  LevelCtrl* ctrl;
+  const double* partials;
+  int nb;                  // blocks of the level's sweep kernel
+  int epoch, dig, nr, ig;
   double eps;
-  T* gu;                      // level Lc correction/potential (global)
-  const T* gb;                // level Lc right-hand side (global)
+  int nexec;               // sweeps launched in the visit
+  int klast0;              // sweeps before the visit's last launch
   int* visit;
   double* vres;
-  CLevel<T> lv[kMaxLevels + 1];
 };
 
-__device__ inline double block_sum_1024(double v, double* ws) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double s = 0.0;
-  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += ws[w];  // same order in every thread
-  return s;
+// L1 residuals after sweeps kf-1 .. kt of the current visit (kt-kf+2 <= kS+1
+// values): the blocks' partial sums in a fixed order (lane strides, then a
+// shuffle tree).  All loads are issued before any sum, so the evaluation
+// costs one memory round trip.  Wave 0, all lanes; r[0] is sweep kf-1 (zero
+// when kf = 1).
+__device__ void ring_res(const double* partials, int nb, int kf, int kt, double* r) {
+  const int lane = threadIdx.x & 63;
+  double s[kS + 1];
+#pragma unroll
+  for (int m = 0; m <= kS; ++m) {
+    const int k = kf - 1 + m;
+    s[m] = 0.0;
+    if (k >= 1 && k <= kt) {
+      const double* p = partials + (size_t)(k % kRing) * nb;
+      for (int q = lane; q < nb; q += 64) s[m] += p[q];
+    }
+  }
+#pragma unroll
+  for (int m = 0; m <= kS; ++m) r[m] = wave_sum(s[m]);
 }
 
+// sors.f90:371-377 for sweeps kf..kt (at most kS of them): the first sweep
+// after which the reference stops (0 if none); res_s is the previous
+// sweep's residual, zero before the first sweep of a visit.
+__device__ int first_stop(const double* partials, int nb, int kf, int kt, int dig, double eps, int nr,
+                          double* r_out) {
+  double r[kS + 1];
+  ring_res(partials, nb, kf, kt, r);
+  double prev = r[0];
+  for (int k = kf; k <= kt; ++k) {
+    const double rk = r[k - kf + 1];
+    bool stop = (dig == 0) ? (rk <= eps) : (prev > 0.0 && rk / prev <= 0.6);
+    if (k >= nr) stop = true;
+    if (stop) {
+      *r_out = rk;
+      return k;
+    }
+    prev = rk;
+  }
+  return 0;
+}
+
+struct Resolved {
+  int ks;                  // sweeps of the visit (0: not finished)
+  int buf;                 // rotating buffer holding the state after them
+  double res;
+};
+
+// Outcome of a visit after all its launches; every block of the consuming
+// kernel evaluates it (wave 0) and block 0 records it.  `canon` records
+// buffer 0 as the level's state (the caller copies the result there).
+__device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
+  LevelCtrl* c = V.ctrl;
+  const int c0 = c->cur[(V.epoch - 1) & 1];
+  Resolved out{0, -1, 0.0};
+  if (c->done_epoch == V.epoch) {
+    out.ks = c->ksor;
+    out.res = c->res;
+  } else {
+    out.ks = first_stop(V.partials, V.nb, V.klast0 + 1, V.nexec, V.dig, V.eps, V.nr, &out.res);
+  }
+  if (out.ks) out.buf = (c0 + out.ks) % kRot;
+  if (writer && out.ks && threadIdx.x == 0) {
+    c->cur[V.epoch & 1] = canon ? 0 : out.buf;
+    V.visit[V.ig] = out.ks;
+    V.vres[V.ig] = out.res;
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// multi-block levels
+// ---------------------------------------------------------------------------
 template <typename T>
-__device__ void c_smooth(const CLevel<T>& L, char* smem, T omega, double eps, int dig, int nr,
-                         double* ws, int* ksor_out, double* res_out) {
-  T* u = (T*)(smem + L.ou);
-  const T* b = (const T*)(smem + L.ob);
-  const int8_t* bc = (const int8_t*)(smem + L.obc);
-  const int n1 = L.n1, n2 = L.n2;
-  const int W = n1 + 3;
-  const int cnt = (n1 + 1) * (n2 + 1);
-  double res_s = 0.0, res = 0.0;
-  int ksor = 0;
-  for (int k = 1; k <= nr; ++k) {
-    for (int color = 1; color >= 0; --color) {  // red (i+j odd) then black
-      for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-        int i = q % (n1 + 1) + 1, j = q / (n1 + 1) + 1;
-        if (((i + j) & 1) != color) continue;
-        int c = j * W + i;
-        if (bc[c] > 0) continue;
-        u[c] = sor_update(u[c], b[q], u[c - 1], u[c + W], u[c - W], u[c + 1], L.aw, L.an, L.ac, omega);
+struct SweepArgs {
+  T* rot[kRot];
+  const T* b;
+  const int8_t* bc;
+  int n1, n2;
+  T aw, an, ac, omega;
+  LevelCtrl* ctrl;
+  double* partials;
+  int nbx, nby;
+  int epoch, dig, nr;
+  double eps;
+};
+
+// Sweeps k0+1 .. k0+kexec of a visit on one tile (see the file comment).
+template <typename T>
+__global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec) {
+  constexpr int TY = TileY<T>::v;
+  constexpr int RW = kTX + 2 * kHalo, RH = TY + 2 * kHalo, RN = RW * RH;
+  constexpr int HW = (RW - 2) / 2;  // nodes of one colour per inner region row
+  static_assert((RW & 1) == 0, "region width must be even");
+  __shared__ T su[RN];
+  __shared__ T sb[RN];
+  __shared__ int8_t sbc[RN];
+  __shared__ int s_go, s_c0;
+  __shared__ double ws[kSweepThreads / 64];
+  const int nb = A.nbx * A.nby;
+  const int bid = blockIdx.y * A.nbx + blockIdx.x;
+  if (threadIdx.x < 64) {
+    LevelCtrl* c = A.ctrl;
+    int go = 1;
+    const int c0 = c->cur[(A.epoch - 1) & 1];
+    if (__hip_atomic_load(&c->done_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == A.epoch) {
+      go = 0;
+    } else if (k0 > 0) {
+      double r = 0.0;
+      const int ks = first_stop(A.partials, nb, k0 - kS + 1, k0, A.dig, A.eps, A.nr, &r);
+      if (ks) {
+        go = 0;
+        if (bid == 0 && threadIdx.x == 0) {
+          c->ksor = ks;
+          c->res = r;
+          __hip_atomic_store(&c->done_epoch, A.epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+    }
+    if (threadIdx.x == 0) {
+      s_go = go;
+      s_c0 = c0;
+    }
+  }
+  __syncthreads();
+  if (!s_go) return;
+
+  const int n1 = A.n1, n2 = A.n2;
+  const int64_t W = n1 + 3;
+  const int gi0 = 1 + (int)blockIdx.x * kTX - kHalo;  // global i of region column 0
+  const int gj0 = 1 + (int)blockIdx.y * TY - kHalo;
+  const T* uin = A.rot[(s_c0 + k0) % kRot];
+  for (int q = threadIdx.x; q < RN; q += kSweepThreads) {
+    const int r = q / RW, cc = q - r * RW;
+    const int i = gi0 + cc, j = gj0 + r;
+    const bool in = i >= 1 && i <= n1 + 1 && j >= 1 && j <= n2 + 1;
+    su[q] = in ? uin[(int64_t)j * W + i] : (T)0;
+    sbc[q] = in ? A.bc[(int64_t)j * W + i] : (int8_t)1;
+    sb[q] = in ? A.b[(int64_t)(j - 1) * (n1 + 1) + (i - 1)] : (T)0;
+  }
+  __syncthreads();
+  const T aw = A.aw, an = A.an, ac = A.ac, om = A.omega;
+  constexpr int NB = 4;  // slots per thread whose loads are issued together
+  constexpr int kPass = (RH - 2) * HW;
+  for (int s = 1; s <= kexec; ++s) {
+    for (int color = 1; color >= 0; --color) {  // red (i+j odd), then black
+      // nodes of one colour never read each other, so a thread gathers all
+      // its slots before it writes any
+      for (int q0 = threadIdx.x; q0 < kPass; q0 += NB * kSweepThreads) {
+        int x[NB];
+        int8_t m[NB];
+        T c[NB], w[NB], e[NB], n[NB], so[NB], bb[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+          const int q = q0 + k * kSweepThreads;
+          const int qq = q < kPass ? q : 0;
+          const int r = 1 + qq / HW;
+          x[k] = r * RW + 1 + 2 * (qq % HW) + ((color ^ ((gi0 + gj0 + r + 1) & 1)) & 1);
+          m[k] = sbc[x[k]];
+          if (q >= kPass) m[k] = 1;
+          c[k] = su[x[k]]; w[k] = su[x[k] - 1]; e[k] = su[x[k] + 1];
+          n[k] = su[x[k] + RW]; so[k] = su[x[k] - RW]; bb[k] = sb[x[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < NB; ++k)
+          if (m[k] <= 0) su[x[k]] = sor_update(c[k], bb[k], w[k], n[k], so[k], e[k], aw, an, ac, om);
       }
       __syncthreads();
     }
-    double r = 0.0;
-    for (int q = threadIdx.x; q < cnt; q += blockDim.x) {
-      int i = q % (n1 + 1) + 1, j = q / (n1 + 1) + 1;
-      int c = j * W + i;
-      if (bc[c] == -1) r += fabs(sor_resid(u[c], b[q], u[c - 1], u[c + W], u[c - W], u[c + 1], L.aw, L.an, L.ac));
+    double rr = 0.0;
+    T* uout = A.rot[(s_c0 + k0 + s) % kRot];
+    for (int q0 = threadIdx.x; q0 < kTX * TY; q0 += NB * kSweepThreads) {
+      int x[NB], gi[NB], gj[NB];
+      int8_t m[NB];
+      T c[NB], w[NB], e[NB], n[NB], so[NB], bb[NB];
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        const int q = q0 + k * kSweepThreads;
+        const int qq = q < kTX * TY ? q : 0;
+        const int tr = qq / kTX, tc = qq - tr * kTX;
+        x[k] = (kHalo + tr) * RW + kHalo + tc;
+        gi[k] = q < kTX * TY ? gi0 + kHalo + tc : n1 + 2;
+        gj[k] = gj0 + kHalo + tr;
+        m[k] = sbc[x[k]];
+        c[k] = su[x[k]]; w[k] = su[x[k] - 1]; e[k] = su[x[k] + 1];
+        n[k] = su[x[k] + RW]; so[k] = su[x[k] - RW]; bb[k] = sb[x[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < NB; ++k) {
+        if (gi[k] > n1 + 1 || gj[k] > n2 + 1) continue;
+        if (m[k] == -1) rr += fabs(sor_resid(c[k], bb[k], w[k], n[k], so[k], e[k], aw, an, ac));
+        uout[(int64_t)gj[k] * W + gi[k]] = c[k];
+      }
     }
-    res = block_sum_1024(r, ws);
-    ksor += 1;
-    bool stop;
-    if (dig == 0) stop = res <= eps;
-    else stop = res_s > 0.0 && (res / res_s) <= 0.6;
-    if (stop) break;
-    res_s = res;
+    rr = wave_sum(rr);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = rr;
+    __syncthreads();
+    if (threadIdx.x < 64) {  // fixed-order sum of the 16 wave sums
+      const double t = wave_sum(threadIdx.x < kSweepThreads / 64 ? ws[threadIdx.x] : 0.0);
+      if (threadIdx.x == 0) A.partials[(size_t)((k0 + s) % kRing) * nb + bid] = t;
+    }
   }
-  *ksor_out = ksor;
-  *res_out = res;
 }
 
+// restriction, mg.f90:335-444, from a multi-block level: resolves the fine
+// level's down visit, writes the coarse right-hand side and zeroes the
+// coarse correction in the buffer the coarse level's visit starts from.
 template <typename T>
-__device__ void c_restrict(const CLevel<T>& F, const CLevel<T>& C, char* smem) {
-  const T* u = (const T*)(smem + F.ou);
-  const T* b = (const T*)(smem + F.ob);
-  const int8_t* bc = (const int8_t*)(smem + F.obc);
-  T* r2h = (T*)(smem + C.ob);
-  T* e2h = (T*)(smem + C.ou);
-  const int8_t* bc2 = (const int8_t*)(smem + C.obc);
-  const int n1 = F.n1, n2 = F.n2, m1 = C.n1, m2 = C.n2, W = n1 + 3;
-  for (int q = threadIdx.x; q < (m1 + 3) * (m2 + 3); q += blockDim.x) e2h[q] = (T)0;
-  for (int q = threadIdx.x; q < (m1 + 1) * (m2 + 1); q += blockDim.x) {
-    int i = q % (m1 + 1) + 1, j = q / (m1 + 1) + 1;
-    int ir = 2 * i - 1, jr = 2 * j - 1;
+struct RestrictArgs {
+  VisitRef vf;
+  T* frot[kRot];
+  const T* fb;
+  const int8_t* fbc;
+  int n1, n2;
+  T aw, an, ac;
+  T* r2h;
+  const int8_t* bc2h;
+  T* crot[kRot];
+  LevelCtrl* cctrl;        // null when the coarse level runs in k_tail
+  int cepoch;
+};
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_restrict(RestrictArgs<T> A) {
+  __shared__ int s_fb, s_cb;
+  if (threadIdx.x < 64) {
+    const bool writer = blockIdx.x == 0 && blockIdx.y == 0;
+    Resolved r = resolve_visit(A.vf, writer, false);
+    if (threadIdx.x == 0) {
+      s_fb = r.buf < 0 ? 0 : r.buf;
+      s_cb = A.cctrl ? A.cctrl->cur[(A.cepoch - 1) & 1] : 0;
+    }
+  }
+  __syncthreads();
+  const T* u = A.frot[s_fb];
+  T* e2h = A.crot[s_cb];
+  const int m1 = A.n1 / 2, m2 = A.n2 / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > m1 + 2) return;
+  for (int j = blockIdx.y; j <= m2 + 2; j += gridDim.y) {
+    e2h[(int64_t)j * (m1 + 3) + i] = (T)0;
+    if (i < 1 || i > m1 + 1 || j < 1 || j > m2 + 1) continue;
     T out = (T)0;
-    if (bc2[j * (m1 + 3) + i] <= 0) {
-      auto B = [&](int a, int c) { return bc[c * W + a]; };
-      T r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0, r8 = 0;
-      T r9 = getres(u, b, ir, jr, n1, F.aw, F.an, F.ac);
-      if (ir != n1 + 1 && jr != n2 + 1 && B(ir + 1, jr + 1) <= 0) r1 = getres(u, b, ir + 1, jr + 1, n1, F.aw, F.an, F.ac);
-      if (ir != n1 + 1 && B(ir + 1, jr) <= 0) r2 = getres(u, b, ir + 1, jr, n1, F.aw, F.an, F.ac);
-      if (B(ir + 1, jr - 1) <= 0 && ir != n1 + 1 && jr != 1) r3 = getres(u, b, ir + 1, jr - 1, n1, F.aw, F.an, F.ac);
-      if (B(ir, jr - 1) <= 0 && jr != 1) r4 = getres(u, b, ir, jr - 1, n1, F.aw, F.an, F.ac);
-      if (B(ir - 1, jr - 1) <= 0 && ir != 1 && jr != 1) r5 = getres(u, b, ir - 1, jr - 1, n1, F.aw, F.an, F.ac);
-      if (B(ir - 1, jr) <= 0 && ir != 1) r6 = getres(u, b, ir - 1, jr, n1, F.aw, F.an, F.ac);
-      if (B(ir - 1, jr + 1) <= 0 && ir != 1 && jr != n2 + 1) r7 = getres(u, b, ir - 1, jr + 1, n1, F.aw, F.an, F.ac);
-      if (B(ir, jr + 1) <= 0 && jr != n2 + 1) r8 = getres(u, b, ir, jr + 1, n1, F.aw, F.an, F.ac);
-      out = ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
+    if (A.bc2h[(int64_t)j * (m1 + 3) + i] <= 0)
+      out = restrict_node(u, A.fb, A.fbc, i, j, A.n1, A.n2, A.aw, A.an, A.ac);
+    A.r2h[(int64_t)(j - 1) * (m1 + 1) + (i - 1)] = out;
+  }
+}
+
+// prolongation into a multi-block level from the level below (resolving the
+// coarse level's up visit when it is multi-block itself)
+template <typename T>
+struct ProlongArgs {
+  T* frot[kRot];
+  LevelCtrl* fctrl;
+  int fepoch;              // epoch of the fine level's coming up visit
+  const int8_t* bc;
+  int n1, n2;
+  T* crot[kRot];
+  int coarse_mb;
+  VisitRef vc;
+};
+
+template <typename T>
+__global__ __launch_bounds__(128) void k_prolong(ProlongArgs<T> A) {
+  __shared__ int s_fb, s_cb;
+  if (threadIdx.x < 64) {
+    int cb = 0;
+    if (A.coarse_mb) {
+      const bool writer = blockIdx.x == 0 && blockIdx.y == 0;
+      Resolved r = resolve_visit(A.vc, writer, false);
+      cb = r.buf < 0 ? 0 : r.buf;
     }
-    r2h[q] = out;
+    if (threadIdx.x == 0) {
+      s_cb = cb;
+      s_fb = A.fctrl->cur[(A.fepoch - 1) & 1];
+    }
   }
   __syncthreads();
+  const int m1 = A.n1 / 2, m2 = A.n2 / 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (i > m1 + 1) return;
+  for (int j = blockIdx.y + 1; j <= m2 + 1; j += gridDim.y)
+    prolong_node(A.frot[s_fb], A.bc, A.n1, (const T*)A.crot[s_cb], m1, i, j);
 }
 
-template <typename T>
-__device__ void c_prolong(const CLevel<T>& F, const CLevel<T>& C, char* smem) {
-  T* u = (T*)(smem + F.ou);
-  const int8_t* bc = (const int8_t*)(smem + F.obc);
-  const T* e2h = (const T*)(smem + C.ou);
-  const int n1 = F.n1, m1 = C.n1, m2 = C.n2, W = n1 + 3, Wc = m1 + 3;
-  for (int q = threadIdx.x; q < (m1 + 1) * (m2 + 1); q += blockDim.x) {
-    int i = q % (m1 + 1) + 1, j = q / (m1 + 1) + 1;
-    T r1 = e2h[j * Wc + i], r2 = e2h[(j - 1) * Wc + i], r3 = e2h[(j - 1) * Wc + i - 1], r4 = e2h[j * Wc + i - 1];
-    int ir = 2 * i - 1, jr = 2 * j - 1, c = jr * W + ir;
-    if (bc[c] <= 0) u[c] = u[c] + r1;
-    if (bc[c - W] <= 0) u[c - W] = u[c - W] + (r1 + r2) / (T)2.0;
-    if (bc[c - W - 1] <= 0) u[c - W - 1] = u[c - W - 1] + (r1 + r2 + r3 + r4) / (T)4.0;
-    if (bc[c - 1] <= 0) u[c - 1] = u[c - 1] + (r1 + r4) / (T)2.0;
+// FP64 mode, end of a V-cycle on a multi-block finest level: resolve the up
+// visit and copy its result into phi (buffer 0) when it lies elsewhere.
+struct Finish64Args {
+  VisitRef v;
+  double* rot[kRot];
+  int n1, n2;
+  int* status;             // sweeps of the visit, 0 if not finished
+};
+
+__global__ __launch_bounds__(256) void k_finish64(Finish64Args A) {
+  __shared__ int s_buf, s_ks;
+  if (threadIdx.x < 64) {
+    const bool writer = blockIdx.x == 0 && blockIdx.y == 0;
+    Resolved r = resolve_visit(A.v, writer, true);
+    if (threadIdx.x == 0) {
+      s_buf = r.buf;
+      s_ks = r.ks;
+      if (writer) *A.status = r.ks;
+    }
   }
   __syncthreads();
+  if (s_ks == 0 || s_buf == 0) return;
+  const int64_t W = A.n1 + 3;
+  for (int j = blockIdx.y + 1; j <= A.n2 + 1; j += gridDim.y)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x + 1; i <= A.n1 + 1; i += gridDim.x * blockDim.x)
+      A.rot[0][(int64_t)j * W + i] = A.rot[s_buf][(int64_t)j * W + i];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kCoarseThreads) void k_coarse(CoarseArgs<T> A) {
-  extern __shared__ __align__(16) char smem[];
-  __shared__ double ws[16];
-  const int Lc = A.Lc, ng = A.ng;
-  for (int l = Lc; l <= ng; ++l) {
-    const CLevel<T>& L = A.lv[l];
-    int8_t* bc = (int8_t*)(smem + L.obc);
-    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) bc[q] = L.gbc[q];
-  }
-  {
-    const CLevel<T>& L = A.lv[Lc];
-    T* u = (T*)(smem + L.ou);
-    T* b = (T*)(smem + L.ob);
-    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) u[q] = A.gu[q];
-    for (int q = threadIdx.x; q < (L.n1 + 1) * (L.n2 + 1); q += blockDim.x) b[q] = A.gb[q];
+// Mixed mode, end of a V-cycle: phi += e (bcnd <= 0 nodes) with the FP32
+// correction of the up visit, which is then zeroed for the next V-cycle.
+struct AddCorrArgs {
+  double* phi;
+  float* rot[kRot];
+  const int8_t* bc;
+  int n1, n2;
+  int mb;
+  VisitRef v;
+};
+
+__global__ __launch_bounds__(256) void k_add_corr(AddCorrArgs A) {
+  __shared__ int s_buf;
+  if (threadIdx.x < 64) {
+    int buf = 0;
+    if (A.mb) {
+      const bool writer = blockIdx.x == 0 && blockIdx.y == 0;
+      Resolved r = resolve_visit(A.v, writer, false);
+      buf = r.buf < 0 ? 0 : r.buf;
+    }
+    if (threadIdx.x == 0) s_buf = buf;
   }
   __syncthreads();
-  int ks;
-  double res;
-  for (int l = Lc; l < ng; ++l) {  // down leg: ig = l
-    c_smooth(A.lv[l], smem, A.omega, A.eps, l - ng, 10, ws, &ks, &res);
-    if (threadIdx.x == 0) A.visit[l] = ks;
-    c_restrict(A.lv[l], A.lv[l + 1], smem);
-  }
-  c_smooth(A.lv[ng], smem, A.omega, A.eps, 0, A.k == 1 ? 1000000 : 10, ws, &ks, &res);  // ig = ng
-  if (threadIdx.x == 0) A.visit[ng] = ks;
-  for (int l = ng - 1; l >= Lc; --l) {  // up leg: ig = 2ng - l
-    c_prolong(A.lv[l], A.lv[l + 1], smem);
-    c_smooth(A.lv[l], smem, A.omega, A.eps, ng - l, 10, ws, &ks, &res);
-    if (threadIdx.x == 0) A.visit[2 * ng - l] = ks;
-  }
-  {
-    const CLevel<T>& L = A.lv[Lc];
-    const T* u = (const T*)(smem + L.ou);
-    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) A.gu[q] = u[q];
-  }
-  if (threadIdx.x == 0) *A.vres = res;
+  float* e = A.rot[s_buf];
+  const int64_t W = A.n1 + 3;
+  for (int j = blockIdx.y + 1; j <= A.n2 + 1; j += gridDim.y)
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x + 1; i <= A.n1 + 1; i += gridDim.x * blockDim.x) {
+      const int64_t q = (int64_t)j * W + i;
+      if (A.bc[q] <= 0) A.phi[q] = A.phi[q] + (double)e[q];
+      e[q] = 0.0f;
+    }
 }
 
-// ---------------------------------------------------------------------------
-// mixed-precision helpers: FP64 residual of phi (reference definition) and
-// its FP32 copy as the right-hand side of the correction equation
-// ---------------------------------------------------------------------------
+// FP64 L1 residual of phi on the finest level (the reference's definition,
+// sors.f90:348-349 order); in mixed mode it also becomes the FP32 right-hand
+// side of the next correction equation.  One partial sum per block; blocks
+// walk rows j = blockIdx.x + 1 + m*gridDim.x.  zero_buf >= 0 also clears the
+// level-1 FP32 correction buffer the next visit starts from.
 __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ u, const double* __restrict__ b,
                                                  const int8_t* __restrict__ bc, int n1, int n2, double aw,
                                                  double an, double ac, float* __restrict__ bf,
-                                                 float* __restrict__ ef, double* __restrict__ partials,
-                                                 unsigned* __restrict__ arrive, double* __restrict__ out) {
+                                                 float* const* __restrict__ erot, const LevelCtrl* ctrl,
+                                                 int epoch_next, double* __restrict__ partials) {
   __shared__ double ws[4];
-  __shared__ bool last;
-  const int j = blockIdx.y + 1;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  __shared__ int s_zb;
+  if (threadIdx.x == 0) s_zb = erot ? (ctrl ? ctrl->cur[(epoch_next - 1) & 1] : 0) : -1;
+  __syncthreads();
+  float* ez = s_zb >= 0 ? erot[s_zb] : nullptr;
+  const int64_t W = n1 + 3;
   double r = 0.0;
-  if (i <= n1 + 1) {
-    const int64_t W = n1 + 3, c = (int64_t)j * W + i;
-    const int64_t q = (int64_t)(j - 1) * (n1 + 1) + (i - 1);
-    double rij = 0.0;
-    if (bc[c] == -1)
-      rij = b[q] - (aw * u[c - 1] + an * u[c + W] + an * u[c - W] + aw * u[c + 1] + ac * u[c]);
-    if (bf) bf[q] = (float)rij;
-    r = fabs(rij);
+  for (int j = blockIdx.x + 1; j <= n2 + 1; j += gridDim.x) {
+    for (int i = threadIdx.x + 1; i <= n1 + 1; i += blockDim.x) {
+      const int64_t c = (int64_t)j * W + i;
+      const int64_t q = (int64_t)(j - 1) * (n1 + 1) + (i - 1);
+      double rij = 0.0;
+      if (bc[c] == -1)
+        rij = b[q] - (aw * u[c - 1] + an * u[c + W] + an * u[c - W] + aw * u[c + 1] + ac * u[c]);
+      if (bf) bf[q] = (float)rij;
+      if (ez) ez[c] = 0.0f;
+      r += fabs(rij);
+    }
   }
-  if (ef) {  // zero the FP32 correction on this block's nodes (and ghosts)
-    const int64_t W = n1 + 3;
-    if (i <= n1 + 1) ef[(int64_t)j * W + i] = 0.0f;
-    if (i == 1) { ef[(int64_t)j * W] = 0.0f; ef[(int64_t)j * W + n1 + 2] = 0.0f; }
-    if (j == 1 && i <= n1 + 1) { ef[i] = 0.0f; ef[(int64_t)(n2 + 2) * W + i] = 0.0f; }
-    if (i == 1 && j == 1) { ef[0] = 0.0f; ef[n1 + 2] = 0.0f; ef[(int64_t)(n2 + 2) * W] = 0.0f; ef[(int64_t)(n2 + 2) * W + n1 + 2] = 0.0f; }
-  }
-  for (int o = 32; o > 0; o >>= 1) r += __shfl_down(r, o, 64);
+  r = wave_sum(r);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = r;
   __syncthreads();
-  const unsigned nb = gridDim.x * gridDim.y;
-  const unsigned bid = blockIdx.y * gridDim.x + blockIdx.x;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[bid], ((ws[0] + ws[1]) + ws[2]) + ws[3], __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    last = atomicAdd(arrive, 1u) == nb - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double s = 0.0;
-  for (unsigned k = threadIdx.x; k < nb; k += blockDim.x)
-    s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    *out = ((ws[0] + ws[1]) + ws[2]) + ws[3];
-    *arrive = 0;
-  }
-}
-
-__global__ void k_add_correction(double* __restrict__ u, const float* __restrict__ e,
-                                 const int8_t* __restrict__ bc, int64_t n) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q < n && bc[q] <= 0) u[q] = u[q] + (double)e[q];
+  if (threadIdx.x == 0) partials[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
 }
 
 __global__ void k_cold_start(double* __restrict__ u, const int8_t* __restrict__ bc, int64_t n) {
@@ -447,92 +520,357 @@ __global__ void k_copy_interior(double* __restrict__ u, const double* __restrict
 }
 
 // ---------------------------------------------------------------------------
-// host side
+// single-workgroup kernel for the coarse levels Ls..ng
 // ---------------------------------------------------------------------------
-struct Plan {
-  int* d_visit;                // [kVisitLog]
-  double* d_vres;
+template <typename T>
+struct TLevel {
+  int n1, n2;
+  T aw, an, ac;
+  T* gu;                   // global arrays
+  T* gb;
+  const int8_t* gbc;
+  int lds;                 // 1: held in LDS at the offsets below
+  uint32_t ou, ob, obc;
 };
 
 template <typename T>
-int launch_sweeps(lepic_ctx* ctx, int l, bool f32, int dig, double eps, int nr, int ig, int count,
-                  int* visit, double* vres) {
-  Lv<T> L = lv_of<T>(ctx, l, f32);
-  LevelCtrl* c = ctx->d_ctrl + l;
-  const int hw = (L.n1 + 2) / 2 + 1;
-  dim3 gc((hw + 127) / 128, L.n2 + 1), gr((L.n1 + 1 + 255) / 256, L.n2 + 1);
-  const T omega = (T)1.4;
-  for (int s = 0; s < count; ++s) {
-    hipLaunchKernelGGL(k_sor_color<T>, gc, dim3(128), 0, ctx->stream, L, omega, 1, (const LevelCtrl*)c);
-    hipLaunchKernelGGL(k_sor_color<T>, gc, dim3(128), 0, ctx->stream, L, omega, 0, (const LevelCtrl*)c);
-    hipLaunchKernelGGL(k_residual<T>, gr, dim3(256), 0, ctx->stream, L, c, ctx->d_partials, dig, eps, nr,
-                       visit, ig, vres);
+struct TailArgs {
+  int Ls, ng, k;
+  T omega;
+  double eps;
+  int* visit;
+  double* vres;
+  TLevel<T> lv[kMaxLevels + 1];
+};
+
+// Sum over the block in a fixed order (wave trees, then a tree over the wave
+// sums that every wave evaluates the same way); the caller guarantees a
+// barrier since the previous call's reads of ws.
+__device__ inline double block_sum_1024(double v, double* ws) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  return wave_sum(lane < nw ? ws[lane] : 0.0);
+}
+
+// explicit address spaces, so LDS-resident levels use ds_ instructions and
+// global ones global_ instructions instead of flat accesses
+template <typename T> using lds_p = __attribute__((address_space(3))) T*;
+template <typename T> using glb_p = __attribute__((address_space(1))) T*;
+
+// red-black SOR at one tail level; lanes walk columns, waves walk rows
+template <typename T, typename PU, typename PB, typename PC>
+__device__ void t_smooth(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega, double eps, int dig,
+                         int nr, double* ws, int* ksor_out, double* res_out) {
+  const int W = n1 + 3;
+  const int hw = (n1 + 2) / 2;   // slots of one colour per row
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double res_s = 0.0, res = 0.0;
+  int ksor = 0;
+  constexpr int NB = sizeof(T) == 8 ? 2 : 4;  // rows per wave whose loads are issued together
+  for (int k = 1; k <= nr; ++k) {
+    for (int color = 1; color >= 0; --color) {  // red (i+j odd) then black
+      for (int j0 = 1 + wv; j0 <= n2 + 1; j0 += NB * nw) {
+        for (int mm = lane; mm < hw; mm += 64) {
+          int x[NB], xb[NB];
+          bool act[NB];
+          T c[NB], w[NB], e[NB], n[NB], so[NB], bb[NB];
+#pragma unroll
+          for (int q = 0; q < NB; ++q) {
+            const int j = j0 + q * nw;
+            const int i = 1 + 2 * mm + ((color ^ ((j + 1) & 1)) & 1);
+            act[q] = j <= n2 + 1 && i <= n1 + 1;
+            x[q] = act[q] ? j * W + i : W + 1;
+            xb[q] = act[q] ? (j - 1) * (n1 + 1) + (i - 1) : 0;
+            const int8_t mq = bc[x[q]];
+            act[q] = act[q] & (mq <= 0);
+            c[q] = u[x[q]]; w[q] = u[x[q] - 1]; e[q] = u[x[q] + 1];
+            n[q] = u[x[q] + W]; so[q] = u[x[q] - W]; bb[q] = b[xb[q]];
+          }
+#pragma unroll
+          for (int q = 0; q < NB; ++q)
+            if (act[q]) u[x[q]] = sor_update(c[q], bb[q], w[q], n[q], so[q], e[q], aw, an, ac, omega);
+        }
+      }
+      __syncthreads();
+    }
+    double r = 0.0;
+    for (int j0 = 1 + wv; j0 <= n2 + 1; j0 += NB * nw)
+      for (int i = 1 + lane; i <= n1 + 1; i += 64) {
+        int x[NB], xb[NB];
+        bool act[NB];
+        T c[NB], w[NB], e[NB], n[NB], so[NB], bb[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+          const int j = j0 + q * nw;
+          act[q] = j <= n2 + 1;
+          x[q] = act[q] ? j * W + i : W + 1;
+          xb[q] = act[q] ? (j - 1) * (n1 + 1) + (i - 1) : 0;
+          const int8_t mq = bc[x[q]];
+          act[q] = act[q] & (mq == -1);
+          c[q] = u[x[q]]; w[q] = u[x[q] - 1]; e[q] = u[x[q] + 1];
+          n[q] = u[x[q] + W]; so[q] = u[x[q] - W]; bb[q] = b[xb[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          if (act[q]) r += fabs(sor_resid(c[q], bb[q], w[q], n[q], so[q], e[q], aw, an, ac));
+      }
+    res = block_sum_1024(r, ws);
+    ksor += 1;
+    bool stop;
+    if (dig == 0) stop = res <= eps;
+    else stop = res_s > 0.0 && (res / res_s) <= 0.6;
+    if (stop) break;
+    res_s = res;
+  }
+  *ksor_out = ksor;
+  *res_out = res;
+}
+
+template <typename T>
+__device__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps, int dig,
+                             int nr, double* ws, int* ks, double* res) {
+  if (L.lds)
+    t_smooth<T>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
+                eps, dig, nr, ws, ks, res);
+  else
+    t_smooth<T>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
+                eps, dig, nr, ws, ks, res);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ double ws[16];
+  __shared__ T* sU[kMaxLevels + 1];
+  __shared__ T* sB[kMaxLevels + 1];
+  __shared__ const int8_t* sBC[kMaxLevels + 1];
+  const int Ls = A.Ls, ng = A.ng;
+  if (threadIdx.x == 0) {
+    for (int l = Ls; l <= ng; ++l) {
+      const TLevel<T>& L = A.lv[l];
+      sU[l] = L.lds ? (T*)(smem + L.ou) : L.gu;
+      sB[l] = L.lds ? (T*)(smem + L.ob) : L.gb;
+      sBC[l] = L.lds ? (const int8_t*)(smem + L.obc) : L.gbc;
+    }
+  }
+  __syncthreads();
+  for (int l = Ls; l <= ng; ++l) {
+    const TLevel<T>& L = A.lv[l];
+    if (!L.lds) continue;
+    int8_t* bc = (int8_t*)(smem + L.obc);
+    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) bc[q] = L.gbc[q];
+    if (l == Ls) {
+      for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) sU[l][q] = L.gu[q];
+      for (int q = threadIdx.x; q < (L.n1 + 1) * (L.n2 + 1); q += blockDim.x) sB[l][q] = L.gb[q];
+    }
+  }
+  __syncthreads();
+  int ks;
+  double res = 0.0;
+  for (int l = Ls; l < ng; ++l) {  // down leg: ig = l
+    const TLevel<T>& F = A.lv[l];
+    const TLevel<T>& C = A.lv[l + 1];
+    smooth_level(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
+    if (threadIdx.x == 0) { A.visit[l] = ks; A.vres[l] = res; }
+    // restriction: coarse rhs and zero coarse correction
+    T* e2h = sU[l + 1];
+    T* r2h = sB[l + 1];
+    for (int q = threadIdx.x; q < (C.n1 + 3) * (C.n2 + 3); q += blockDim.x) e2h[q] = (T)0;
+    for (int q = threadIdx.x; q < (C.n1 + 1) * (C.n2 + 1); q += blockDim.x) {
+      const int i = q % (C.n1 + 1) + 1, j = q / (C.n1 + 1) + 1;
+      T out = (T)0;
+      if (sBC[l + 1][j * (C.n1 + 3) + i] <= 0)
+        out = restrict_node(sU[l], sB[l], sBC[l], i, j, F.n1, F.n2, F.aw, F.an, F.ac);
+      r2h[q] = out;
+    }
+    __syncthreads();
+  }
+  {  // coarsest: ig = ng; the first V-cycle iterates to eps (sors.f90:63-64)
+    const TLevel<T>& L = A.lv[ng];
+    smooth_level(L, sU[ng], sB[ng], sBC[ng], A.omega, A.eps, 0, A.k == 1 ? 1000000 : kNr, ws, &ks, &res);
+    if (threadIdx.x == 0) { A.visit[ng] = ks; A.vres[ng] = res; }
+  }
+  for (int l = ng - 1; l >= Ls; --l) {  // up leg: ig = 2ng - l
+    const TLevel<T>& F = A.lv[l];
+    const TLevel<T>& C = A.lv[l + 1];
+    for (int q = threadIdx.x; q < (C.n1 + 1) * (C.n2 + 1); q += blockDim.x)
+      prolong_node(sU[l], sBC[l], F.n1, (const T*)sU[l + 1], C.n1, q % (C.n1 + 1) + 1, q / (C.n1 + 1) + 1);
+    __syncthreads();
+    smooth_level(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
+    if (threadIdx.x == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
+  }
+  const TLevel<T>& L = A.lv[Ls];
+  if (L.lds)
+    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) L.gu[q] = sU[Ls][q];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+T* rot_of(Level& L, int r) { return (T*)L.rot[r]; }
+
+template <typename T>
+T* u_of(Level& L, bool f32) { return f32 ? (T*)L.uf : (T*)L.u; }
+
+template <typename T>
+T* b_of(Level& L, bool f32) { return f32 ? (T*)L.bf : (T*)L.b; }
+
+int* d_visit_of(lepic_ctx* ctx) { return (int*)(ctx->d_scalar + 64); }
+double* d_vres_of(lepic_ctx* ctx) { return ctx->d_scalar + 32; }
+int* d_status_of(lepic_ctx* ctx) { return (int*)(ctx->d_scalar + 64) + kVisitLog; }
+
+VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, double eps, int nexec) {
+  Level& L = ctx->lv[l];
+  VisitRef v;
+  v.ctrl = ctx->d_ctrl + l;
+  v.partials = L.partials;
+  v.nb = L.nbx * L.nby;
+  v.epoch = epoch; v.dig = dig; v.nr = nr; v.ig = ig; v.eps = eps;
+  v.nexec = nexec;
+  v.klast0 = ((nexec - 1) / kS) * kS;
+  v.visit = d_visit_of(ctx);
+  v.vres = d_vres_of(ctx);
+  return v;
+}
+
+// launches of one visit of a multi-block level: sweeps k0+1 .. k_end
+template <typename T>
+int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end) {
+  Level& L = ctx->lv[l];
+  SweepArgs<T> A;
+  for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<T>(L, r);
+  A.b = b_of<T>(L, f32);
+  A.bc = L.bc;
+  A.n1 = L.n1; A.n2 = L.n2;
+  A.aw = (T)L.aw; A.an = (T)L.an; A.ac = (T)L.ac; A.omega = (T)1.4;
+  A.ctrl = ctx->d_ctrl + l;
+  A.partials = L.partials;
+  A.nbx = L.nbx; A.nby = L.nby;
+  A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
+  for (int k = k0; k < k_end; k += kS) {
+    const int kexec = std::min(kS, k_end - k);
+    hipLaunchKernelGGL(k_sweeps<T>, dim3(L.nbx, L.nby), dim3(kSweepThreads), 0, ctx->stream, A, k, kexec);
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
 
 template <typename T>
-int launch_coarse(lepic_ctx* ctx, bool f32, int k, double eps, int* visit, double* vres) {
-  CoarseArgs<T> A{};
-  A.Lc = ctx->Lc; A.ng = ctx->ng; A.k = k; A.omega = (T)1.4; A.eps = eps;
-  A.visit = visit; A.vres = vres;
-  uint32_t off = 0;
-  auto al = [](uint32_t x) { return (x + 15u) & ~15u; };
-  for (int l = ctx->Lc; l <= ctx->ng; ++l) {
+int launch_restrict(lepic_ctx* ctx, int l, bool f32, int epoch, double eps) {
+  const int ng = ctx->ng;
+  Level& F = ctx->lv[l];
+  Level& C = ctx->lv[l + 1];
+  RestrictArgs<T> A;
+  A.vf = visit_ref(ctx, l, epoch, l - ng, kNr, l, eps, kNr);
+  for (int r = 0; r < kRot; ++r) A.frot[r] = rot_of<T>(F, r);
+  A.fb = b_of<T>(F, f32);
+  A.fbc = F.bc;
+  A.n1 = F.n1; A.n2 = F.n2;
+  A.aw = (T)F.aw; A.an = (T)F.an; A.ac = (T)F.ac;
+  A.r2h = b_of<T>(C, f32);
+  A.bc2h = C.bc;
+  for (int r = 0; r < kRot; ++r) A.crot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
+  A.cctrl = C.mb ? ctx->d_ctrl + (l + 1) : nullptr;
+  A.cepoch = epoch;
+  dim3 g((C.n1 + 3 + 127) / 128, std::min(C.n2 + 3, kRowBlocks));
+  hipLaunchKernelGGL(k_restrict<T>, g, dim3(128), 0, ctx->stream, A);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_prolong(lepic_ctx* ctx, int l, bool f32, int epoch_up, double eps) {
+  const int ng = ctx->ng;
+  Level& F = ctx->lv[l];
+  Level& C = ctx->lv[l + 1];
+  ProlongArgs<T> A;
+  for (int r = 0; r < kRot; ++r) A.frot[r] = rot_of<T>(F, r);
+  A.fctrl = ctx->d_ctrl + l;
+  A.fepoch = epoch_up;
+  A.bc = F.bc;
+  A.n1 = F.n1; A.n2 = F.n2;
+  for (int r = 0; r < kRot; ++r) A.crot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
+  A.coarse_mb = C.mb ? 1 : 0;
+  if (C.mb) A.vc = visit_ref(ctx, l + 1, epoch_up, ng - (l + 1), kNr, 2 * ng - (l + 1), eps, kNr);
+  else A.vc = VisitRef{};
+  dim3 g((C.n1 + 1 + 127) / 128, std::min(C.n2 + 1, kRowBlocks));
+  hipLaunchKernelGGL(k_prolong<T>, g, dim3(128), 0, ctx->stream, A);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
+  TailArgs<T> A{};
+  A.Ls = ctx->Lc; A.ng = ctx->ng; A.k = k; A.omega = (T)1.4; A.eps = eps;
+  A.visit = d_visit_of(ctx);
+  A.vres = d_vres_of(ctx);
+  auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  size_t off = 0;
+  bool fits = true;
+  for (int l = ctx->ng; l >= ctx->Lc; --l) {  // coarsest first into LDS
     Level& L = ctx->lv[l];
-    CLevel<T>& C = A.lv[l];
+    TLevel<T>& C = A.lv[l];
     C.n1 = L.n1; C.n2 = L.n2; C.aw = (T)L.aw; C.an = (T)L.an; C.ac = (T)L.ac;
-    C.ou = off; off = al(off + (uint32_t)((L.n1 + 3) * (L.n2 + 3) * sizeof(T)));
-    C.ob = off; off = al(off + (uint32_t)((L.n1 + 1) * (L.n2 + 1) * sizeof(T)));
-    C.obc = off; off = al(off + (uint32_t)((L.n1 + 3) * (L.n2 + 3)));
-    C.gbc = L.bc;
+    C.gu = u_of<T>(L, f32); C.gb = b_of<T>(L, f32); C.gbc = L.bc;
+    const size_t need = al((size_t)(L.n1 + 3) * (L.n2 + 3) * sizeof(T)) +
+                        al((size_t)(L.n1 + 1) * (L.n2 + 1) * sizeof(T)) + al((size_t)(L.n1 + 3) * (L.n2 + 3));
+    C.lds = 0;
+    if (fits && off + need <= kTailLds) {
+      C.lds = 1;
+      C.ou = (uint32_t)off; off = al(off + (size_t)(L.n1 + 3) * (L.n2 + 3) * sizeof(T));
+      C.ob = (uint32_t)off; off = al(off + (size_t)(L.n1 + 1) * (L.n2 + 1) * sizeof(T));
+      C.obc = (uint32_t)off; off = al(off + (size_t)(L.n1 + 3) * (L.n2 + 3));
+    } else {
+      fits = false;  // finer levels stay in global memory
+    }
   }
-  if (off > kCoarseLds) return ctx->fail(LEPIC_ERR_ARG, "coarse levels exceed the LDS budget");
-  Level& L = ctx->lv[ctx->Lc];
-  A.gu = f32 ? (T*)L.uf : (T*)L.u;
-  A.gb = f32 ? (const T*)L.bf : (const T*)L.b;
-  hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(kCoarseThreads), off, ctx->stream, A);
+  hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), off, ctx->stream, A);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
 
-// One V-cycle (mg.f90:85-225) at precision T.  Levels below Lc use the
-// multi-block kernels, Lc..ng the single-workgroup kernel.
+// One V-cycle (mg.f90:85-225) at precision T: levels 1..Ls-1 multi-block,
+// Ls..ng in k_tail; then the end-of-cycle kernel of the mode.
 template <typename T>
-int vcycle(lepic_ctx* ctx, bool f32, int k, double eps, int* visit, double* vres) {
-  const int ng = ctx->ng, Lc = ctx->Lc;
-  for (int l = 1; l < Lc; ++l) {  // down leg
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl + l);
-    if (int rc = launch_sweeps<T>(ctx, l, f32, l - ng, eps, 10, l, 10, visit, vres)) return rc;
-    Lv<T> F = lv_of<T>(ctx, l, f32);
-    Level& C = ctx->lv[l + 1];
-    T* r2 = f32 ? (T*)C.bf : (T*)C.b;
-    T* e2 = f32 ? (T*)C.uf : (T*)C.u;
-    dim3 g((C.n1 + 3 + 127) / 128, C.n2 + 3);
-    hipLaunchKernelGGL(k_restrict<T>, g, dim3(128), 0, ctx->stream, F, r2, e2, (const int8_t*)C.bc);
+int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
+  const int ng = ctx->ng, Ls = ctx->Lc;
+  const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
+  ctx->mg_epoch += 2;
+  for (int l = 1; l < Ls; ++l) {  // down leg
+    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, kNr)) return rc;
+    if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
   }
-  if (int rc = launch_coarse<T>(ctx, f32, k, eps, visit, vres)) return rc;
-  for (int l = Lc - 1; l >= 1; --l) {  // up leg
-    Level& F = ctx->lv[l];
-    Level& C = ctx->lv[l + 1];
-    T* u = f32 ? (T*)F.uf : (T*)F.u;
-    const T* e2 = f32 ? (const T*)C.uf : (const T*)C.u;
-    dim3 g((C.n1 + 1 + 127) / 128, C.n2 + 1);
-    hipLaunchKernelGGL(k_prolong<T>, g, dim3(128), 0, ctx->stream, u, (const int8_t*)F.bc, F.n1, F.n2, e2);
-    hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl + l);
-    if (int rc = launch_sweeps<T>(ctx, l, f32, ng - l, eps, 10, 2 * ng - l, 10, visit, vres)) return rc;
+  if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
+  for (int l = Ls - 1; l >= 1; --l) {  // up leg
+    if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
+    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, kNr)) return rc;
+  }
+  Level& L1 = ctx->lv[1];
+  const unsigned gx = (unsigned)((L1.n1 + 1 + 255) / 256);
+  if (!f32) {
+    if (L1.mb) {
+      Finish64Args A;
+      A.v = visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr);
+      for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<double>(L1, r);
+      A.n1 = L1.n1; A.n2 = L1.n2;
+      A.status = d_status_of(ctx);
+      hipLaunchKernelGGL(k_finish64, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
+    }
+  } else {
+    AddCorrArgs A;
+    A.phi = ctx->phi;
+    for (int r = 0; r < kRot; ++r) A.rot[r] = L1.mb ? rot_of<float>(L1, r) : L1.uf;
+    A.bc = ctx->bc;
+    A.n1 = L1.n1; A.n2 = L1.n2;
+    A.mb = L1.mb ? 1 : 0;
+    A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr) : VisitRef{};
+    hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
   }
   LEPIC_HIP(hipGetLastError());
-  return 0;
-}
-
-int read_back(lepic_ctx* ctx, int* visit_h, double* vres_h, int* d_visit, double* d_vres) {
-  LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, d_vres, 8, hipMemcpyDeviceToHost, ctx->stream));
-  LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned + 1, d_visit, kVisitLog * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  std::memcpy(vres_h, ctx->h_pinned, 8);
-  std::memcpy(visit_h, ctx->h_pinned + 1, kVisitLog * sizeof(int));
   return 0;
 }
 
@@ -541,20 +879,41 @@ double alpha_of(lepic_ctx* ctx) {
   return std::fabs((double)((ctx->g.n1 + 1) * (ctx->g.n2 + 1)) * ctx->lv[1].ac) * 1.0;
 }
 
-int resid64(lepic_ctx* ctx, bool fill_f32, double* res_h) {
+int resid_blocks(lepic_ctx* ctx) { return std::min(ctx->g.n2 + 1, kResidBlocks); }
+
+// FP64 residual of phi; fills the FP32 right-hand side (and clears the
+// correction the next visit starts from) when fill_f32
+int launch_resid64(lepic_ctx* ctx, bool fill_f32) {
   Level& L = ctx->lv[1];
-  dim3 g((L.n1 + 1 + 255) / 256, L.n2 + 1);
-  double* out = ctx->d_scalar;
-  unsigned* arrive = (unsigned*)(ctx->d_scalar + 8);
-  hipLaunchKernelGGL(k_resid64, g, dim3(256), 0, ctx->stream, (const double*)ctx->phi,
+  float* const* erot = nullptr;
+  if (fill_f32) erot = (float* const*)ctx->d_erot;
+  hipLaunchKernelGGL(k_resid64, dim3(resid_blocks(ctx)), dim3(256), 0, ctx->stream, (const double*)ctx->phi,
                      (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2, L.aw, L.an, L.ac,
-                     fill_f32 ? L.bf : nullptr, fill_f32 ? L.uf : nullptr, ctx->d_partials, arrive, out);
+                     fill_f32 ? L.bf : nullptr, erot, L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr,
+                     ctx->mg_epoch + 1, ctx->d_partials);
   LEPIC_HIP(hipGetLastError());
-  if (res_h) {
-    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, out, 8, hipMemcpyDeviceToHost, ctx->stream));
-    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-    std::memcpy(res_h, ctx->h_pinned, 8);
-  }
+  return 0;
+}
+
+double sum_resid_partials(const double* p, int nb) {
+  double s = 0.0;
+  for (int q = 0; q < nb; ++q) s += p[q];
+  return s;
+}
+
+// read back the visit log, the per-visit residuals and (mixed) the FP64
+// residual partial sums of the finest level
+int read_back(lepic_ctx* ctx, int* visit, double* vres, double* resid_sum) {
+  double* h = ctx->h_pinned;
+  LEPIC_HIP(hipMemcpyAsync(h, ctx->d_scalar + 32, 32 * 8 + (kVisitLog + 1) * sizeof(int),
+                           hipMemcpyDeviceToHost, ctx->stream));
+  const int nb = resid_blocks(ctx);
+  if (resid_sum)
+    LEPIC_HIP(hipMemcpyAsync(h + 128, ctx->d_partials, nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(vres, h, kVisitLog * 8);
+  std::memcpy(visit, (char*)h + 32 * 8, kVisitLog * sizeof(int));
+  if (resid_sum) *resid_sum = sum_resid_partials(h + 128, nb);
   return 0;
 }
 
@@ -565,6 +924,7 @@ namespace lepic {
 int poisson_setup(lepic_ctx* ctx) {
   const int ng = ctx->ng;
   const double eps0 = ctx->g.eps0;
+  const bool mixed = ctx->g.precision == LEPIC_MIXED;
   for (int l = 1; l <= ng; ++l) {
     Level& L = ctx->lv[l];
     L.n1 = ctx->g.n1 >> (l - 1);
@@ -588,37 +948,66 @@ int poisson_setup(lepic_ctx* ctx) {
       LEPIC_HIP(hipMalloc(&L.bc, nn));
       LEPIC_HIP(hipMemset(L.bc, 1, nn));
     }
-    if (ctx->g.precision == LEPIC_MIXED) {
+    if (mixed) {
       LEPIC_HIP(hipMalloc(&L.uf, nn * 4));
       LEPIC_HIP(hipMemset(L.uf, 0, nn * 4));
       LEPIC_HIP(hipMalloc(&L.bf, nr * 4));
       LEPIC_HIP(hipMemset(L.bf, 0, nr * 4));
     }
   }
-  // coarse kernel takes the finest level from which all coarser levels fit
-  // in the LDS budget at FP64 (the FP32 copy then fits as well)
-  auto bytes = [&](int l) {
+  // levels with more nodes than the tail threshold run multi-block; the
+  // coarsest level always runs in k_tail (unless ng = 1)
+  const int64_t tail_nodes = ctx->g.mg_tail_nodes > 0 ? ctx->g.mg_tail_nodes : kDefaultTailNodes;
+  ctx->Lc = ng;
+  for (int l = 1; l <= ng; ++l) {
     const Level& L = ctx->lv[l];
-    auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
-    return al((size_t)(L.n1 + 3) * (L.n2 + 3) * 8) + al((size_t)(L.n1 + 1) * (L.n2 + 1) * 8) +
-           al((size_t)(L.n1 + 3) * (L.n2 + 3));
-  };
-  size_t tot = 0;
-  ctx->Lc = ng + 1;
-  for (int l = ng; l >= 1; --l) {
-    if (tot + bytes(l) > kCoarseLds) break;
-    tot += bytes(l);
-    ctx->Lc = l;
+    if ((int64_t)(L.n1 + 1) * (L.n2 + 1) <= tail_nodes) { ctx->Lc = l; break; }
   }
-  if (ctx->Lc > ng && ng > 1) return ctx->fail(LEPIC_ERR_ARG, "coarsest MG level does not fit in LDS");
+  if (ng == 1 && (int64_t)(ctx->lv[1].n1 + 1) * (ctx->lv[1].n2 + 1) > tail_nodes) ctx->Lc = 2;
+  const size_t esz = mixed ? 4 : 8;
+  const int ty = mixed ? TileY<float>::v : TileY<double>::v;
+  for (int l = 1; l < ctx->Lc && l <= ng; ++l) {
+    Level& L = ctx->lv[l];
+    L.mb = true;
+    const size_t nn = (size_t)(L.n1 + 3) * (L.n2 + 3);
+    L.rot[0] = mixed ? (void*)L.uf : (void*)L.u;
+    for (int r = 1; r < kRot; ++r) {
+      LEPIC_HIP(hipMalloc(&L.rot[r], nn * esz));
+      LEPIC_HIP(hipMemset(L.rot[r], 0, nn * esz));
+    }
+    L.nbx = (L.n1 + 1 + kTX - 1) / kTX;
+    L.nby = (L.n2 + 1 + ty - 1) / ty;
+    const size_t np = (size_t)kRing * L.nbx * L.nby;
+    LEPIC_HIP(hipMalloc(&L.partials, np * 8));
+    LEPIC_HIP(hipMemset(L.partials, 0, np * 8));
+  }
+  if (mixed && ctx->lv[1].mb) {
+    LEPIC_HIP(hipMalloc(&ctx->d_erot, kRot * sizeof(float*)));
+    LEPIC_HIP(hipMemcpy(ctx->d_erot, ctx->lv[1].rot, kRot * sizeof(float*), hipMemcpyHostToDevice));
+  } else if (mixed) {
+    std::vector<float*> e(kRot, ctx->lv[1].uf);
+    LEPIC_HIP(hipMalloc(&ctx->d_erot, kRot * sizeof(float*)));
+    LEPIC_HIP(hipMemcpy(ctx->d_erot, e.data(), kRot * sizeof(float*), hipMemcpyHostToDevice));
+  }
   LEPIC_HIP(hipMalloc(&ctx->d_ctrl, (kMaxLevels + 1) * sizeof(LevelCtrl)));
   LEPIC_HIP(hipMemset(ctx->d_ctrl, 0, (kMaxLevels + 1) * sizeof(LevelCtrl)));
-  int64_t maxblocks = (int64_t)((ctx->g.n1 + 1 + 255) / 256 + 1) * (ctx->g.n2 + 1);
-  LEPIC_HIP(hipMalloc(&ctx->d_partials, maxblocks * 8));
-  ctx->partials_cap = maxblocks;
-  LEPIC_HIP(hipMalloc(&ctx->d_scalar, 64 * 8 + kVisitLog * sizeof(int)));
-  LEPIC_HIP(hipMemset(ctx->d_scalar, 0, 64 * 8 + kVisitLog * sizeof(int)));
+  ctx->mg_epoch = 0;
+  LEPIC_HIP(hipMalloc(&ctx->d_partials, kResidBlocks * 8));
+  LEPIC_HIP(hipMemset(ctx->d_partials, 0, kResidBlocks * 8));
+  ctx->partials_cap = kResidBlocks;
+  LEPIC_HIP(hipMalloc(&ctx->d_scalar, 64 * 8 + (kVisitLog + 1) * sizeof(int)));
+  LEPIC_HIP(hipMemset(ctx->d_scalar, 0, 64 * 8 + (kVisitLog + 1) * sizeof(int)));
   return 0;
+}
+
+void poisson_teardown(lepic_ctx* ctx) {
+  for (int l = 1; l <= kMaxLevels; ++l) {
+    Level& L = ctx->lv[l];
+    for (int r = 1; r < kRot; ++r)
+      if (L.rot[r]) (void)hipFree(L.rot[r]);
+    if (L.partials) (void)hipFree(L.partials);
+  }
+  if (ctx->d_erot) (void)hipFree(ctx->d_erot);
 }
 
 int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_info* info,
@@ -627,6 +1016,7 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
   const int64_t n = ctx->nodes;
   const dim3 g1((unsigned)((n + 255) / 256));
+  if (ng == 1 && mixed) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "ng=1 runs in FP64 only");
   if (start == LEPIC_START_COLD)
     hipLaunchKernelGGL(k_cold_start, g1, dim3(256), 0, ctx->stream, ctx->phi, (const int8_t*)ctx->bc, n);
   else if (start == LEPIC_START_GIVEN)
@@ -634,74 +1024,67 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
                        (const double*)ctx->phi_given, (const int8_t*)ctx->bc, n);
   LEPIC_HIP(hipGetLastError());
   const double alpha = alpha_of(ctx);
-  int* d_visit = (int*)(ctx->d_scalar + 64);
-  double* d_vres = ctx->d_scalar + 16;
   int visit[kVisitLog];
-  double vres = 0.0, res = 0.0, res_s = 0.0, ktot = 0.0;
+  double vres[kVisitLog];
+  double res = 0.0, res_s = 0.0, ktot = 0.0;
   info->nlog = 0;
   info->start_used = start;
-  double r0 = 0.0;
   // residual of the starting guess; in mixed mode it also fills the FP32
   // right-hand side of the first correction equation
-  if (int rc = resid64(ctx, mixed && ng > 1, &r0)) return rc;
+  double r0 = 0.0;
+  if (int rc = launch_resid64(ctx, mixed)) return rc;
+  if (int rc = read_back(ctx, visit, vres, &r0)) return rc;
   info->res0 = r0 / alpha;
   int k = 1;
   info->status = 0;
-  if (ng == 1) {
-    // red-black SOR alone (mg.f90:197-198 with ng=1): the first call
-    // iterates to eps (up to 10^6 sweeps), later calls do 10 sweeps.
-    for (;;) {
-      hipLaunchKernelGGL(k_ctrl_reset, dim3(1), dim3(1), 0, ctx->stream, ctx->d_ctrl + 1);
-      const int nr = (k == 1) ? 1000000 : 10;
-      int done_sweeps = 0;
-      if (mixed) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "ng=1 runs in FP64 only");
-      for (;;) {
-        const int batch = std::min(nr - done_sweeps, 64);
-        if (int rc = launch_sweeps<double>(ctx, 1, false, 0, alpha * eps, nr, 1, batch, d_visit, d_vres)) return rc;
-        done_sweeps += batch;
-        LevelCtrl c;
-        LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_ctrl + 1, sizeof(LevelCtrl), hipMemcpyDeviceToHost, ctx->stream));
-        LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-        std::memcpy(&c, ctx->h_pinned, sizeof(c));
-        if (c.done || done_sweeps >= nr) { vres = c.res; visit[1] = c.ksor; break; }
-      }
-      ktot += visit[1];
-      if (log && info->nlog < log_cap) log[info->nlog] = visit[1];
-      info->nlog++;
-      res = vres / alpha;
-      if (k == 1) res_s = res;
-      else if (std::fabs(res / res_s) > 10.0) { info->vcycles = k; return ctx->fail(LEPIC_ERR_DIVERGED, "PDE solver is diverging"); }
-      if (res <= eps || k == ncycl) break;
-      ++k;
-    }
-    info->vcycles = k;
-    info->status = (res <= eps) ? 0 : 1;
-    info->res = res;
-    info->ktot = ktot;
-    return 0;
-  }
   for (;;) {
-    if (!mixed) {
-      if (int rc = vcycle<double>(ctx, false, k, alpha * eps, d_visit, d_vres)) return rc;
-      if (int rc = read_back(ctx, visit, &vres, d_visit, d_vres)) return rc;
+    if (ng == 1 && ctx->lv[1].mb) {
+      // red-black SOR alone on a multi-block grid (mg.f90:197-198 with
+      // ng=1): the first call iterates to eps (up to 10^6 sweeps), later
+      // calls do 10; launches go in batches with a host check in between
+      const int nr = (k == 1) ? 1000000 : kNr;
+      const int e = ++ctx->mg_epoch;
+      int k0 = 0, ks = 0;
+      while (k0 < nr) {
+        const int k_end = std::min(nr, k0 + 16 * kS);
+        if (int rc = launch_visit<double>(ctx, 1, false, e, 0, nr, alpha * eps, k0, k_end)) return rc;
+        k0 = k_end;
+        Level& L1 = ctx->lv[1];
+        Finish64Args A;
+        A.v = visit_ref(ctx, 1, e, 0, nr, 1, alpha * eps, k0);
+        for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<double>(L1, r);
+        A.n1 = L1.n1; A.n2 = L1.n2;
+        A.status = d_status_of(ctx);
+        hipLaunchKernelGGL(k_finish64, dim3((unsigned)((L1.n1 + 1 + 255) / 256), std::min(L1.n2 + 1, kRowBlocks)),
+                           dim3(256), 0, ctx->stream, A);
+        LEPIC_HIP(hipGetLastError());
+        LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, d_status_of(ctx), 4, hipMemcpyDeviceToHost, ctx->stream));
+        LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+        std::memcpy(&ks, ctx->h_pinned, 4);
+        if (ks) break;
+      }
+      if (!ks) return ctx->fail(LEPIC_ERR_STATE, "SOR visit did not finish");
+      if (int rc = read_back(ctx, visit, vres, nullptr)) return rc;
+    } else if (!mixed) {
+      if (int rc = vcycle<double>(ctx, false, k, alpha * eps)) return rc;
+      if (int rc = read_back(ctx, visit, vres, nullptr)) return rc;
     } else {
       // FP32 correction V-cycle on the FP64 residual, then phi += e; the new
       // FP64 residual is both the convergence test and the next right-hand side
-      if (int rc = vcycle<float>(ctx, true, k, alpha * eps, d_visit, d_vres)) return rc;
-      hipLaunchKernelGGL(k_add_correction, g1, dim3(256), 0, ctx->stream, ctx->phi,
-                         (const float*)ctx->lv[1].uf, (const int8_t*)ctx->bc, n);
-      LEPIC_HIP(hipGetLastError());
-      if (int rc = resid64(ctx, true, &vres)) return rc;
-      double dummy;
-      if (int rc = read_back(ctx, visit, &dummy, d_visit, d_vres)) return rc;
+      if (int rc = vcycle<float>(ctx, true, k, alpha * eps)) return rc;
+      if (int rc = launch_resid64(ctx, true)) return rc;
+      double rs = 0.0;
+      if (int rc = read_back(ctx, visit, vres, &rs)) return rc;
+      vres[2 * ng - 1] = rs;
     }
-    for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
+    const int nvis = 2 * ng - 1;
+    for (int ig = 1; ig <= nvis; ++ig) {
       int l = ig < ng ? ig : (ig > ng ? 2 * ng - ig : ng);
       ktot += (double)visit[ig] / std::pow(2.0, 2 * (l - 1));
       if (log && info->nlog < log_cap) log[info->nlog] = visit[ig];
       info->nlog++;
     }
-    res = vres / alpha;
+    res = vres[nvis] / alpha;
     if (k == 1) res_s = res;
     else if (std::fabs(res / res_s) > 10.0) {
       info->vcycles = k;

@@ -14,7 +14,7 @@ module lepic_capi
   type, bind(C) :: lepic_grid_desc
      integer(c_int32_t) :: n1, n2
      real(c_double) :: h1, h2, zmax, dt, eps0, qe
-     integer(c_int32_t) :: ng, ngrid, nspecies, precision, flag_B_pos, tag_neg, track_z, reserved
+     integer(c_int32_t) :: ng, ngrid, nspecies, precision, flag_B_pos, tag_neg, track_z, mg_tail_nodes
   end type lepic_grid_desc
 
   type, bind(C) :: lepic_species_desc

@@ -18,9 +18,10 @@ E_CHG, E_MASS = -O.QE, O.ME
 I_CHG, I_MASS = O.QE, 39.95 * O.AMU
 
 
-def make_ctx(dom, species, ng=1, precision=L.FP64, track_z=1, phi=None):
+def make_ctx(dom, species, ng=1, precision=L.FP64, track_z=1, phi=None, tail=0):
     ctx = L.Context(dom.n1, dom.n2, dom.h1, dom.h2, dom.dt, species, zmax=dom.zmax, ng=ng,
-                    ngrid=dom.ngrid, precision=precision, flag_B_pos=dom.flag_B_pos, track_z=track_z)
+                    ngrid=dom.ngrid, precision=precision, flag_B_pos=dom.flag_B_pos, track_z=track_z,
+                    mg_tail_nodes=tail)
     ctx.set_boundary(dom.bcnd, np.zeros((dom.n2 + 3, dom.n1 + 3)) if phi is None else phi)
     if dom.Bi is not None:
         ctx.set_bfield(dom.Bi, dom.nB, dom.hB)
@@ -116,10 +117,14 @@ def poisson_case(n1, n2, seed):
     return dom, phi, b
 
 
-@pytest.mark.parametrize("n,ng", [(32, 5), (128, 7), (256, 8), (64, 1)])
-def test_poisson_fp64_matches_oracle(n, ng):
+# tail = mg_tail_nodes: 0 keeps the library default (levels up to 20000 nodes in
+# the single-workgroup kernel); 100 puts every level but the last few on the
+# multi-block sweep kernels, including the ng=1 SOR-only solve
+@pytest.mark.parametrize("n,ng,tail", [(32, 5, 0), (128, 7, 0), (256, 8, 0), (64, 1, 0),
+                                       (128, 7, 100), (256, 8, 100), (64, 1, 100)])
+def test_poisson_fp64_matches_oracle(n, ng, tail):
     dom, phi, b = poisson_case(n, n, 27 + n)
-    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, phi=phi) as ctx:
+    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, phi=phi, tail=tail) as ctx:
         ctx.set_grid(L.GRID_RHS, b)
         info = ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
         got = ctx.get_grid(L.GRID_PHI)
@@ -131,16 +136,48 @@ def test_poisson_fp64_matches_oracle(n, ng):
     assert info["res"] <= 1e-10
 
 
-def test_poisson_mixed_within_1e6_of_fp64():
+@pytest.mark.parametrize("tail", [0, 100])
+def test_poisson_fp64_warm_start_sequence(tail):
+    """Three solves in a row, each starting from the previous phi with a new
+    right-hand side (the time loop's pattern), against the oracle doing the
+    same; the state carried between solves must not leak into the next."""
+    n, ng = 128, 7
+    dom, phi, b = poisson_case(n, n, 5)
+    rng = np.random.default_rng(9)
+    ref_phi = phi.copy()
+    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, phi=phi, tail=tail) as ctx:
+        for step in range(3):
+            bb = b * (1.0 + 0.05 * step) + rng.normal(0, 1e-5, b.shape) * (dom.bcnd[1:n + 2, 1:n + 2] == -1)
+            ctx.set_grid(L.GRID_RHS, bb)
+            info = ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
+            got = ctx.get_grid(L.GRID_PHI)
+            ref = O.iterative_solver(ref_phi, bb, dom.bcnd, (dom.h1, dom.h2), n, n, ng, eps=1e-10)
+            ref_phi = ref["phi"]
+            assert info["vcycles"] == ref["vcycles"], step
+            assert np.array_equal(info["sweeps"], ref["sweeps"][:len(info["sweeps"])]), step
+            assert np.abs(got - ref_phi).max() <= 1e-12 * np.abs(ref_phi).max(), step
+
+
+@pytest.mark.parametrize("tail", [0, 100])
+def test_poisson_mixed_within_1e6_of_fp64(tail):
     n, ng = 128, 7
     dom, phi, b = poisson_case(n, n, 31)
-    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, precision=L.MIXED, phi=phi) as ctx:
+    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, precision=L.MIXED, phi=phi,
+                  tail=tail) as ctx:
         ctx.set_grid(L.GRID_RHS, b)
         info = ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
         got = ctx.get_grid(L.GRID_PHI)
+        # a second solve from the converged phi with a changed rhs
+        b2 = b * 1.1
+        ctx.set_grid(L.GRID_RHS, b2)
+        info2 = ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
+        got2 = ctx.get_grid(L.GRID_PHI)
     ref = O.iterative_solver(phi, b, dom.bcnd, (dom.h1, dom.h2), n, n, ng, eps=1e-10)
     assert info["res"] <= 1e-10
     assert np.abs(got - ref["phi"]).max() <= 1e-6 * np.abs(ref["phi"]).max()
+    ref2 = O.iterative_solver(ref["phi"], b2, dom.bcnd, (dom.h1, dom.h2), n, n, ng, eps=1e-10)
+    assert info2["res"] <= 1e-10
+    assert np.abs(got2 - ref2["phi"]).max() <= 1e-6 * np.abs(ref2["phi"]).max()
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
