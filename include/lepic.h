@@ -70,7 +70,7 @@ typedef struct {
   int32_t tag_neg;        /* tagged negative-ion species (1-based) or 0 */
   int32_t track_z;        /* mixed precision only: keep the virtual z coordinate */
   int32_t mg_tail_nodes;  /* MG levels with at most this many nodes ((n1+1)(n2+1)) run in
-                             one single-workgroup kernel; 0 = library default (20000) */
+                             one single-workgroup kernel; 0 = library default (5000) */
 } lepic_grid_desc;
 
 typedef struct {
@@ -108,6 +108,12 @@ const char *lepic_last_error(const lepic_ctx *ctx);
 int lepic_sync(lepic_ctx *ctx);
 /* HIP device index of the context and its stream handle (hipStream_t) */
 int lepic_stream(lepic_ctx *ctx, void **stream);
+/* Measurement hook, no counterpart in the reference: when enabled, each
+ * particle-mover kernel launch is bracketed by HIP events on the context's
+ * stream (the density clearing before it is not); lepic_get_kernel_ms waits
+ * for the last mover kernel of ptype and returns its duration. */
+int lepic_kernel_timing(lepic_ctx *ctx, int32_t enable);
+int lepic_get_kernel_ms(lepic_ctx *ctx, int32_t ptype, float *ms);
 
 /* ---- static inputs ---- */
 /* bcnd and the Dirichlet values of generate_boundary.f90 (phi at walls). */

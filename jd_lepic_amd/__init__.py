@@ -100,6 +100,16 @@ class Context:
     def sync(self):
         self._check(self.lib.lepic_sync(self._ctx))
 
+    def kernel_timing(self, enable=True):
+        """Bracket each mover kernel with HIP events (measurement only)."""
+        self._check(self.lib.lepic_kernel_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_ms(self, ptype):
+        """Duration (ms) of the last timed mover kernel of ptype."""
+        ms = C.c_float()
+        self._check(self.lib.lepic_get_kernel_ms(self._ctx, ptype, C.byref(ms)))
+        return float(ms.value)
+
     # ---- static inputs ----
     def set_boundary(self, bcnd, phi):
         b = np.ascontiguousarray(bcnd, dtype=np.int32)

@@ -76,6 +76,8 @@ SIGNATURES = {
     "lepic_last_error": (C.c_char_p, [_ctxp]),
     "lepic_sync": (C.c_int, [_ctxp]),
     "lepic_stream": (C.c_int, [_ctxp, C.POINTER(C.c_void_p)]),
+    "lepic_kernel_timing": (C.c_int, [_ctxp, C.c_int32]),
+    "lepic_get_kernel_ms": (C.c_int, [_ctxp, C.c_int32, C.POINTER(C.c_float)]),
     "lepic_set_boundary": (C.c_int, [_ctxp, _ip, _dp]),
     "lepic_set_bfield": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp]),
     "lepic_upload_particles": (C.c_int, [_ctxp, C.c_int32, _dp, C.c_int64]),

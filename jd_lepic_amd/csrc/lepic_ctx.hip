@@ -310,6 +310,9 @@ int lepic_destroy(lepic_ctx* ctx) {
   if (!ctx) return 0;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   poisson_teardown(ctx);
+  for (int p = 0; p < kMaxSpecies; ++p)
+    for (int k = 0; k < 2; ++k)
+      if (ctx->ev_mover[p][k]) (void)hipEventDestroy(ctx->ev_mover[p][k]);
   for (int p = 0; p < ctx->nsp; ++p) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);
@@ -352,6 +355,23 @@ int lepic_sync(lepic_ctx* ctx) {
 int lepic_stream(lepic_ctx* ctx, void** stream) {
   if (!ctx || !stream) return LEPIC_ERR_ARG;
   *stream = (void*)ctx->stream;
+  return 0;
+}
+
+int lepic_kernel_timing(lepic_ctx* ctx, int32_t enable) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (enable && !ctx->ev_mover[0][0])
+    for (int p = 0; p < kMaxSpecies; ++p)
+      for (int k = 0; k < 2; ++k) LEPIC_HIP(hipEventCreate(&ctx->ev_mover[p][k]));
+  ctx->timing = enable != 0;
+  return 0;
+}
+
+int lepic_get_kernel_ms(lepic_ctx* ctx, int32_t ptype, float* ms) {
+  if (!ctx || !ms || ptype < 1 || ptype > ctx->nsp) return LEPIC_ERR_ARG;
+  if (!ctx->ev_recorded[ptype - 1]) return ctx->fail(LEPIC_ERR_STATE, "no timed mover launch for this species");
+  LEPIC_HIP(hipEventSynchronize(ctx->ev_mover[ptype - 1][1]));
+  LEPIC_HIP(hipEventElapsedTime(ms, ctx->ev_mover[ptype - 1][0], ctx->ev_mover[ptype - 1][1]));
   return 0;
 }
 

@@ -143,8 +143,11 @@ struct lepic_ctx {
   uint64_t* lb_status = nullptr;     // look-back status words
   int64_t lb_cap = 0;
   uint32_t lb_epoch = 0;
-  unsigned long long* d_ticket = nullptr;
-  unsigned long long ticket_base = 0;
+  unsigned long long* d_ticket = nullptr;   // monotonic block ticket of the movers
+  unsigned long long ticket_base = 0;       // tickets taken by earlier launches
+  bool timing = false;                      // lepic_kernel_timing
+  hipEvent_t ev_mover[lepic::kMaxSpecies][2] = {};
+  bool ev_recorded[lepic::kMaxSpecies] = {};
   int32_t* d_error = nullptr;        // device-side error flags
 
   // sort scratch
@@ -205,37 +208,51 @@ __device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint6
   return ((uint64_t)(epoch & 0xFFFFFFu) << 40) | ((uint64_t)flag << 38) | (v & ((1ull << 38) - 1));
 }
 
-// Exclusive prefix of block `bid` given its aggregate; thread-0 only.
+// Exclusive prefix of block `bid` given its aggregate (decoupled look-back).
+// Called by all 64 lanes of one wave: each round reads the status words of
+// the 64 preceding blocks at once, so a walk over blocks that have only
+// published their aggregates costs one memory round trip per 64 blocks.
+// Waits are bounded: on timeout the error flag is set and the walk ends.
 __device__ inline int64_t lookback_exclusive(uint64_t* status, int64_t bid, uint64_t aggregate,
                                              uint32_t epoch, int32_t* err) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t ep = epoch & 0xFFFFFFu;
   if (bid == 0) {
-    __hip_atomic_store(&status[0], lb_pack(epoch, 2, aggregate), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
+    if (lane == 0)
+      __hip_atomic_store(&status[0], lb_pack(epoch, 2, aggregate), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return 0;
   }
-  __hip_atomic_store(&status[bid], lb_pack(epoch, 1, aggregate), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[bid], lb_pack(epoch, 1, aggregate), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   uint64_t excl = 0;
   int64_t j = bid - 1;
   uint32_t spins = 0;
-  while (j >= 0) {
-    uint64_t s = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t ep = (uint32_t)(s >> 40);
-    uint32_t fl = (uint32_t)(s >> 38) & 3u;
-    if (ep != (epoch & 0xFFFFFFu) || fl == 0) {
+  for (;;) {
+    const int64_t q = j - lane;
+    uint64_t sw = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         : lb_pack(epoch, 2, 0);  // before block 0: inclusive zero
+    const uint32_t fl = (uint32_t)(sw >> 38) & 3u;
+    const bool ready = (uint32_t)(sw >> 40) == ep && fl != 0;
+    const uint64_t incl = __ballot(ready && fl == 2);
+    const int pos = incl ? __builtin_ctzll(incl) : 64;        // nearest inclusive prefix
+    const uint64_t need = pos >= 63 ? ~0ull : ((2ull << pos) - 1ull);
+    if (__ballot(!ready) & need) {
       if (++spins > (1u << 22)) {  // bounded wait: report and give up
-        atomicOr(err, 1);
+        if (lane == 0) atomicOr(err, 1);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl += s & ((1ull << 38) - 1);
-    if (fl == 2) break;
-    --j;
+    unsigned long long v = lane <= pos ? (sw & ((1ull << 38) - 1)) : 0ull;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    excl += v;
+    if (pos < 64) break;
+    j -= 64;
   }
-  __hip_atomic_store(&status[bid], lb_pack(epoch, 2, excl + aggregate), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+  if (lane == 0)
+    __hip_atomic_store(&status[bid], lb_pack(epoch, 2, excl + aggregate), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
   return (int64_t)excl;
 }
 

@@ -44,7 +44,7 @@ constexpr int kS = kSweepsPerLaunch;
 constexpr int kHalo = 2 * kS + 1;             // validity shrinks by one node per colour pass
 constexpr int kTX = 64;
 constexpr int kSweepThreads = 1024;
-constexpr int kDefaultTailNodes = 20000;
+constexpr int kDefaultTailNodes = 5000;
 constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 constexpr int kRowBlocks = 128;               // row-loop blocks of the grid-transfer kernels

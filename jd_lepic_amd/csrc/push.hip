@@ -1,8 +1,8 @@
 // push.hip -- particle mover (part_expmover.f90:1-517) fused with charge
 // deposition (part_expmover.f90:520-610), FP64 and mixed precision.
 //
-// One block handles kChunk consecutive particles (thread t, item k ->
-// particle base + k*256 + t, so loads are coalesced).  Survivors are
+// One block handles 4 x (threads per block) consecutive particles (thread t,
+// item k -> particle base + k*threads + t, so loads are coalesced).  Survivors are
 // compacted in particle order (part_expmover.f90:495-503) through a block
 // ballot scan and a decoupled look-back across blocks, writing to the other
 // particle buffer.  Deposition reduces equal-cell runs inside each wave
@@ -23,6 +23,10 @@ namespace {
 constexpr int kWinRows = 4;
 constexpr int kWinCols = 128;
 constexpr int kWin = kWinRows * kWinCols;
+// threads per mover block: the decoupled look-back costs a few microseconds of
+// latency per block, so blocks take 4 x threads particles each
+constexpr int kMoverThreadsMixed = 1024;
+constexpr int kMoverThreads64 = 512;
 
 struct MoverCommon {
   const int8_t* bc;
@@ -145,9 +149,10 @@ __device__ inline void tally_dead(const MoverCommon& a, double Eki) {
 
 // Block-wide order-preserving offsets of survivors; returns the global base
 // through *gbase and each item's local offset in off[].
+template <int BT>
 __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const bool surv[kItems],
                                        uint32_t off[kItems], int64_t* gbase) {
-  __shared__ uint32_t s_cnt[kItems][kBlock / 64];
+  __shared__ uint32_t s_cnt[kItems][BT / 64];
   __shared__ int64_t s_base;
   const int w = threadIdx.x >> 6;
   uint64_t mask[kItems];
@@ -161,16 +166,18 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const 
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     uint32_t before = run;
-    for (int v = 0; v < kBlock / 64; ++v) {
+    for (int v = 0; v < BT / 64; ++v) {
       if (v < w) before += s_cnt[k][v];
       run += s_cnt[k][v];
     }
     off[k] = before + (uint32_t)__popcll(mask[k] & lanemask_lt());
   }
-  if (threadIdx.x == 0) {
-    int64_t excl = bid < a.nblocks ? lookback_exclusive(a.status, bid, run, a.epoch, a.err) : 0;
-    s_base = excl;
-    if (bid == a.nblocks - 1) *a.n_out = excl + run;
+  if (threadIdx.x < 64) {
+    const int64_t excl = bid < a.nblocks ? lookback_exclusive(a.status, bid, run, a.epoch, a.err) : 0;
+    if (threadIdx.x == 0) {
+      s_base = excl;
+      if (bid == a.nblocks - 1) *a.n_out = excl + run;
+    }
   }
   __syncthreads();
   *gbase = s_base;
@@ -179,7 +186,8 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const 
 // ---------------------------------------------------------------------------
 // FP64 mover
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_mover64(MoverCommon a, Soa64 in, Soa64W out,
+template <int BT>
+__global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W out,
                                                     int8_t* flag_dead, const double* __restrict__ E,
                                                     const double* __restrict__ B) {
   __shared__ int64_t s_bid;
@@ -189,13 +197,13 @@ __global__ __launch_bounds__(kBlock) void k_mover64(MoverCommon a, Soa64 in, Soa
   __syncthreads();
   const int64_t bid = s_bid;
   const int64_t N = *a.n_in;
-  const int64_t base = bid * kChunk;
+  const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const double h1 = a.h1, h2 = a.h2, dt = a.dt, k1 = a.k1;
 
   if (a.deposit) {
-    for (int e = t; e < kWin; e += kBlock) win[e] = 0.0;
+    for (int e = t; e < kWin; e += BT) win[e] = 0.0;
     if (t == 0) {
       int ix0 = 1, iy0 = 1;
       if (base < N) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(kBlock) void k_mover64(MoverCommon a, Soa64 in, Soa
   bool surv[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const int64_t i = base + (int64_t)k * kBlock + t;
+    const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
     if (i >= N) continue;
     double xp = in.x[i], yp = in.y[i], zp = in.z[i];
@@ -294,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void k_mover64(MoverCommon a, Soa64 in, Soa
 
   uint32_t off[kItems];
   int64_t gbase;
-  compact_offsets(a, bid, surv, off, &gbase);  // contains __syncthreads
+  compact_offsets<BT>(a, bid, surv, off, &gbase);  // contains __syncthreads
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (!surv[k]) continue;
@@ -338,7 +346,8 @@ struct MixedParams {
   float hB1r, hB2r;   // h/hB ratios for a B map on another grid
 };
 
-__global__ __launch_bounds__(kBlock) void k_mover_mixed(MoverCommon a, MixedParams m, SoaMx in,
+template <int BT>
+__global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m, SoaMx in,
                                                         SoaMxW out, int8_t* flag_dead,
                                                         const float* __restrict__ E,
                                                         const float* __restrict__ B) {
@@ -349,12 +358,12 @@ __global__ __launch_bounds__(kBlock) void k_mover_mixed(MoverCommon a, MixedPara
   __syncthreads();
   const int64_t bid = s_bid;
   const int64_t N = *a.n_in;
-  const int64_t base = bid * kChunk;
+  const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const float k1 = m.k1f;
   if (a.deposit) {
-    for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
+    for (int e = t; e < kWin; e += BT) win[e] = 0.0f;
     if (t == 0) {
       int c0 = base < N ? in.cell[base] : ((1 << 16) | 1);
       int ix0 = c0 & 0xFFFF, iy0 = c0 >> 16;
@@ -367,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_mover_mixed(MoverCommon a, MixedPara
   bool surv[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const int64_t i = base + (int64_t)k * kBlock + t;
+    const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
     if (i >= N) continue;
     int c = in.cell[i];
@@ -459,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_mover_mixed(MoverCommon a, MixedPara
   }
   uint32_t off[kItems];
   int64_t gbase;
-  compact_offsets(a, bid, surv, off, &gbase);
+  compact_offsets<BT>(a, bid, surv, off, &gbase);
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (!surv[k]) continue;
@@ -601,18 +610,21 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   Species& s = ctx->sp[p];
   MoverCommon a = common_args(ctx, p);
   a.deposit = deposit ? 1 : 0;
-  const int64_t nb = std::max<int64_t>((s.ub + kChunk - 1) / kChunk, 1);
+  const bool mixed = ctx->g.precision == LEPIC_MIXED;
+  const int64_t chunk = (int64_t)(mixed ? kMoverThreadsMixed : kMoverThreads64) * kItems;
+  const int64_t nb = std::max<int64_t>((s.ub + chunk - 1) / chunk, 1);
   if (int rc = ensure_lookback(ctx, nb)) return rc;
   a.status = ctx->lb_status;
   a.epoch = ctx->lb_epoch;
-  if (int rc = zero_async(ctx, ctx->d_ticket, 8)) return rc;
-  a.ticket = ctx->d_ticket;
-  a.ticket_base = 0;
+  a.ticket = ctx->d_ticket;           // monotonic: this launch's blocks take the next nb tickets
+  a.ticket_base = ctx->ticket_base;
+  ctx->ticket_base += (unsigned long long)nb;
   a.nblocks = nb;
   if (deposit)
     if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
   const int b = s.cur, o = 1 - s.cur;
-  if (ctx->g.precision == LEPIC_MIXED) {
+  if (ctx->timing) LEPIC_HIP(hipEventRecord(ctx->ev_mover[p][0], ctx->stream));
+  if (mixed) {
     MixedParams m;
     m.k1f = (float)a.k1;
     m.dtoh1 = (float)(ctx->g.dt / ctx->g.h1);
@@ -622,15 +634,21 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
     m.hB2r = ctx->flag_gridB ? (float)(ctx->g.h2 / ctx->hB2) : 1.0f;
     SoaMx in{s.cell[b], s.f[b][0], s.f[b][1], s.f[b][2], s.f[b][3], s.f[b][4], s.f[b][5]};
     SoaMxW out{s.cell[o], s.f[o][0], s.f[o][1], s.f[o][2], s.f[o][3], s.f[o][4], s.f[o][5]};
-    hipLaunchKernelGGL(k_mover_mixed, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, m, in, out,
+    hipLaunchKernelGGL(k_mover_mixed<kMoverThreadsMixed>, dim3((unsigned)nb), dim3(kMoverThreadsMixed), 0,
+                       ctx->stream, a, m, in, out,
                        s.flag_dead, (const float*)ctx->Ef, (const float*)(ctx->Bf ? ctx->Bf : (float*)ctx->Ef));
   } else {
     Soa64 in{s.d[b][0], s.d[b][1], s.d[b][2], s.d[b][3], s.d[b][4], s.d[b][5]};
     Soa64W out{s.d[o][0], s.d[o][1], s.d[o][2], s.d[o][3], s.d[o][4], s.d[o][5]};
-    hipLaunchKernelGGL(k_mover64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, in, out,
+    hipLaunchKernelGGL(k_mover64<kMoverThreads64>, dim3((unsigned)nb), dim3(kMoverThreads64), 0, ctx->stream,
+                       a, in, out,
                        s.flag_dead, (const double*)ctx->E, (const double*)ctx->Bi);
   }
   LEPIC_HIP(hipGetLastError());
+  if (ctx->timing) {
+    LEPIC_HIP(hipEventRecord(ctx->ev_mover[p][1], ctx->stream));
+    ctx->ev_recorded[p] = true;
+  }
   s.cur = o;
   return 0;
 }

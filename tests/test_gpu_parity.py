@@ -117,7 +117,7 @@ def poisson_case(n1, n2, seed):
     return dom, phi, b
 
 
-# tail = mg_tail_nodes: 0 keeps the library default (levels up to 20000 nodes in
+# tail = mg_tail_nodes: 0 keeps the library default (levels up to 5000 nodes in
 # the single-workgroup kernel); 100 puts every level but the last few on the
 # multi-block sweep kernels, including the ng=1 SOR-only solve
 @pytest.mark.parametrize("n,ng,tail", [(32, 5, 0), (128, 7, 0), (256, 8, 0), (64, 1, 0),
