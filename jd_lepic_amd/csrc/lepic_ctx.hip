@@ -78,6 +78,49 @@ __global__ void k_mixed_to_aos(double* __restrict__ a, int64_t n, double h1, dou
   p[3] = vx[i]; p[4] = vy[i]; p[5] = vz[i];
 }
 
+// ---- segmented particle layout ----
+// packed counts: seg[s] = clamp(N - s*kChunk, 0, kChunk)
+__global__ void k_seg_from_count(int32_t* __restrict__ seg, int64_t nseg, const int64_t* __restrict__ count) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  const int64_t r = *count - s * kChunk;
+  seg[s] = (int32_t)(r < 0 ? 0 : (r > kChunk ? kChunk : r));
+}
+
+__global__ void k_seg_copy(const int32_t* __restrict__ seg, int64_t nseg, uint32_t* __restrict__ out) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nseg) out[s] = (uint32_t)seg[s];
+}
+
+// move the particles of every segment to their dense positions
+__global__ void k_pack64(const int32_t* __restrict__ seg, const uint32_t* __restrict__ off, int64_t nslots,
+                         const double* a0, const double* a1, const double* a2, const double* a3,
+                         const double* a4, const double* a5, double* b0, double* b1, double* b2, double* b3,
+                         double* b4, double* b5) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslots) return;
+  const int64_t sg = q / kChunk;
+  const int k = (int)(q - sg * kChunk);
+  if (k >= seg[sg]) return;
+  const int64_t d = (int64_t)off[sg] + k;
+  b0[d] = a0[q]; b1[d] = a1[q]; b2[d] = a2[q]; b3[d] = a3[q]; b4[d] = a4[q]; b5[d] = a5[q];
+}
+
+__global__ void k_pack_mixed(const int32_t* __restrict__ seg, const uint32_t* __restrict__ off, int64_t nslots,
+                             const int32_t* c0, const float* a0, const float* a1, const float* a2,
+                             const float* a3, const float* a4, const float* a5, int32_t* d0, float* b0,
+                             float* b1, float* b2, float* b3, float* b4, float* b5) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nslots) return;
+  const int64_t sg = q / kChunk;
+  const int k = (int)(q - sg * kChunk);
+  if (k >= seg[sg]) return;
+  const int64_t d = (int64_t)off[sg] + k;
+  d0[d] = c0[q]; b0[d] = a0[q]; b1[d] = a1[q];
+  if (b2) b2[d] = a2[q];
+  b3[d] = a3[q]; b4[d] = a4[q]; b5[d] = a5[q];
+}
+
 // ---- generic exclusive scan (uint32, in place) ----
 constexpr int kScanItems = 16;
 constexpr int kScanTile = 256 * kScanItems;
@@ -150,21 +193,41 @@ int zero_async(lepic_ctx* ctx, void* p, size_t bytes) {
   return 0;
 }
 
-int ensure_lookback(lepic_ctx* ctx, int64_t nblocks) {
-  if (nblocks > ctx->lb_cap) {
-    if (ctx->lb_status) LEPIC_HIP(hipFree(ctx->lb_status));
-    int64_t cap = std::max<int64_t>(nblocks, 1024);
-    LEPIC_HIP(hipMalloc(&ctx->lb_status, cap * sizeof(uint64_t)));
-    ctx->lb_cap = cap;
-    if (int rc = zero_async(ctx, ctx->lb_status, cap * sizeof(uint64_t))) return rc;
-    ctx->lb_epoch = 0;
-  }
-  ctx->lb_epoch = (ctx->lb_epoch + 1) & 0xFFFFFFu;
-  if (ctx->lb_epoch == 0) {  // wrapped: clear stale words, restart at 1
-    if (int rc = zero_async(ctx, ctx->lb_status, ctx->lb_cap * sizeof(uint64_t))) return rc;
-    ctx->lb_epoch = 1;
-  }
+int set_packed_counts(lepic_ctx* ctx, int p) {
+  Species& s = ctx->sp[p];
+  hipLaunchKernelGGL(k_seg_from_count, dim3((unsigned)((s.nseg_cap + 255) / 256)), dim3(256), 0, ctx->stream,
+                     s.seg[s.cur], s.nseg_cap, (const int64_t*)(ctx->d_counts + p));
+  LEPIC_HIP(hipGetLastError());
+  s.packed = true;
   return 0;
+}
+
+int ensure_packed(lepic_ctx* ctx, int p) {
+  Species& s = ctx->sp[p];
+  if (s.packed) return 0;
+  const int64_t nseg = std::min<int64_t>(nseg_of(s), s.nseg_cap);
+  if (nseg > 0) {
+    const int b = s.cur, o = 1 - s.cur;
+    hipLaunchKernelGGL(k_seg_copy, dim3((unsigned)((nseg + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const int32_t*)s.seg[b], nseg, ctx->d_segoff);
+    LEPIC_HIP(hipGetLastError());
+    if (int rc = scan_exclusive_u32(ctx, ctx->d_segoff, nseg)) return rc;
+    const int64_t nslots = nseg * kChunk;
+    const dim3 g((unsigned)((nslots + 255) / 256));
+    if (ctx->g.precision == LEPIC_MIXED)
+      hipLaunchKernelGGL(k_pack_mixed, g, dim3(256), 0, ctx->stream, (const int32_t*)s.seg[b],
+                         (const uint32_t*)ctx->d_segoff, nslots, s.cell[b], s.f[b][0], s.f[b][1], s.f[b][2],
+                         s.f[b][3], s.f[b][4], s.f[b][5], s.cell[o], s.f[o][0], s.f[o][1], s.f[o][2],
+                         s.f[o][3], s.f[o][4], s.f[o][5]);
+    else
+      hipLaunchKernelGGL(k_pack64, g, dim3(256), 0, ctx->stream, (const int32_t*)s.seg[b],
+                         (const uint32_t*)ctx->d_segoff, nslots, s.d[b][0], s.d[b][1], s.d[b][2], s.d[b][3],
+                         s.d[b][4], s.d[b][5], s.d[o][0], s.d[o][1], s.d[o][2], s.d[o][3], s.d[o][4],
+                         s.d[o][5]);
+    LEPIC_HIP(hipGetLastError());
+    s.cur = o;
+  }
+  return set_packed_counts(ctx, p);
 }
 
 int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n) {
@@ -213,6 +276,17 @@ int alloc_species(lepic_ctx* ctx, int p) {
   }
   LEPIC_HIP(hipMalloc(&s.flag_dead, cap));
   LEPIC_HIP(hipMemset(s.flag_dead, 0, cap));
+  s.nseg_cap = (cap + kChunk - 1) / kChunk + 1;
+  for (int b = 0; b < 2; ++b) {
+    LEPIC_HIP(hipMalloc(&s.seg[b], s.nseg_cap * 4));
+    LEPIC_HIP(hipMemset(s.seg[b], 0, s.nseg_cap * 4));
+  }
+  if (s.nseg_cap > ctx->segoff_cap) {
+    if (ctx->d_segoff) LEPIC_HIP(hipFree(ctx->d_segoff));
+    LEPIC_HIP(hipMalloc(&ctx->d_segoff, s.nseg_cap * 4));
+    ctx->segoff_cap = s.nseg_cap;
+  }
+  s.packed = true;
   LEPIC_HIP(hipMalloc(&s.np, ctx->nodes * 8));
   LEPIC_HIP(hipMemset(s.np, 0, ctx->nodes * 8));
   LEPIC_HIP(hipMalloc(&s.np_red, ctx->nodes * 8));
@@ -317,10 +391,11 @@ int lepic_destroy(lepic_ctx* ctx) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);
     (void)hipFree(s.flag_dead); (void)hipFree(s.np); (void)hipFree(s.np_red); (void)hipFree(s.sink);
+    (void)hipFree(s.seg[0]); (void)hipFree(s.seg[1]);
   }
   void* ptrs[] = {ctx->bc, ctx->phi, ctx->phi_given, ctx->rhs, ctx->E, ctx->Ef, ctx->Bi, ctx->Bf,
                   ctx->d_ctrl, ctx->d_partials, ctx->d_scalar, ctx->d_resid64, ctx->d_corr,
-                  ctx->d_counts, ctx->d_tally, ctx->lb_status, ctx->d_ticket, ctx->d_error,
+                  ctx->d_counts, ctx->d_tally, ctx->d_segoff, ctx->d_ticket, ctx->d_error,
                   ctx->s_key[0], ctx->s_key[1], ctx->s_val[0], ctx->s_val[1], ctx->s_hist,
                   ctx->d_plist, ctx->d_scan_tmp, ctx->rx.d_sig_Er, ctx->rx.d_sig,
                   ctx->rx.d_sig_Er_f, ctx->rx.d_sig_f, ctx->rx.d_cnt_col, ctx->d_claim,
@@ -495,7 +570,7 @@ int lepic_upload_particles(lepic_ctx* ctx, int32_t ptype, const double* vxp, int
   LEPIC_HIP(hipFree(stage));
   LEPIC_HIP(hipMemcpy(ctx->d_counts + (ptype - 1), &np, 8, hipMemcpyHostToDevice));
   s.ub = np;
-  return 0;
+  return set_packed_counts(ctx, ptype - 1);
 }
 
 int lepic_get_count(lepic_ctx* ctx, int32_t ptype, int64_t* np) {
@@ -503,12 +578,14 @@ int lepic_get_count(lepic_ctx* ctx, int32_t ptype, int64_t* np) {
   LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_counts + (ptype - 1), 8, hipMemcpyDeviceToHost, ctx->stream));
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(np, ctx->h_pinned, 8);
-  ctx->sp[ptype - 1].ub = *np;
+  // the segment extent only shrinks when the species is packed
+  if (ctx->sp[ptype - 1].packed) ctx->sp[ptype - 1].ub = *np;
   return 0;
 }
 
 int lepic_download_particles(lepic_ctx* ctx, int32_t ptype, double* vxp, int64_t cap, int64_t* np) {
   if (!ctx || !valid_ptype(ctx, ptype) || !np) return LEPIC_ERR_ARG;
+  if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
   int64_t n = 0;
   if (int rc = lepic_get_count(ctx, ptype, &n)) return rc;
   *np = n;
@@ -543,6 +620,7 @@ int lepic_download_particles(lepic_ctx* ctx, int32_t ptype, double* vxp, int64_t
 int lepic_set_flag_dead(lepic_ctx* ctx, int32_t ptype, const int8_t* flags, int64_t np) {
   if (!ctx || !valid_ptype(ctx, ptype) || np < 0 || (np && !flags)) return LEPIC_ERR_ARG;
   if (np > ctx->sp[ptype - 1].cap) return ctx->fail(LEPIC_ERR_CAPACITY, "np exceeds capacity");
+  if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipMemcpy(ctx->sp[ptype - 1].flag_dead, flags, np, hipMemcpyHostToDevice));
   return 0;
@@ -551,6 +629,7 @@ int lepic_set_flag_dead(lepic_ctx* ctx, int32_t ptype, const int8_t* flags, int6
 int lepic_get_flag_dead(lepic_ctx* ctx, int32_t ptype, int8_t* flags, int64_t np) {
   if (!ctx || !valid_ptype(ctx, ptype) || np < 0 || (np && !flags)) return LEPIC_ERR_ARG;
   if (np > ctx->sp[ptype - 1].cap) return ctx->fail(LEPIC_ERR_CAPACITY, "np exceeds capacity");
+  if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipMemcpy(flags, ctx->sp[ptype - 1].flag_dead, np, hipMemcpyDeviceToHost));
   return 0;

@@ -18,7 +18,7 @@ constexpr int kMaxReac = 64;
 constexpr int kMaxTypes = 32;
 constexpr int kBlock = 256;          // threads per block of the particle kernels
 constexpr int kItems = 4;            // particles per thread in the mover
-constexpr int kChunk = kBlock * kItems;
+constexpr int kChunk = kBlock * kItems;  // slots per particle segment
 constexpr int kSweepsPerLaunch = 5;  // red-black sweeps per launch on multi-block MG levels
 constexpr int kRot = kSweepsPerLaunch + 1;  // rotating state buffers of a multi-block level
 constexpr int kRing = 16;            // residual partial-sum slots (sweep index mod kRing)
@@ -61,11 +61,21 @@ struct LevelCtrl {
   int32_t cur[2];
   double res;
 };
+// Particles of a species live in segments of kChunk slots: segment s holds
+// seg[cur][s] particles in its first slots, and the species' order is the
+// segments' contents in segment order.  The mover compacts each segment in
+// place of the other buffer, so survivors keep the reference's order
+// (part_expmover.f90:496-503) without any cross-block prefix.  "Packed"
+// means every segment but the last is full (a dense array of count slots);
+// kernels that index particles globally run on packed arrays.
 struct Species {
   double charge = 0, mass = 0, Nm = 0;
   int64_t cap = 0;
   int64_t ub = 0;                    // host-side upper bound of the count
   int cur = 0;                       // current particle buffer
+  int32_t* seg[2] = {nullptr, nullptr};  // per-buffer segment counts
+  int64_t nseg_cap = 0;
+  bool packed = true;
   void* mem[2] = {nullptr, nullptr};
   double* d[2][6] = {};              // FP64: x y z vx vy vz
   int32_t* cell[2] = {};             // mixed: (iy<<16)|ix, 1-based cell
@@ -140,9 +150,8 @@ struct lepic_ctx {
   // device scalars
   int64_t* d_counts = nullptr;       // [kMaxSpecies]
   lepic::Tally* d_tally = nullptr;   // [kMaxSpecies]
-  uint64_t* lb_status = nullptr;     // look-back status words
-  int64_t lb_cap = 0;
-  uint32_t lb_epoch = 0;
+  uint32_t* d_segoff = nullptr;      // segment offsets for packing (max nseg_cap)
+  int64_t segoff_cap = 0;
   unsigned long long* d_ticket = nullptr;   // monotonic block ticket of the movers
   unsigned long long ticket_base = 0;       // tickets taken by earlier launches
   bool timing = false;                      // lepic_kernel_timing
@@ -178,7 +187,9 @@ struct lepic_ctx {
 namespace lepic {
 
 // ---- host helpers implemented in lepic_ctx.hip ----
-int ensure_lookback(lepic_ctx* ctx, int64_t nblocks);
+int ensure_packed(lepic_ctx* ctx, int p);        // 0-based species
+int set_packed_counts(lepic_ctx* ctx, int p);    // segments from the device count
+inline int64_t nseg_of(const Species& s) { return (s.ub + kChunk - 1) / kChunk; }
 int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n);   // in place
 int zero_async(lepic_ctx* ctx, void* p, size_t bytes);
 
@@ -201,59 +212,6 @@ int load_uniform_launch(lepic_ctx* ctx, int ptype, int ppc, double vt, double xl
 // ---- device helpers ----
 __device__ __forceinline__ int64_t node_index(int ix, int iy, int n1) {
   return (int64_t)iy * (n1 + 3) + ix;
-}
-
-// Decoupled look-back status word: epoch (24 bits) | flag (2 bits) | value (38 bits).
-__device__ __forceinline__ uint64_t lb_pack(uint32_t epoch, uint32_t flag, uint64_t v) {
-  return ((uint64_t)(epoch & 0xFFFFFFu) << 40) | ((uint64_t)flag << 38) | (v & ((1ull << 38) - 1));
-}
-
-// Exclusive prefix of block `bid` given its aggregate (decoupled look-back).
-// Called by all 64 lanes of one wave: each round reads the status words of
-// the 64 preceding blocks at once, so a walk over blocks that have only
-// published their aggregates costs one memory round trip per 64 blocks.
-// Waits are bounded: on timeout the error flag is set and the walk ends.
-__device__ inline int64_t lookback_exclusive(uint64_t* status, int64_t bid, uint64_t aggregate,
-                                             uint32_t epoch, int32_t* err) {
-  const int lane = threadIdx.x & 63;
-  const uint32_t ep = epoch & 0xFFFFFFu;
-  if (bid == 0) {
-    if (lane == 0)
-      __hip_atomic_store(&status[0], lb_pack(epoch, 2, aggregate), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return 0;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[bid], lb_pack(epoch, 1, aggregate), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  uint64_t excl = 0;
-  int64_t j = bid - 1;
-  uint32_t spins = 0;
-  for (;;) {
-    const int64_t q = j - lane;
-    uint64_t sw = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                         : lb_pack(epoch, 2, 0);  // before block 0: inclusive zero
-    const uint32_t fl = (uint32_t)(sw >> 38) & 3u;
-    const bool ready = (uint32_t)(sw >> 40) == ep && fl != 0;
-    const uint64_t incl = __ballot(ready && fl == 2);
-    const int pos = incl ? __builtin_ctzll(incl) : 64;        // nearest inclusive prefix
-    const uint64_t need = pos >= 63 ? ~0ull : ((2ull << pos) - 1ull);
-    if (__ballot(!ready) & need) {
-      if (++spins > (1u << 22)) {  // bounded wait: report and give up
-        if (lane == 0) atomicOr(err, 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    unsigned long long v = lane <= pos ? (sw & ((1ull << 38) - 1)) : 0ull;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    excl += v;
-    if (pos < 64) break;
-    j -= 64;
-  }
-  if (lane == 0)
-    __hip_atomic_store(&status[bid], lb_pack(epoch, 2, excl + aggregate), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-  return (int64_t)excl;
 }
 
 __device__ __forceinline__ void atomic_add_f64(double* p, double v) { unsafeAtomicAdd(p, v); }

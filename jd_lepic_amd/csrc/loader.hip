@@ -101,6 +101,7 @@ int load_uniform_launch(lepic_ctx* ctx, int p, int ppc, double vt, double xl, do
   const int n1 = ctx->g.n1, n2 = ctx->g.n2;
   const int64_t nc = (int64_t)n1 * n2;
   int64_t N0 = 0;
+  if (int rc = ensure_packed(ctx, p)) return rc;
   if (int rc = lepic_get_count(ctx, p + 1, &N0)) return rc;
   uint32_t* buf = nullptr;
   LEPIC_HIP(hipMalloc(&buf, 2 * nc * 4));
@@ -134,7 +135,7 @@ int load_uniform_launch(lepic_ctx* ctx, int p, int ppc, double vt, double xl, do
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipFree(buf));
   s.ub = N0 + add;
-  return 0;
+  return set_packed_counts(ctx, p);
 }
 
 }  // namespace lepic

@@ -687,6 +687,9 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
   if (int rc = zero_async(ctx, d_created, 16 * 8)) return rc;
   if (int rc = zero_async(ctx, d_status, 4)) return rc;
   double* d_ploss3 = ctx->d_mcc_f64;  // [ptype], accumulated until lepic_reset_tallies
+  // candidates are drawn by particle index: dense arrays
+  for (int q = 0; q < ctx->nsp; ++q)
+    if (int rc = ensure_packed(ctx, q)) return rc;
   // counts on the host
   int64_t N[kMaxSpecies] = {};
   LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_counts, kMaxSpecies * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -821,7 +824,10 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(cr, ctx->h_pinned, 8 * 8);
   std::memcpy(&ms, ctx->h_pinned + 8, 8);
-  for (int q = 0; q < ctx->nsp; ++q) ctx->sp[q].ub = N[q] + (int64_t)cr[q];
+  for (int q = 0; q < ctx->nsp; ++q) {
+    ctx->sp[q].ub = N[q] + (int64_t)cr[q];
+    if (int rc = set_packed_counts(ctx, q)) return rc;  // appended particles extend the last segments
+  }
   if (info) {
     for (int c = 0; c < R.ncol && c < 32; ++c) info->events[c] = (int64_t)ev[c];
     for (int q = 0; q < ctx->nsp; ++q) info->created[q] = (int64_t)cr[q];

@@ -1,11 +1,12 @@
 // push.hip -- particle mover (part_expmover.f90:1-517) fused with charge
 // deposition (part_expmover.f90:520-610), FP64 and mixed precision.
 //
-// One block handles 4 x (threads per block) consecutive particles (thread t,
-// item k -> particle base + k*threads + t, so loads are coalesced).  Survivors are
-// compacted in particle order (part_expmover.f90:495-503) through a block
-// ballot scan and a decoupled look-back across blocks, writing to the other
-// particle buffer.  Deposition reduces equal-cell runs inside each wave
+// One block handles one segment of kChunk = 4 x 256 slots (thread t, item k
+// -> slot base + k*256 + t, so loads are coalesced).  Survivors are compacted
+// in particle order (part_expmover.f90:495-503) by a block ballot scan into
+// the same segment of the other particle buffer; the segments' contents in
+// segment order are the species' particles, so no prefix across blocks is
+// needed and blocks never wait for each other.  Deposition reduces equal-cell runs inside each wave
 // (particles are cell-sorted, so neighbours share a cell), accumulates the
 // runs in an LDS window of 4 rows x 128 columns of nodes around the block's
 // first cell, and flushes the window to the global density with FP64
@@ -23,10 +24,9 @@ namespace {
 constexpr int kWinRows = 4;
 constexpr int kWinCols = 128;
 constexpr int kWin = kWinRows * kWinCols;
-// threads per mover block: the decoupled look-back costs a few microseconds of
-// latency per block, so blocks take 4 x threads particles each
-constexpr int kMoverThreadsMixed = 1024;
-constexpr int kMoverThreads64 = 512;
+// threads per mover block; a block owns one segment of kChunk slots
+constexpr int kMoverThreadsMixed = kBlock;
+constexpr int kMoverThreads64 = kBlock;
 
 struct MoverCommon {
   const int8_t* bc;
@@ -43,13 +43,9 @@ struct MoverCommon {
   Tally* tally;
   double* sink;
   int ngrid;
-  const int64_t* n_in;
-  int64_t* n_out;
-  uint64_t* status;
-  uint32_t epoch;
-  unsigned long long* ticket;
-  unsigned long long ticket_base;
-  int64_t nblocks;
+  const int32_t* seg_in;        // particles per input segment
+  int32_t* seg_out;             // survivors per output segment
+  unsigned long long* count;    // species count, reduced by the lost particles
   int32_t* err;
 };
 
@@ -147,13 +143,13 @@ __device__ inline void tally_dead(const MoverCommon& a, double Eki) {
   if (a.tagneg) atomic_add_f64(&a.tally->cnt_dead, a.NmQ);
 }
 
-// Block-wide order-preserving offsets of survivors; returns the global base
-// through *gbase and each item's local offset in off[].
+// Order-preserving offsets of the survivors inside the block's segment;
+// records the segment's survivor count and takes the lost ones off the
+// species count.  Returns each item's offset in off[].
 template <int BT>
-__device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const bool surv[kItems],
-                                       uint32_t off[kItems], int64_t* gbase) {
+__device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cnt_in, const bool surv[kItems],
+                                       uint32_t off[kItems]) {
   __shared__ uint32_t s_cnt[kItems][BT / 64];
-  __shared__ int64_t s_base;
   const int w = threadIdx.x >> 6;
   uint64_t mask[kItems];
 #pragma unroll
@@ -172,15 +168,11 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, const 
     }
     off[k] = before + (uint32_t)__popcll(mask[k] & lanemask_lt());
   }
-  if (threadIdx.x < 64) {
-    const int64_t excl = bid < a.nblocks ? lookback_exclusive(a.status, bid, run, a.epoch, a.err) : 0;
-    if (threadIdx.x == 0) {
-      s_base = excl;
-      if (bid == a.nblocks - 1) *a.n_out = excl + run;
-    }
+  if (threadIdx.x == 0) {
+    a.seg_out[bid] = (int32_t)run;
+    const int lost = cnt_in - (int)run;
+    if (lost > 0) atomicAdd(a.count, (unsigned long long)(-(long long)lost));
   }
-  __syncthreads();
-  *gbase = s_base;
 }
 
 // ---------------------------------------------------------------------------
@@ -190,13 +182,10 @@ template <int BT>
 __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W out,
                                                     int8_t* flag_dead, const double* __restrict__ E,
                                                     const double* __restrict__ B) {
-  __shared__ int64_t s_bid;
   __shared__ double win[kWin];
   __shared__ int s_wx0, s_wy0;
-  if (threadIdx.x == 0) s_bid = (int64_t)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
-  __syncthreads();
-  const int64_t bid = s_bid;
-  const int64_t N = *a.n_in;
+  const int64_t bid = blockIdx.x;
+  const int cnt_in = a.seg_in[bid];
   const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
@@ -206,7 +195,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     for (int e = t; e < kWin; e += BT) win[e] = 0.0;
     if (t == 0) {
       int ix0 = 1, iy0 = 1;
-      if (base < N) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
+      if (cnt_in > 0) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
       s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
       s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
     }
@@ -218,7 +207,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
   for (int k = 0; k < kItems; ++k) {
     const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
-    if (i >= N) continue;
+    if (k * BT + t >= cnt_in) continue;
     double xp = in.x[i], yp = in.y[i], zp = in.z[i];
     double vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
     if (flag_dead[i] == 1) {  // part_expmover.f90:84-98
@@ -301,8 +290,8 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
   }
 
   uint32_t off[kItems];
-  int64_t gbase;
-  compact_offsets<BT>(a, bid, surv, off, &gbase);  // contains __syncthreads
+  compact_offsets<BT>(a, bid, cnt_in, surv, off);  // contains __syncthreads
+  const int64_t gbase = base;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (!surv[k]) continue;
@@ -351,13 +340,10 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
                                                         SoaMxW out, int8_t* flag_dead,
                                                         const float* __restrict__ E,
                                                         const float* __restrict__ B) {
-  __shared__ int64_t s_bid;
   __shared__ float win[kWin];
   __shared__ int s_wx0, s_wy0;
-  if (threadIdx.x == 0) s_bid = (int64_t)(atomicAdd(a.ticket, 1ull) - a.ticket_base);
-  __syncthreads();
-  const int64_t bid = s_bid;
-  const int64_t N = *a.n_in;
+  const int64_t bid = blockIdx.x;
+  const int cnt_in = a.seg_in[bid];
   const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
@@ -365,7 +351,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   if (a.deposit) {
     for (int e = t; e < kWin; e += BT) win[e] = 0.0f;
     if (t == 0) {
-      int c0 = base < N ? in.cell[base] : ((1 << 16) | 1);
+      int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
       int ix0 = c0 & 0xFFFF, iy0 = c0 >> 16;
       s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
       s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
@@ -378,7 +364,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   for (int k = 0; k < kItems; ++k) {
     const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
-    if (i >= N) continue;
+    if (k * BT + t >= cnt_in) continue;
     int c = in.cell[i];
     float fx = in.fx[i], fy = in.fy[i];
     float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
@@ -467,8 +453,8 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     surv[k] = true;
   }
   uint32_t off[kItems];
-  int64_t gbase;
-  compact_offsets<BT>(a, bid, surv, off, &gbase);
+  compact_offsets<BT>(a, bid, cnt_in, surv, off);
+  const int64_t gbase = base;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     if (!surv[k]) continue;
@@ -503,13 +489,13 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
 __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
   __shared__ double win[kWin];
   __shared__ int s_wx0, s_wy0;
-  const int64_t N = *a.n_in;
+  const int cnt_in = a.seg_in[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kChunk;
   const int t = threadIdx.x;
   for (int e = t; e < kWin; e += kBlock) win[e] = 0.0;
   if (t == 0) {
     int ix0 = 1, iy0 = 1;
-    if (base < N) { ix0 = (int)(in.x[base] / a.h1) + 1; iy0 = (int)(in.y[base] / a.h2) + 1; }
+    if (cnt_in > 0) { ix0 = (int)(in.x[base] / a.h1) + 1; iy0 = (int)(in.y[base] / a.h2) + 1; }
     s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
     s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
   }
@@ -521,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
     int64_t key = -2 - (int64_t)t;
     double w[4] = {0, 0, 0, 0};
     int ix = 0, iy = 0;
-    bool ok = i < N;
+    bool ok = k * kBlock + t < cnt_in;
     if (ok) {
       double xp = in.x[i], yp = in.y[i];
       ix = (int)(xp / a.h1) + 1;
@@ -545,12 +531,12 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
 __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx in) {
   __shared__ float win[kWin];
   __shared__ int s_wx0, s_wy0;
-  const int64_t N = *a.n_in;
+  const int cnt_in = a.seg_in[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kChunk;
   const int t = threadIdx.x;
   for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
   if (t == 0) {
-    int c0 = base < N ? in.cell[base] : ((1 << 16) | 1);
+    int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
     s_wx0 = (c0 & 0xFFFF) - 2 < 0 ? 0 : (c0 & 0xFFFF) - 2;
     s_wy0 = (c0 >> 16) - 1 < 0 ? 0 : (c0 >> 16) - 1;
   }
@@ -562,7 +548,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx i
     int64_t key = -2 - (int64_t)t;
     float w[4] = {0.f, 0.f, 0.f, 0.f};
     int ix = 0, iy = 0;
-    bool ok = i < N;
+    bool ok = k * kBlock + t < cnt_in;
     if (ok) {
       int c = in.cell[i];
       ix = c & 0xFFFF; iy = c >> 16;
@@ -596,8 +582,9 @@ MoverCommon common_args(lepic_ctx* ctx, int p) {
   a.tally = ctx->d_tally + p;
   a.sink = s.sink;
   a.ngrid = ctx->g.ngrid;
-  a.n_in = ctx->d_counts + p;
-  a.n_out = ctx->d_counts + p;
+  a.seg_in = s.seg[s.cur];
+  a.seg_out = s.seg[1 - s.cur];
+  a.count = (unsigned long long*)(ctx->d_counts + p);
   a.err = ctx->d_error;
   return a;
 }
@@ -611,15 +598,7 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   MoverCommon a = common_args(ctx, p);
   a.deposit = deposit ? 1 : 0;
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
-  const int64_t chunk = (int64_t)(mixed ? kMoverThreadsMixed : kMoverThreads64) * kItems;
-  const int64_t nb = std::max<int64_t>((s.ub + chunk - 1) / chunk, 1);
-  if (int rc = ensure_lookback(ctx, nb)) return rc;
-  a.status = ctx->lb_status;
-  a.epoch = ctx->lb_epoch;
-  a.ticket = ctx->d_ticket;           // monotonic: this launch's blocks take the next nb tickets
-  a.ticket_base = ctx->ticket_base;
-  ctx->ticket_base += (unsigned long long)nb;
-  a.nblocks = nb;
+  const int64_t nb = std::max<int64_t>(nseg_of(s), 1);  // one block per segment
   if (deposit)
     if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
   const int b = s.cur, o = 1 - s.cur;
@@ -650,6 +629,7 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
     ctx->ev_recorded[p] = true;
   }
   s.cur = o;
+  s.packed = false;
   return 0;
 }
 
@@ -658,7 +638,7 @@ int deposit_launch(lepic_ctx* ctx, int p) {
   MoverCommon a = common_args(ctx, p);
   a.deposit = 1;
   if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
-  const int64_t nb = std::max<int64_t>((s.ub + kChunk - 1) / kChunk, 1);
+  const int64_t nb = std::max<int64_t>(nseg_of(s), 1);
   const int b = s.cur;
   if (ctx->g.precision == LEPIC_MIXED) {
     SoaMx in{s.cell[b], s.f[b][0], s.f[b][1], s.f[b][2], s.f[b][3], s.f[b][4], s.f[b][5]};

@@ -200,6 +200,7 @@ namespace lepic {
 
 int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
   Species& s = ctx->sp[p];
+  if (int rc = ensure_packed(ctx, p)) return rc;
   int64_t N = 0;
   if (int rc = lepic_get_count(ctx, p + 1, &N)) return rc;
   const int n1 = ctx->g.n1, n2 = ctx->g.n2;
@@ -245,6 +246,7 @@ int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
                        s.d[o][3], s.d[o][4], s.d[o][5]);
   LEPIC_HIP(hipGetLastError());
   s.cur = o;
+  if (int rc = set_packed_counts(ctx, p)) return rc;
   // Plist(c) = number of particles in cells 1..c (1-based): exclusive scan of
   // the 0-based counts, shifted by one
   if (int rc = scan_exclusive_u32(ctx, cnt, pl_max + 1)) return rc;
