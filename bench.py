@@ -11,9 +11,12 @@ device loader (counter-based RNG, recorded seed).
 
 One step = rhs -> Poisson -> E -> [sort] -> [MCC] -> push+deposit of both
 species (main.f90:689-1090 order).  The headline value is particles/s of
-push + deposit + MCC in the mixed-precision form (FP32 push/deposit with
-FP64 grid sums, FP32 MG smoothing in FP64 refinement); the plain FP64 GPU
-form is measured on the same workload beside it.
+push + deposit + MCC in the mixed-precision form with the learned warm start
+(FP32 push/deposit with FP64 grid sums, FP32 MG smoothing in FP64
+refinement, Poisson started from the network's guess when it lowers the
+initial residual; the network is trained on the run's own first solves,
+untimed); plain mixed and plain FP64 are measured on the same workload
+beside it.
 
 Prints one JSON line.
 """
@@ -104,24 +107,38 @@ def build_ctx(W, precision, seed=20261202):
     return ctx
 
 
-def run_mode(W, precision, steps, warmup, hip, seed=7):
+def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_steps=40, train_iters=300):
+    """One mode of the step loop.  With learned=True the first learn_steps
+    (untimed) solves record (drhs, dphi) samples, the network is trained on
+    them (its time reported apart), and every later solve starts from the
+    network's guess when that lowers the initial residual."""
     ctx = build_ctx(W, precision)
     ctx.kernel_timing(True)      # mover kernels bracketed by events inside the library
     stream = C.c_void_p()
     ctx.lib.lepic_stream(ctx._ctx, C.byref(stream))
-    nsp = 2
     ev = [hip.event() for _ in range(12)]
-    acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[], []])
+    acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[], []],
+               net_used=0, res0_gain=[], train_s=0.0, loss=None)
+    pre = learn_steps if learned else 0
+    if learned:
+        ctx.net_create(capacity=64, seed=seed)
+        ctx.net_recording(True)
     # the first solve starts cold and converges from zero; later ones warm-start
-    for it in range(1, warmup + steps + 1):
-        timed = it > warmup
-        if it == warmup + 1:
+    for it in range(1, pre + warmup + steps + 1):
+        timed = it > pre + warmup
+        if learned and it == pre + 1:
+            ctx.net_recording(False)
+            hip.sync()
+            t = time.perf_counter()
+            acc["loss"] = ctx.net_train(train_iters, batch=4, lr=2e-3)
+            acc["train_s"] = time.perf_counter() - t
+        if it == pre + warmup + 1:
             hip.sync()
             t0 = time.perf_counter()
+        start = L.START_COLD if it == 1 else (L.START_NET if learned and it > pre else L.START_PREV)
         hip.record(ev[0], stream)
         ctx.calc_rhs()
-        info = ctx.iterative_solver(eps=1e-10, ncycl=20000,
-                                    start=L.START_COLD if it == 1 else L.START_PREV)
+        info = ctx.iterative_solver(eps=1e-10, ncycl=20000, start=start)
         ctx.calc_efield()
         hip.record(ev[1], stream)
         if it % 10 == 1:
@@ -143,6 +160,9 @@ def run_mode(W, precision, steps, warmup, hip, seed=7):
             acc["mcc"] += hip.elapsed(ev[2], ev[3])
             acc["push"] += hip.elapsed(ev[3], ev[8])
             acc["vcycles"] += info["vcycles"]
+            if learned and info["start_used"] == L.START_NET:
+                acc["net_used"] += 1
+                acc["res0_gain"].append(info["res0_alt"] / info["res0"])
             for p in (0, 1):
                 acc["push_kernel_ms"][p].append(ctx.kernel_ms(p + 1))
     hip.sync()
@@ -210,6 +230,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-plain-mixed", action="store_true")
+    ap.add_argument("--skip-learned", action="store_true")
+    ap.add_argument("--learn-steps", type=int, default=40)
+    ap.add_argument("--train-iters", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     args = ap.parse_args()
@@ -221,56 +245,75 @@ def main():
     W = workload(args.n)
     N = 2 * args.n * args.n * W["ppc"]
     K = args.steps
-    print(f"[bench] mixed precision, {N} particles, {args.warmup}+{K} steps", file=sys.stderr, flush=True)
-    mx, wall_mx, cnt_mx = run_mode(W, L.MIXED, K, args.warmup, hip)
-    fp = None
-    if not args.skip_fp64:
-        print("[bench] FP64", file=sys.stderr, flush=True)
-        fp, wall_fp, cnt_fp = run_mode(W, L.FP64, K, args.warmup, hip)
 
     def summary(a, wall, cnt):
         pdm = (a["push"] + a["mcc"]) / K            # push + deposit + MCC per step (ms)
         step = (a["field"] + a["sort"] + a["mcc"] + a["push"]) / K
-        return dict(ms_per_step=step, ms_push_deposit=a["push"] / K, ms_mcc=a["mcc"] / K,
-                    ms_sort=a["sort"] / K, ms_field=a["field"] / K, vcycles_per_step=a["vcycles"] / K,
-                    particles_per_s=sum(cnt) / (pdm * 1e-3), wall_s=wall, particles=sum(cnt))
+        out = dict(ms_per_step=step, ms_push_deposit=a["push"] / K, ms_mcc=a["mcc"] / K,
+                   ms_sort=a["sort"] / K, ms_field=a["field"] / K, vcycles_per_step=a["vcycles"] / K,
+                   particles_per_s=sum(cnt) / (pdm * 1e-3), wall_s=wall, particles=sum(cnt),
+                   electron_mover_ms=float(np.mean(a["push_kernel_ms"][0])),
+                   ion_mover_ms=float(np.mean(a["push_kernel_ms"][1])))
+        if a["loss"] is not None:
+            out.update(train_s=a["train_s"], train_loss_first=a["loss"][0], train_loss_last=a["loss"][1],
+                       net_start_used=a["net_used"] / K,
+                       res0_gain_median=float(np.median(a["res0_gain"])) if a["res0_gain"] else None)
+        return out
 
-    smx = summary(mx, wall_mx, cnt_mx)
+    res = {}
+    if not args.skip_learned:
+        print(f"[bench] mixed precision + learned warm start, {N} particles, "
+              f"{args.learn_steps} recorded + {args.warmup}+{K} steps", file=sys.stderr, flush=True)
+        r = run_mode(W, L.MIXED, K, args.warmup, hip, learned=True, learn_steps=args.learn_steps,
+                     train_iters=args.train_iters)
+        res["mixed_learned"] = (r, summary(*r))
+    if not args.skip_plain_mixed:
+        print(f"[bench] mixed precision, {N} particles, {args.warmup}+{K} steps", file=sys.stderr, flush=True)
+        r = run_mode(W, L.MIXED, K, args.warmup, hip)
+        res["mixed"] = (r, summary(*r))
+    if not args.skip_fp64:
+        print("[bench] FP64", file=sys.stderr, flush=True)
+        r = run_mode(W, L.FP64, K, args.warmup, hip)
+        res["fp64"] = (r, summary(*r))
+    head = "mixed_learned" if "mixed_learned" in res else "mixed"
+    (acc_h, _, cnt_h), s_h = res[head]
     # roofline of the dominant kernel (the electron mover, mixed precision):
     # algorithmic bytes = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities,
     # read and written) per particle and launch; the launch duration comes from
     # HIP events around that kernel alone, on the library's stream
     bpp = 48.0
-    ke = np.array(mx["push_kernel_ms"][0])
-    alg = cnt_mx[0] * bpp
+    ke = np.array(acc_h["push_kernel_ms"][0])
+    alg = cnt_h[0] * bpp
     achieved = alg / (ke.mean() * 1e-3) / 1e9
     cpu = None if args.skip_cpu else cpu_baseline(W, args.cpu_budget)
     out = {
         "metric": "ms/step & particles/s (push+deposit+MCC), mixed+learned vs FP64 GPU",
-        "value": smx["particles_per_s"],
+        "value": s_h["particles_per_s"],
         "unit": "particles/s",
         "n_gpus": 1, "steps": K, "warmup": args.warmup,
-        "ms_per_step": smx["ms_per_step"],
+        "ms_per_step": s_h["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.n}x{args.n} cells, e-/Ar+ 100 ppc each ({N} particles), "
                                "B filter 10 mT, Ar 0.3 Pa analytic MCC, dt 5e-12, MG 1e-10, "
-                               "mixed precision (learned warm start not in this build)",
+                               + ("mixed precision + learned warm start" if head == "mixed_learned"
+                                  else "mixed precision"),
                    "global_batch": N, "seq_len": 1, "parallelism": "single GPU"},
-        "mixed": smx,
-        "fp64": summary(fp, wall_fp, cnt_fp) if fp else None,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_mover_mixed (electrons)", "alg_bytes_per_launch": alg,
                      "avg_launch_ms": float(ke.mean())},
         "cpu_baseline": cpu,
     }
-    if fp:
-        kf = np.array(fp["push_kernel_ms"][0])
-        out["fp64"]["roofline_push_GBs"] = cnt_fp[0] * 96.0 / (kf.mean() * 1e-3) / 1e9
-        out["speedup_mixed_vs_fp64_step"] = out["fp64"]["ms_per_step"] / smx["ms_per_step"]
-        out["speedup_mixed_vs_fp64_pdm"] = smx["particles_per_s"] / out["fp64"]["particles_per_s"]
+    for k, (_, sm) in res.items():
+        out[k] = sm
+    if "fp64" in res:
+        (acc_f, _, cnt_f), s_f = res["fp64"]
+        kf = np.array(acc_f["push_kernel_ms"][0])
+        out["fp64"]["roofline_push_GBs"] = cnt_f[0] * 96.0 / (kf.mean() * 1e-3) / 1e9
+        out["speedup_vs_fp64_step"] = s_f["ms_per_step"] / s_h["ms_per_step"]
+        out["speedup_vs_fp64_pdm"] = s_h["particles_per_s"] / s_f["particles_per_s"]
     print(json.dumps(out))
 
 

@@ -45,7 +45,7 @@ enum {
 
 enum { LEPIC_FP64 = 0, LEPIC_MIXED = 1 };
 enum { LEPIC_MCC_CHECK = 0, LEPIC_MCC_PROD = 1 };
-enum { LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2 };
+enum { LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2, LEPIC_START_NET = 3 };
 enum {
   LEPIC_GRID_PHI = 0,   /* phi(0:n1+2,0:n2+2) */
   LEPIC_GRID_RHS = 1,   /* rhs(n1+1,n2+1) */
@@ -88,6 +88,7 @@ typedef struct {
   double res0;            /* residual / alpha of the starting guess (-1 if not computed) */
   int32_t start_used;     /* LEPIC_START_* actually used */
   int32_t nlog;           /* entries written to the sweep log */
+  double res0_alt;        /* LEPIC_START_NET: residual / alpha of the start not used (-1 otherwise) */
 } lepic_solve_info;
 
 typedef struct {
@@ -215,6 +216,27 @@ typedef struct {
   lepic_solve_info solve;
   double ms_rhs_poisson_e, ms_sort, ms_mcc, ms_push;   /* phase times (HIP events) */
 } lepic_step_args;
+/* ---- learned warm start (no counterpart in the reference) ----
+ * A U-Net (4 levels, 16-32-64-128 channels, BF16 MFMA kernels with FP32
+ * accumulation) maps the change of rhs since the last solve and the interior
+ * mask to the change of phi.  lepic_iterative_solver with LEPIC_START_NET
+ * starts from the network's guess only when it lowers the initial residual
+ * (else from the previous phi) and always iterates to the same tolerance.
+ * Samples are the (drhs, dphi) pairs of consecutive converged solves,
+ * recorded while recording is on. */
+int lepic_net_create(lepic_ctx *ctx, int32_t capacity, uint64_t seed);
+int lepic_net_recording(lepic_ctx *ctx, int32_t on);
+int lepic_net_samples(lepic_ctx *ctx, int32_t *n);
+int lepic_net_train(lepic_ctx *ctx, int32_t iters, int32_t batch, float lr, float *loss_first,
+                    float *loss_last);
+/* parameters in the flat layout [layer][cout][tap][cin] + 16 output weights */
+int lepic_net_set_params(lepic_ctx *ctx, const float *flat, int64_t n);
+int lepic_net_get_params(lepic_ctx *ctx, float *flat, int64_t n);
+/* forward pass and masked-MSE gradient on a host batch x(B,H,W,2) */
+int lepic_net_forward(lepic_ctx *ctx, const float *x, int32_t B, int32_t H, int32_t W, float *y);
+int lepic_net_loss_grad(lepic_ctx *ctx, const float *x, const float *target, const float *mask, int32_t B,
+                        int32_t H, int32_t W, double *loss, float *grad, int64_t n);
+
 int lepic_step(lepic_ctx *ctx, int32_t it, lepic_step_args *a);
 
 #ifdef __cplusplus

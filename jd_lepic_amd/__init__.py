@@ -15,13 +15,13 @@ import numpy as np
 
 from . import capi
 from .capi import (FP64, GRID_E, GRID_NP, GRID_NPRED, GRID_PHI, GRID_RHS, GRID_SINK, MCC_CHECK,
-                   MCC_PROD, MIXED, START_COLD, START_GIVEN, START_PREV)
+                   MCC_PROD, MIXED, START_COLD, START_GIVEN, START_NET, START_PREV)
 from .reactions import AMU, ME, QE
 
 EPS0 = 8.854187817e-12   # main.f90:122
 
 __all__ = ["Context", "LepicError", "Species", "FP64", "MIXED", "MCC_CHECK", "MCC_PROD",
-           "START_PREV", "START_COLD", "START_GIVEN", "GRID_PHI", "GRID_RHS", "GRID_E", "GRID_NP",
+           "START_PREV", "START_COLD", "START_GIVEN", "START_NET", "GRID_PHI", "GRID_RHS", "GRID_E", "GRID_NP",
            "GRID_NPRED", "GRID_SINK", "QE", "AMU", "ME", "EPS0"]
 
 
@@ -201,7 +201,55 @@ class Context:
         self._check(self.lib.lepic_iterative_solver(self._ctx, eps, ncycl, start, C.byref(info),
                                                     _ip(log), log_cap))
         return dict(vcycles=info.vcycles, status=info.status, res=info.res, ktot=info.ktot,
-                    res0=info.res0, sweeps=log[:min(info.nlog, log_cap)].copy())
+                    res0=info.res0, res0_alt=info.res0_alt, start_used=info.start_used,
+                    sweeps=log[:min(info.nlog, log_cap)].copy())
+
+    # ---- learned warm start (no counterpart in the reference) ----
+    def net_create(self, capacity=64, seed=1):
+        self._check(self.lib.lepic_net_create(self._ctx, capacity, seed))
+
+    def net_recording(self, on=True):
+        self._check(self.lib.lepic_net_recording(self._ctx, 1 if on else 0))
+
+    def net_samples(self):
+        n = C.c_int32()
+        self._check(self.lib.lepic_net_samples(self._ctx, C.byref(n)))
+        return n.value
+
+    def net_train(self, iters, batch=4, lr=1e-3):
+        a, b = C.c_float(), C.c_float()
+        self._check(self.lib.lepic_net_train(self._ctx, iters, batch, lr, C.byref(a), C.byref(b)))
+        return float(a.value), float(b.value)
+
+    def net_set_params(self, flat):
+        f = np.ascontiguousarray(flat, np.float32)
+        self._check(self.lib.lepic_net_set_params(self._ctx, f.ctypes.data_as(C.POINTER(C.c_float)), f.size))
+
+    def net_get_params(self, n):
+        f = np.zeros(n, np.float32)
+        self._check(self.lib.lepic_net_get_params(self._ctx, f.ctypes.data_as(C.POINTER(C.c_float)), n))
+        return f
+
+    def net_forward(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        B, H, W, _ = x.shape
+        y = np.zeros((B, H, W), np.float32)
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lepic_net_forward(self._ctx, x.ctypes.data_as(fp), B, H, W, y.ctypes.data_as(fp)))
+        return y
+
+    def net_loss_grad(self, x, target, mask, nparam):
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(target, np.float32)
+        m = np.ascontiguousarray(mask, np.float32)
+        B, H, W, _ = x.shape
+        g = np.zeros(nparam, np.float32)
+        loss = C.c_double()
+        fp = C.POINTER(C.c_float)
+        self._check(self.lib.lepic_net_loss_grad(self._ctx, x.ctypes.data_as(fp), t.ctypes.data_as(fp),
+                                                 m.ctypes.data_as(fp), B, H, W, C.byref(loss),
+                                                 g.ctypes.data_as(fp), nparam))
+        return float(loss.value), g
 
     def part_sorting(self, ptype, want_plist=False):
         if want_plist:

@@ -15,7 +15,7 @@ LEPIC_OK, LEPIC_ERR_ARG, LEPIC_ERR_UNSUPPORTED, LEPIC_ERR_HIP = 0, -1, -2, -3
 LEPIC_ERR_CAPACITY, LEPIC_ERR_DIVERGED, LEPIC_ERR_MCC, LEPIC_ERR_STATE = -4, -5, -6, -7
 FP64, MIXED = 0, 1
 MCC_CHECK, MCC_PROD = 0, 1
-START_PREV, START_COLD, START_GIVEN = 0, 1, 2
+START_PREV, START_COLD, START_GIVEN, START_NET = 0, 1, 2, 3
 GRID_PHI, GRID_RHS, GRID_E, GRID_NP, GRID_NPRED, GRID_SINK = range(6)
 
 _dp = C.POINTER(C.c_double)
@@ -39,7 +39,7 @@ class SpeciesDesc(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("vcycles", C.c_int32), ("status", C.c_int32), ("res", C.c_double),
                 ("ktot", C.c_double), ("res0", C.c_double), ("start_used", C.c_int32),
-                ("nlog", C.c_int32)]
+                ("nlog", C.c_int32), ("res0_alt", C.c_double)]
 
 
 class MccInfo(C.Structure):
@@ -77,6 +77,17 @@ SIGNATURES = {
     "lepic_sync": (C.c_int, [_ctxp]),
     "lepic_stream": (C.c_int, [_ctxp, C.POINTER(C.c_void_p)]),
     "lepic_kernel_timing": (C.c_int, [_ctxp, C.c_int32]),
+    "lepic_net_create": (C.c_int, [_ctxp, C.c_int32, C.c_uint64]),
+    "lepic_net_recording": (C.c_int, [_ctxp, C.c_int32]),
+    "lepic_net_samples": (C.c_int, [_ctxp, _ip]),
+    "lepic_net_train": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float),
+                                  C.POINTER(C.c_float)]),
+    "lepic_net_set_params": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int64]),
+    "lepic_net_get_params": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int64]),
+    "lepic_net_forward": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32,
+                                    C.POINTER(C.c_float)]),
+    "lepic_net_loss_grad": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.c_int32, C.c_int32, C.c_int32, _dp, C.POINTER(C.c_float), C.c_int64]),
     "lepic_get_kernel_ms": (C.c_int, [_ctxp, C.c_int32, C.POINTER(C.c_float)]),
     "lepic_set_boundary": (C.c_int, [_ctxp, _ip, _dp]),
     "lepic_set_bfield": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp]),

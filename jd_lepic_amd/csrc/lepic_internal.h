@@ -12,6 +12,8 @@
 
 namespace lepic {
 
+struct Net;
+
 constexpr int kMaxSpecies = 8;
 constexpr int kMaxLevels = 12;
 constexpr int kMaxReac = 64;
@@ -178,6 +180,9 @@ struct lepic_ctx {
   double* d_mcc_f64 = nullptr;       // [64] sums: [0..8] P_loss(3,ptype), [40..44] 1/np_tot_rg
   void* d_mcc_tab = nullptr;         // device copy of the reaction table
 
+  // learned warm start (unet.hip)
+  lepic::Net* net = nullptr;
+
   int fail(int code, const std::string& msg) {
     err = msg;
     return code;
@@ -208,6 +213,9 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed,
             lepic_mcc_info* info);
 int load_uniform_launch(lepic_ctx* ctx, int ptype, int ppc, double vt, double xl, double xr,
                         uint64_t seed);
+void net_destroy(lepic_ctx* ctx);
+int net_after_solve(lepic_ctx* ctx);
+int net_guess(lepic_ctx* ctx, double* out);   // 1: no guess available
 
 // ---- device helpers ----
 __device__ __forceinline__ int64_t node_index(int ix, int iy, int n1) {

@@ -883,11 +883,11 @@ int resid_blocks(lepic_ctx* ctx) { return std::min(ctx->g.n2 + 1, kResidBlocks);
 
 // FP64 residual of phi; fills the FP32 right-hand side (and clears the
 // correction the next visit starts from) when fill_f32
-int launch_resid64(lepic_ctx* ctx, bool fill_f32) {
+int launch_resid64(lepic_ctx* ctx, bool fill_f32, const double* u = nullptr) {
   Level& L = ctx->lv[1];
   float* const* erot = nullptr;
   if (fill_f32) erot = (float* const*)ctx->d_erot;
-  hipLaunchKernelGGL(k_resid64, dim3(resid_blocks(ctx)), dim3(256), 0, ctx->stream, (const double*)ctx->phi,
+  hipLaunchKernelGGL(k_resid64, dim3(resid_blocks(ctx)), dim3(256), 0, ctx->stream, u ? u : (const double*)ctx->phi,
                      (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2, L.aw, L.an, L.ac,
                      fill_f32 ? L.bf : nullptr, erot, L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr,
                      ctx->mg_epoch + 1, ctx->d_partials);
@@ -1017,18 +1017,41 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
   const int64_t n = ctx->nodes;
   const dim3 g1((unsigned)((n + 255) / 256));
   if (ng == 1 && mixed) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "ng=1 runs in FP64 only");
-  if (start == LEPIC_START_COLD)
-    hipLaunchKernelGGL(k_cold_start, g1, dim3(256), 0, ctx->stream, ctx->phi, (const int8_t*)ctx->bc, n);
-  else if (start == LEPIC_START_GIVEN)
-    hipLaunchKernelGGL(k_copy_interior, g1, dim3(256), 0, ctx->stream, ctx->phi,
-                       (const double*)ctx->phi_given, (const int8_t*)ctx->bc, n);
-  LEPIC_HIP(hipGetLastError());
   const double alpha = alpha_of(ctx);
   int visit[kVisitLog];
   double vres[kVisitLog];
   double res = 0.0, res_s = 0.0, ktot = 0.0;
   info->nlog = 0;
   info->start_used = start;
+  info->res0_alt = -1.0;
+  if (start == LEPIC_START_NET) {
+    // the network's guess replaces the previous phi only if it lowers the
+    // initial residual; otherwise the solve starts from the previous phi
+    info->start_used = LEPIC_START_PREV;
+    const int g = net_guess(ctx, ctx->phi_given);
+    if (g < 0) return g;
+    if (g == 0) {
+      double r_prev = 0.0, r_net = 0.0;
+      if (int rc = launch_resid64(ctx, false)) return rc;
+      if (int rc = read_back(ctx, visit, vres, &r_prev)) return rc;
+      if (int rc = launch_resid64(ctx, false, ctx->phi_given)) return rc;
+      if (int rc = read_back(ctx, visit, vres, &r_net)) return rc;
+      if (r_net < r_prev) {
+        hipLaunchKernelGGL(k_copy_interior, g1, dim3(256), 0, ctx->stream, ctx->phi,
+                           (const double*)ctx->phi_given, (const int8_t*)ctx->bc, n);
+        info->start_used = LEPIC_START_NET;
+        info->res0_alt = r_prev / alpha;
+      } else {
+        info->res0_alt = r_net / alpha;
+      }
+    }
+  } else if (start == LEPIC_START_COLD) {
+    hipLaunchKernelGGL(k_cold_start, g1, dim3(256), 0, ctx->stream, ctx->phi, (const int8_t*)ctx->bc, n);
+  } else if (start == LEPIC_START_GIVEN) {
+    hipLaunchKernelGGL(k_copy_interior, g1, dim3(256), 0, ctx->stream, ctx->phi,
+                       (const double*)ctx->phi_given, (const int8_t*)ctx->bc, n);
+  }
+  LEPIC_HIP(hipGetLastError());
   // residual of the starting guess; in mixed mode it also fills the FP32
   // right-hand side of the first correction equation
   double r0 = 0.0;
@@ -1098,7 +1121,7 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
   info->status = (res <= eps) ? 0 : 1;
   info->res = res;
   info->ktot = ktot;
-  return 0;
+  return net_after_solve(ctx);
 }
 
 }  // namespace lepic

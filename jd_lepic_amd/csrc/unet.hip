@@ -1,0 +1,1007 @@
+// unet.hip -- learned warm start of the Poisson solve.  No counterpart in the
+// reference: iterative_solver.f90:1-69 remains the solver of record and
+// always iterates to its tolerance; the network only proposes the starting
+// phi, which is kept when it lowers the initial residual.
+//
+// Network: U-Net with 4 levels (16-32-64-128 channels), 3x3 convolutions
+// without bias, ReLU, 2x2 average pooling, nearest upsampling with skip
+// concatenation, 1x1 FP32 output layer.  It maps (change of rhs since the
+// last solve, interior mask) on the node grid to the change of phi.
+//
+// Precision: BF16 inputs, weights, activations and activation gradients
+// (round to nearest even), FP32 accumulation on MFMA
+// (v_mfma_f32_16x16x16_bf16), FP32 master weights and Adam moments.
+// Tensors are NHWC with a one-pixel zero border, so a 3x3 convolution reads
+// its neighbours without bounds tests; channel slices of one buffer hold a
+// skip connection next to the upsampled decoder features, so concatenation
+// costs nothing.
+//
+// Convolution as implicit GEMM: one wave computes 16 pixels of a row times
+// all output channels; per tap and 16-channel chunk a lane loads 4 channels
+// of one pixel (8 B) and 4 input channels of one output channel's weights.
+// The input gradient is the same kernel with the weights transposed and the
+// taps flipped; the weight gradient reduces over pixels with MFMA (pixels
+// are the K dimension) and adds FP32 partial sums atomically.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "lepic_internal.h"
+
+using namespace lepic;
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t bf16_t;
+
+__device__ __forceinline__ float bf2f(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ bool bf_pos(bf16_t h) { return (int16_t)h > 0; }
+
+// element (b, y, x, c) of a padded NHWC tensor with C channels per pixel
+__device__ __forceinline__ size_t px(int b, int y, int x, int H, int W) {
+  return ((size_t)b * (H + 2) + (y + 1)) * (size_t)(W + 2) + (x + 1);
+}
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution (forward, or input gradient with transposed flipped weights)
+// ---------------------------------------------------------------------------
+struct ConvArgs {
+  const bf16_t* in;
+  int in_C, in_off, cin;      // input channels [in_off, in_off+cin) of in_C per pixel
+  const bf16_t* w;            // [cout][9][cin]
+  bf16_t* out;
+  int out_C, out_off, cout;
+  const bf16_t* mask;         // optional: zero where mask <= 0 (ReLU backward)
+  int mask_C, mask_off;
+  int relu;
+  int B, H, W;
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs A) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x0 = (blockIdx.x * 4 + wv) * 16;
+  const int y = blockIdx.y, b = blockIdx.z;
+  if (x0 >= A.W) return;
+  const int m = lane & 15, g = lane >> 4;
+  const int x = x0 + m;
+  const bool valid = x < A.W;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 9; ++t) {
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const bf16_t* row = A.in + px(b, y + dy, valid ? x + dx : -1, A.H, A.W) * A.in_C + A.in_off + 4 * g;
+    const bf16_t* wt = A.w + ((size_t)m * 9 + t) * A.cin + 4 * g;
+    for (int c0 = 0; c0 < A.cin; c0 += 16) {
+      bf16x4 a = valid ? *(const bf16x4*)(row + c0) : bf16x4{0, 0, 0, 0};
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const bf16x4 bw = *(const bf16x4*)(wt + (size_t)16 * j * 9 * A.cin + c0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[j], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int xp = x0 + 4 * g + i;
+      if (xp >= A.W) continue;
+      const int co = 16 * j + m;
+      float v = acc[j][i];
+      if (A.relu) v = fmaxf(v, 0.f);
+      const size_t p = px(b, y, xp, A.H, A.W);
+      if (A.mask && !bf_pos(A.mask[p * A.mask_C + A.mask_off + co])) v = 0.f;
+      A.out[p * A.out_C + A.out_off + co] = f2bf(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// weight gradient: dW[co][t][ci] += sum_p dz[p][co] * x[p + off(t)][ci]
+// ---------------------------------------------------------------------------
+struct WgradArgs {
+  const bf16_t* x;
+  int x_C, x_off, cin;
+  const bf16_t* dz;
+  int dz_C, dz_off, cout;
+  float* dw;                  // [cout][9][cin]
+  int B, H, W;
+  int rows_per_wave;
+};
+
+__global__ __launch_bounds__(256) void k_wgrad(WgradArgs A) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int ntci = A.cin / 16;
+  const int co0 = (blockIdx.y / ntci) * 16, ci0 = (blockIdx.y % ntci) * 16;
+  const int m = lane & 15, g = lane >> 4;
+  const int r0 = (blockIdx.x * 4 + wv) * A.rows_per_wave;
+  const int nrows = A.B * A.H;
+  if (r0 >= nrows) return;
+  const int r1 = min(r0 + A.rows_per_wave, nrows);
+  f32x4 acc[9];
+#pragma unroll
+  for (int t = 0; t < 9; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int r = r0; r < r1; ++r) {
+    const int b = r / A.H, y = r % A.H;
+    for (int xs = 0; xs < A.W; xs += 16) {
+      bf16x4 a;
+      int xk[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        xk[i] = xs + 4 * g + i;
+        a[i] = xk[i] < A.W ? (short)A.dz[px(b, y, xk[i], A.H, A.W) * A.dz_C + A.dz_off + co0 + m] : (short)0;
+      }
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        bf16x4 bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          bv[i] = xk[i] < A.W ? (short)A.x[px(b, y + dy, xk[i] + dx, A.H, A.W) * A.x_C + A.x_off + ci0 + m]
+                              : (short)0;
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // lane holds D[co0 + 4g + i][ci0 + m]
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (acc[t][i] != 0.f)
+        atomicAdd(&A.dw[((size_t)(co0 + 4 * g + i) * 9 + t) * A.cin + ci0 + m], acc[t][i]);
+}
+
+// ---------------------------------------------------------------------------
+// pooling, upsampling, skip gradients
+// ---------------------------------------------------------------------------
+// out (Ho=H/2, Wo=W/2, C) = average of 2x2 input pixels (channels in_off..)
+__global__ void k_pool(const bf16_t* __restrict__ in, int in_C, int in_off, bf16_t* __restrict__ out, int C,
+                       int B, int Ho, int Wo) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)B * Ho * Wo * C;
+  if (q >= n) return;
+  const int c = (int)(q % C);
+  int64_t r = q / C;
+  const int x = (int)(r % Wo);
+  r /= Wo;
+  const int y = (int)(r % Ho), b = (int)(r / Ho);
+  const int H = 2 * Ho, W = 2 * Wo;
+  const float s = (bf2f(in[px(b, 2 * y, 2 * x, H, W) * in_C + in_off + c]) +
+                   bf2f(in[px(b, 2 * y, 2 * x + 1, H, W) * in_C + in_off + c])) +
+                  (bf2f(in[px(b, 2 * y + 1, 2 * x, H, W) * in_C + in_off + c]) +
+                   bf2f(in[px(b, 2 * y + 1, 2 * x + 1, H, W) * in_C + in_off + c]));
+  out[px(b, y, x, Ho, Wo) * C + c] = f2bf(0.25f * s);
+}
+
+// nearest upsampling of in (H/2, W/2, C) into channels [0, C) of out (H, W, out_C)
+__global__ void k_up(const bf16_t* __restrict__ in, int C, bf16_t* __restrict__ out, int out_C, int B, int H,
+                     int W) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)B * H * W * C;
+  if (q >= n) return;
+  const int c = (int)(q % C);
+  int64_t r = q / C;
+  const int x = (int)(r % W);
+  r /= W;
+  const int y = (int)(r % H), b = (int)(r / H);
+  out[px(b, y, x, H, W) * out_C + c] = in[px(b, y / 2, x / 2, H / 2, W / 2) * C + c];
+}
+
+// gradient of the upsampling: sum of the 4 fine gradients (channels [0, C)
+// of g (H, W, g_C)), masked by the ReLU of the coarse activation act
+__global__ void k_up_bwd(const bf16_t* __restrict__ g, int g_C, const bf16_t* __restrict__ act,
+                         bf16_t* __restrict__ out, int C, int B, int Ho, int Wo) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)B * Ho * Wo * C;
+  if (q >= n) return;
+  const int c = (int)(q % C);
+  int64_t r = q / C;
+  const int x = (int)(r % Wo);
+  r /= Wo;
+  const int y = (int)(r % Ho), b = (int)(r / Ho);
+  const int H = 2 * Ho, W = 2 * Wo;
+  const float s = (bf2f(g[px(b, 2 * y, 2 * x, H, W) * g_C + c]) + bf2f(g[px(b, 2 * y, 2 * x + 1, H, W) * g_C + c])) +
+                  (bf2f(g[px(b, 2 * y + 1, 2 * x, H, W) * g_C + c]) +
+                   bf2f(g[px(b, 2 * y + 1, 2 * x + 1, H, W) * g_C + c]));
+  const size_t p = px(b, y, x, Ho, Wo) * C + c;
+  out[p] = bf_pos(act[p]) ? f2bf(s) : (bf16_t)0;
+}
+
+// skip gradient, in place in the concat buffer: g[s] += pool_bwd(gp), masked
+// by the ReLU of the skip activation (same buffer slice of the activations)
+__global__ void k_skip_grad(bf16_t* __restrict__ g, const bf16_t* __restrict__ act, int cat_C, int off,
+                            const bf16_t* __restrict__ gp, int C, int B, int H, int W) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)B * H * W * C;
+  if (q >= n) return;
+  const int c = (int)(q % C);
+  int64_t r = q / C;
+  const int x = (int)(r % W);
+  r /= W;
+  const int y = (int)(r % H), b = (int)(r / H);
+  const size_t p = px(b, y, x, H, W) * cat_C + off + c;
+  const float v = bf2f(g[p]) + 0.25f * bf2f(gp[px(b, y / 2, x / 2, H / 2, W / 2) * C + c]);
+  g[p] = bf_pos(act[p]) ? f2bf(v) : (bf16_t)0;
+}
+
+// ---------------------------------------------------------------------------
+// output layer, loss and their gradients
+// ---------------------------------------------------------------------------
+__global__ void k_out(const bf16_t* __restrict__ d0, const float* __restrict__ w, float* __restrict__ y, int B,
+                      int H, int W) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)B * H * W) return;
+  const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
+  const bf16_t* a = d0 + px(b, yy, x, H, W) * 16;
+  float s = 0.f;
+  for (int c = 0; c < 16; ++c) s += bf2f(a[c]) * w[c];
+  y[q] = s;
+}
+
+// loss sum and dL/dy = 2 mask (y - t) / nm
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ mk,
+                       float inv_nm, float* __restrict__ dy, double* __restrict__ loss, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (q < n) {
+    const float d = (y[q] - t[q]) * mk[q];
+    dy[q] = 2.0f * d * inv_nm;
+    l = (double)d * (double)d;
+  }
+  l = wave_sum(l);
+  if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+}
+
+// gradient of the output layer: dw[c] += sum d0*dy; g_d0 = bf16(dy*w) * (d0 > 0)
+__global__ void k_out_bwd(const bf16_t* __restrict__ d0, const float* __restrict__ w, const float* __restrict__ dy,
+                          float* __restrict__ dw, bf16_t* __restrict__ g, int B, int H, int W) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = q < (int64_t)B * H * W;
+  float part[16];
+  if (ok) {
+    const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
+    const size_t p = px(b, yy, x, H, W) * 16;
+    const float d = dy[q];
+    for (int c = 0; c < 16; ++c) {
+      const bf16_t a = d0[p + c];
+      part[c] = bf2f(a) * d;
+      g[p + c] = bf_pos(a) ? f2bf(d * w[c]) : (bf16_t)0;
+    }
+  } else {
+    for (int c = 0; c < 16; ++c) part[c] = 0.f;
+  }
+  for (int c = 0; c < 16; ++c) {
+    float v = part[c];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v != 0.f) atomicAdd(&dw[c], v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// parameters
+// ---------------------------------------------------------------------------
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, int64_t n, float lr, float b1, float b2, float c1, float c2,
+                       float eps) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const float gq = g[q];
+  const float mq = b1 * m[q] + (1.f - b1) * gq;
+  const float vq = b2 * v[q] + (1.f - b2) * gq * gq;
+  m[q] = mq;
+  v[q] = vq;
+  w[q] -= lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+}
+
+// BF16 copies of a 3x3 layer: w[co][t][ci] and the input-gradient weights
+// wt[ci][8-t][co]
+__global__ void k_wcast(const float* __restrict__ w, bf16_t* __restrict__ wb, bf16_t* __restrict__ wt, int cout,
+                        int cin) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)cout * 9 * cin) return;
+  const int ci = (int)(q % cin), t = (int)((q / cin) % 9), co = (int)(q / ((int64_t)cin * 9));
+  const bf16_t h = f2bf(w[q]);
+  wb[q] = h;
+  wt[((size_t)ci * 9 + (8 - t)) * cout + co] = h;
+}
+
+// ---------------------------------------------------------------------------
+// samples: node-grid fields to network images and back
+// ---------------------------------------------------------------------------
+// Node (i, j), i in 1..n1+1, j in 1..n2+1 maps to pixel (x = i-1, y = j-1)
+// of an H x W image (H, W multiples of 8 covering n2+1, n1+1).
+__global__ void k_diff_to_image(const double* __restrict__ a, const double* __restrict__ a_prev, int a_nodes,
+                                int n1, int n2, int W, int H, float* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)H * W) return;
+  const int x = (int)(q % W), y = (int)(q / W);
+  float v = 0.f;
+  if (x <= n1 && y <= n2) {
+    // a_nodes: 1 -> (n1+3)-wide node array with ghosts, 0 -> (n1+1)-wide rhs array
+    const int64_t k = a_nodes ? (int64_t)(y + 1) * (n1 + 3) + (x + 1) : (int64_t)y * (n1 + 1) + x;
+    v = (float)(a[k] - a_prev[k]);
+  }
+  out[q] = v;
+}
+
+__global__ void k_mask_image(const int8_t* __restrict__ bc, int n1, int n2, int W, int H, float* __restrict__ mk) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)H * W) return;
+  const int x = (int)(q % W), y = (int)(q / W);
+  mk[q] = (x <= n1 && y <= n2 && bc[(int64_t)(y + 1) * (n1 + 3) + (x + 1)] == -1) ? 1.f : 0.f;
+}
+
+// sum of squares over mask (double, atomics)
+__global__ void k_sumsq(const float* __restrict__ v, const float* __restrict__ mk, int64_t n, double* out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double s = 0.0;
+  if (q < n) s = (double)v[q] * (double)v[q] * (double)mk[q];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(out, s);
+}
+
+// batch slot: x channels (v * sx, mask) into the 16-channel input tensor,
+// target v * st (FP32)
+__global__ void k_fill_input(const float* __restrict__ xin, const float* __restrict__ mk, float sx,
+                             bf16_t* __restrict__ X, int b, int H, int W, const float* __restrict__ tin, float st,
+                             float* __restrict__ T, float* __restrict__ M) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)H * W) return;
+  const int x = (int)(q % W), y = (int)(q / W);
+  bf16_t* p = X + px(b, y, x, H, W) * 16;
+  p[0] = f2bf(xin[q] * sx);
+  p[1] = f2bf(mk[q]);
+  if (T) {
+    T[(int64_t)b * H * W + q] = tin[q] * st;
+    M[(int64_t)b * H * W + q] = mk[q];
+  }
+}
+
+// guess = phi + scale * y on interior nodes
+__global__ void k_apply_guess(const double* __restrict__ phi, const float* __restrict__ y, double scale,
+                              const int8_t* __restrict__ bc, int n1, int n2, int W, double* __restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (int64_t)(n1 + 3) * (n2 + 3)) return;
+  const int i = (int)(k % (n1 + 3)), j = (int)(k / (n1 + 3));
+  double v = phi[k];
+  if (bc[k] == -1) v += scale * (double)y[(int64_t)(j - 1) * W + (i - 1)];
+  out[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct Layer {
+  const char* name;
+  int cin, cout;       // cin as stored on the GPU (the input layer pads 2 -> 16)
+  int cin_logical;
+  size_t off;          // offset in the parameter arrays
+};
+
+const Layer kLayers[14] = {
+    {"enc0a", 16, 16, 2, 0},   {"enc0b", 16, 16, 16, 0}, {"enc1a", 16, 32, 16, 0}, {"enc1b", 32, 32, 32, 0},
+    {"enc2a", 32, 64, 32, 0},  {"enc2b", 64, 64, 64, 0}, {"mida", 64, 128, 64, 0}, {"midb", 128, 128, 128, 0},
+    {"dec2a", 192, 64, 192, 0}, {"dec2b", 64, 64, 64, 0}, {"dec1a", 96, 32, 96, 0}, {"dec1b", 32, 32, 32, 0},
+    {"dec0a", 48, 16, 48, 0},  {"dec0b", 16, 16, 16, 0}};
+enum { ENC0A, ENC0B, ENC1A, ENC1B, ENC2A, ENC2B, MIDA, MIDB, DEC2A, DEC2B, DEC1A, DEC1B, DEC0A, DEC0B };
+
+struct T4 {                 // padded NHWC bf16 tensor
+  bf16_t* p = nullptr;
+  int C = 0;
+};
+
+}  // namespace
+
+namespace lepic {
+
+struct Net {
+  Layer L[14];
+  size_t nconv = 0, nparam = 0;      // GPU layout sizes (conv weights, all incl. output layer)
+  float* w = nullptr;                // master weights, then output layer (16)
+  float* m = nullptr;
+  float* v = nullptr;
+  float* g = nullptr;
+  bf16_t* wb = nullptr;              // BF16 weights
+  bf16_t* wt = nullptr;              // BF16 transposed flipped weights
+  int64_t adam_t = 0;
+  // activations and gradients for (B, H, W)
+  int B = 0, H = 0, W = 0;
+  void* arena = nullptr;
+  T4 X, E0A, CAT0, D0A, D0, P0, E1A, CAT1, D1A, D1, P1, E2A, CAT2, D2A, D2, P2, MA, M;
+  T4 G_D0, G_D0A, G_CAT0, G_D1, G_D1A, G_CAT1, G_D2, G_D2A, G_CAT2, G_M, G_MA, G_P2, G_E2A, G_P1, G_E1A, G_P0,
+      G_E0A;
+  float* Y = nullptr;
+  float* Tt = nullptr;
+  float* Mk = nullptr;
+  float* dY = nullptr;
+  double* d_loss = nullptr;
+  // samples on the node grid image (Hs x Ws)
+  int Hs = 0, Ws = 0;
+  int cap = 0, count = 0, head = 0;
+  float* s_x = nullptr;              // [cap][Hs*Ws] change of rhs
+  float* s_t = nullptr;              // [cap][Hs*Ws] change of phi
+  std::vector<double> s_sx, s_st;    // rms over the mask
+  float* mask_img = nullptr;
+  double nmask = 0.0;
+  double ratio = 0.0;                // mean rms(dphi)/rms(drhs) of the training set
+  bool trained = false;
+  bool recording = false;
+  bool have_last = false;
+  double* rhs_last = nullptr;
+  double* phi_last = nullptr;
+  float* img_tmp = nullptr;
+  double* d_tmp = nullptr;
+  std::mt19937_64 rng;
+};
+
+}  // namespace lepic
+
+namespace {
+
+int round8(int v) { return (v + 7) / 8 * 8; }
+
+int alloc_tensors(lepic_ctx* ctx, Net& N, int B, int H, int W) {
+  if (N.arena && B <= N.B && H == N.H && W == N.W) return 0;
+  if (N.arena) LEPIC_HIP(hipFree(N.arena));
+  N.arena = nullptr;
+  B = std::max(B, N.B);
+  struct Spec { T4* t; int lvl; int C; };
+  const Spec specs[] = {
+      {&N.X, 0, 16}, {&N.E0A, 0, 16}, {&N.CAT0, 0, 48}, {&N.D0A, 0, 16}, {&N.D0, 0, 16},
+      {&N.P0, 1, 16}, {&N.E1A, 1, 32}, {&N.CAT1, 1, 96}, {&N.D1A, 1, 32}, {&N.D1, 1, 32},
+      {&N.P1, 2, 32}, {&N.E2A, 2, 64}, {&N.CAT2, 2, 192}, {&N.D2A, 2, 64}, {&N.D2, 2, 64},
+      {&N.P2, 3, 64}, {&N.MA, 3, 128}, {&N.M, 3, 128},
+      {&N.G_D0, 0, 16}, {&N.G_D0A, 0, 16}, {&N.G_CAT0, 0, 48}, {&N.G_E0A, 0, 16}, {&N.G_P0, 1, 16},
+      {&N.G_D1, 1, 32}, {&N.G_D1A, 1, 32}, {&N.G_CAT1, 1, 96}, {&N.G_E1A, 1, 32}, {&N.G_P1, 2, 32},
+      {&N.G_D2, 2, 64}, {&N.G_D2A, 2, 64}, {&N.G_CAT2, 2, 192}, {&N.G_E2A, 2, 64}, {&N.G_P2, 3, 64},
+      {&N.G_M, 3, 128}, {&N.G_MA, 3, 128}};
+  auto elems = [&](int lvl, int C) {
+    const int h = H >> lvl, w = W >> lvl;
+    return (size_t)B * (h + 2) * (w + 2) * C;
+  };
+  size_t total = 0;
+  for (const Spec& s : specs) total += (elems(s.lvl, s.C) * 2 + 255) & ~(size_t)255;
+  const size_t hw = (size_t)B * H * W;
+  const size_t fbytes = 4 * ((hw * 4 + 63) & ~(size_t)63);
+  LEPIC_HIP(hipMalloc(&N.arena, total + fbytes + 256));
+  LEPIC_HIP(hipMemset(N.arena, 0, total + fbytes + 256));
+  char* p = (char*)N.arena;
+  for (const Spec& s : specs) {
+    s.t->p = (bf16_t*)p;
+    s.t->C = s.C;
+    p += (elems(s.lvl, s.C) * 2 + 255) & ~(size_t)255;
+  }
+  const size_t fstride = (hw * 4 + 63) & ~(size_t)63;
+  N.Y = (float*)p; N.Tt = (float*)(p + fstride); N.Mk = (float*)(p + 2 * fstride); N.dY = (float*)(p + 3 * fstride);
+  N.B = B; N.H = H; N.W = W;
+  return 0;
+}
+
+unsigned nblk(int64_t n, int t = 256) { return (unsigned)((n + t - 1) / t); }
+
+int conv(lepic_ctx* ctx, Net& N, const T4& in, int in_off, int cin, const bf16_t* w, const T4& out, int out_off,
+         int cout, int relu, const T4* mask, int mask_off, int B, int H, int W) {
+  ConvArgs A;
+  A.in = in.p; A.in_C = in.C; A.in_off = in_off; A.cin = cin;
+  A.w = w;
+  A.out = out.p; A.out_C = out.C; A.out_off = out_off; A.cout = cout;
+  A.mask = mask ? mask->p : nullptr; A.mask_C = mask ? mask->C : 0; A.mask_off = mask_off;
+  A.relu = relu; A.B = B; A.H = H; A.W = W;
+  const dim3 grid((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)B);
+  switch (cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 12: hipLaunchKernelGGL(k_conv3x3<12>, grid, dim3(256), 0, ctx->stream, A); break;
+    default: return ctx->fail(LEPIC_ERR_ARG, "unsupported channel count");
+  }
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int fwd_layer(lepic_ctx* ctx, Net& N, int l, const T4& in, int in_off, const T4& out, int out_off, int lvl) {
+  const Layer& L = N.L[l];
+  return conv(ctx, N, in, in_off, L.cin, N.wb + L.off, out, out_off, L.cout, 1, nullptr, 0, N.B, N.H >> lvl,
+              N.W >> lvl);
+}
+
+// input gradient of layer l: dz (cout channels) -> d_in (cin channels)
+int dgrad_layer(lepic_ctx* ctx, Net& N, int l, const T4& dz, int dz_off, const T4& din, int din_off,
+                const T4* mask, int mask_off, int lvl) {
+  const Layer& L = N.L[l];
+  return conv(ctx, N, dz, dz_off, L.cout, N.wt + L.off, din, din_off, L.cin, 0, mask, mask_off, N.B,
+              N.H >> lvl, N.W >> lvl);
+}
+
+int wgrad_layer(lepic_ctx* ctx, Net& N, int l, const T4& x, int x_off, const T4& dz, int dz_off, int lvl) {
+  const Layer& L = N.L[l];
+  WgradArgs A;
+  A.x = x.p; A.x_C = x.C; A.x_off = x_off; A.cin = L.cin;
+  A.dz = dz.p; A.dz_C = dz.C; A.dz_off = dz_off; A.cout = L.cout;
+  A.dw = N.g + L.off;
+  A.B = N.B; A.H = N.H >> lvl; A.W = N.W >> lvl;
+  const int tiles = (L.cout / 16) * (L.cin / 16);
+  const int rows = A.B * A.H;
+  A.rows_per_wave = std::max(1, (int)((int64_t)rows * tiles / 8192));
+  const int waves = (rows + A.rows_per_wave - 1) / A.rows_per_wave;
+  hipLaunchKernelGGL(k_wgrad, dim3((unsigned)((waves + 3) / 4), (unsigned)tiles), dim3(256), 0, ctx->stream, A);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int pool(lepic_ctx* ctx, Net& N, const T4& in, int in_off, const T4& out, int C, int lvl_out) {
+  const int Ho = N.H >> lvl_out, Wo = N.W >> lvl_out;
+  hipLaunchKernelGGL(k_pool, dim3(nblk((int64_t)N.B * Ho * Wo * C)), dim3(256), 0, ctx->stream, in.p, in.C, in_off,
+                     out.p, C, N.B, Ho, Wo);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int upsample(lepic_ctx* ctx, Net& N, const T4& in, int C, const T4& out, int lvl_out) {
+  const int H = N.H >> lvl_out, W = N.W >> lvl_out;
+  hipLaunchKernelGGL(k_up, dim3(nblk((int64_t)N.B * H * W * C)), dim3(256), 0, ctx->stream, in.p, C, out.p, out.C,
+                     N.B, H, W);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int forward(lepic_ctx* ctx, Net& N) {
+  int rc = 0;
+  auto F = [&](int l, const T4& in, int io, const T4& out, int oo, int lvl) {
+    if (!rc) rc = fwd_layer(ctx, N, l, in, io, out, oo, lvl);
+  };
+  F(ENC0A, N.X, 0, N.E0A, 0, 0);
+  F(ENC0B, N.E0A, 0, N.CAT0, 32, 0);
+  if (!rc) rc = pool(ctx, N, N.CAT0, 32, N.P0, 16, 1);
+  F(ENC1A, N.P0, 0, N.E1A, 0, 1);
+  F(ENC1B, N.E1A, 0, N.CAT1, 64, 1);
+  if (!rc) rc = pool(ctx, N, N.CAT1, 64, N.P1, 32, 2);
+  F(ENC2A, N.P1, 0, N.E2A, 0, 2);
+  F(ENC2B, N.E2A, 0, N.CAT2, 128, 2);
+  if (!rc) rc = pool(ctx, N, N.CAT2, 128, N.P2, 64, 3);
+  F(MIDA, N.P2, 0, N.MA, 0, 3);
+  F(MIDB, N.MA, 0, N.M, 0, 3);
+  if (!rc) rc = upsample(ctx, N, N.M, 128, N.CAT2, 2);
+  F(DEC2A, N.CAT2, 0, N.D2A, 0, 2);
+  F(DEC2B, N.D2A, 0, N.D2, 0, 2);
+  if (!rc) rc = upsample(ctx, N, N.D2, 64, N.CAT1, 1);
+  F(DEC1A, N.CAT1, 0, N.D1A, 0, 1);
+  F(DEC1B, N.D1A, 0, N.D1, 0, 1);
+  if (!rc) rc = upsample(ctx, N, N.D1, 32, N.CAT0, 0);
+  F(DEC0A, N.CAT0, 0, N.D0A, 0, 0);
+  F(DEC0B, N.D0A, 0, N.D0, 0, 0);
+  if (rc) return rc;
+  const int64_t hw = (int64_t)N.B * N.H * N.W;
+  hipLaunchKernelGGL(k_out, dim3(nblk(hw)), dim3(256), 0, ctx->stream, N.D0.p, (const float*)(N.w + N.nconv), N.Y,
+                     N.B, N.H, N.W);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int up_bwd(lepic_ctx* ctx, Net& N, const T4& g, const T4& act, const T4& out, int C, int lvl_out) {
+  const int Ho = N.H >> lvl_out, Wo = N.W >> lvl_out;
+  hipLaunchKernelGGL(k_up_bwd, dim3(nblk((int64_t)N.B * Ho * Wo * C)), dim3(256), 0, ctx->stream, g.p, g.C, act.p,
+                     out.p, C, N.B, Ho, Wo);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int skip_grad(lepic_ctx* ctx, Net& N, const T4& gcat, const T4& acat, int off, const T4& gp, int C, int lvl) {
+  const int H = N.H >> lvl, W = N.W >> lvl;
+  hipLaunchKernelGGL(k_skip_grad, dim3(nblk((int64_t)N.B * H * W * C)), dim3(256), 0, ctx->stream, gcat.p, acat.p,
+                     gcat.C, off, gp.p, C, N.B, H, W);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+// gradients of the loss (dY already set) into N.g
+int backward(lepic_ctx* ctx, Net& N) {
+  int rc = zero_async(ctx, N.g, N.nparam * 4);
+  if (rc) return rc;
+  const int64_t hw = (int64_t)N.B * N.H * N.W;
+  hipLaunchKernelGGL(k_out_bwd, dim3(nblk(hw)), dim3(256), 0, ctx->stream, N.D0.p, (const float*)(N.w + N.nconv),
+                     (const float*)N.dY, N.g + N.nconv, N.G_D0.p, N.B, N.H, N.W);
+  LEPIC_HIP(hipGetLastError());
+  auto WG = [&](int l, const T4& x, int xo, const T4& dz, int dzo, int lvl) {
+    if (!rc) rc = wgrad_layer(ctx, N, l, x, xo, dz, dzo, lvl);
+  };
+  auto DG = [&](int l, const T4& dz, int dzo, const T4& din, int dino, const T4* mask, int mo, int lvl) {
+    if (!rc) rc = dgrad_layer(ctx, N, l, dz, dzo, din, dino, mask, mo, lvl);
+  };
+  WG(DEC0B, N.D0A, 0, N.G_D0, 0, 0);     DG(DEC0B, N.G_D0, 0, N.G_D0A, 0, &N.D0A, 0, 0);
+  WG(DEC0A, N.CAT0, 0, N.G_D0A, 0, 0);   DG(DEC0A, N.G_D0A, 0, N.G_CAT0, 0, nullptr, 0, 0);
+  if (!rc) rc = up_bwd(ctx, N, N.G_CAT0, N.D1, N.G_D1, 32, 1);
+  WG(DEC1B, N.D1A, 0, N.G_D1, 0, 1);     DG(DEC1B, N.G_D1, 0, N.G_D1A, 0, &N.D1A, 0, 1);
+  WG(DEC1A, N.CAT1, 0, N.G_D1A, 0, 1);   DG(DEC1A, N.G_D1A, 0, N.G_CAT1, 0, nullptr, 0, 1);
+  if (!rc) rc = up_bwd(ctx, N, N.G_CAT1, N.D2, N.G_D2, 64, 2);
+  WG(DEC2B, N.D2A, 0, N.G_D2, 0, 2);     DG(DEC2B, N.G_D2, 0, N.G_D2A, 0, &N.D2A, 0, 2);
+  WG(DEC2A, N.CAT2, 0, N.G_D2A, 0, 2);   DG(DEC2A, N.G_D2A, 0, N.G_CAT2, 0, nullptr, 0, 2);
+  if (!rc) rc = up_bwd(ctx, N, N.G_CAT2, N.M, N.G_M, 128, 3);
+  WG(MIDB, N.MA, 0, N.G_M, 0, 3);        DG(MIDB, N.G_M, 0, N.G_MA, 0, &N.MA, 0, 3);
+  WG(MIDA, N.P2, 0, N.G_MA, 0, 3);       DG(MIDA, N.G_MA, 0, N.G_P2, 0, nullptr, 0, 3);
+  if (!rc) rc = skip_grad(ctx, N, N.G_CAT2, N.CAT2, 128, N.G_P2, 64, 2);
+  WG(ENC2B, N.E2A, 0, N.G_CAT2, 128, 2); DG(ENC2B, N.G_CAT2, 128, N.G_E2A, 0, &N.E2A, 0, 2);
+  WG(ENC2A, N.P1, 0, N.G_E2A, 0, 2);     DG(ENC2A, N.G_E2A, 0, N.G_P1, 0, nullptr, 0, 2);
+  if (!rc) rc = skip_grad(ctx, N, N.G_CAT1, N.CAT1, 64, N.G_P1, 32, 1);
+  WG(ENC1B, N.E1A, 0, N.G_CAT1, 64, 1);  DG(ENC1B, N.G_CAT1, 64, N.G_E1A, 0, &N.E1A, 0, 1);
+  WG(ENC1A, N.P0, 0, N.G_E1A, 0, 1);     DG(ENC1A, N.G_E1A, 0, N.G_P0, 0, nullptr, 0, 1);
+  if (!rc) rc = skip_grad(ctx, N, N.G_CAT0, N.CAT0, 32, N.G_P0, 16, 0);
+  WG(ENC0B, N.E0A, 0, N.G_CAT0, 32, 0);  DG(ENC0B, N.G_CAT0, 32, N.G_E0A, 0, &N.E0A, 0, 0);
+  WG(ENC0A, N.X, 0, N.G_E0A, 0, 0);
+  return rc;
+}
+
+int refresh_bf16(lepic_ctx* ctx, Net& N) {
+  for (int l = 0; l < 14; ++l) {
+    const Layer& L = N.L[l];
+    const int64_t n = (int64_t)L.cout * 9 * L.cin;
+    hipLaunchKernelGGL(k_wcast, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)(N.w + L.off), N.wb + L.off,
+                       N.wt + L.off, L.cout, L.cin);
+  }
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+// logical flat layout (the oracle's) <-> GPU layout (input layer padded)
+void logical_to_gpu(const Net& N, const float* flat, std::vector<float>& gpu) {
+  gpu.assign(N.nparam, 0.f);
+  size_t o = 0;
+  for (int l = 0; l < 14; ++l) {
+    const Layer& L = N.L[l];
+    for (int co = 0; co < L.cout; ++co)
+      for (int t = 0; t < 9; ++t)
+        for (int ci = 0; ci < L.cin_logical; ++ci)
+          gpu[L.off + ((size_t)co * 9 + t) * L.cin + ci] = flat[o++];
+  }
+  for (int c = 0; c < 16; ++c) gpu[N.nconv + c] = flat[o++];
+}
+
+void gpu_to_logical(const Net& N, const float* gpu, float* flat) {
+  size_t o = 0;
+  for (int l = 0; l < 14; ++l) {
+    const Layer& L = N.L[l];
+    for (int co = 0; co < L.cout; ++co)
+      for (int t = 0; t < 9; ++t)
+        for (int ci = 0; ci < L.cin_logical; ++ci) flat[o++] = gpu[L.off + ((size_t)co * 9 + t) * L.cin + ci];
+  }
+  for (int c = 0; c < 16; ++c) flat[o++] = gpu[N.nconv + c];
+}
+
+size_t logical_size(const Net& N) {
+  size_t n = 16;
+  for (int l = 0; l < 14; ++l) n += (size_t)N.L[l].cout * 9 * N.L[l].cin_logical;
+  return n;
+}
+
+Net* need_net(lepic_ctx* ctx) { return ctx ? ctx->net : nullptr; }
+
+// copy host (B,H,W,2) input and optional target/mask into the batch tensors
+int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const float* mk, int B, int H, int W) {
+  if (int rc = alloc_tensors(ctx, N, B, H, W)) return rc;
+  N.B = B;
+  const size_t hw = (size_t)H * W;
+  std::vector<float> a(hw), c(hw);
+  float* dx = nullptr;
+  LEPIC_HIP(hipMalloc(&dx, 2 * hw * 4));
+  for (int b = 0; b < B; ++b) {
+    for (size_t q = 0; q < hw; ++q) { a[q] = x[((size_t)b * hw + q) * 2]; c[q] = x[((size_t)b * hw + q) * 2 + 1]; }
+    LEPIC_HIP(hipMemcpy(dx, a.data(), hw * 4, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpy(dx + hw, c.data(), hw * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)dx,
+                       (const float*)(dx + hw), 1.0f, N.X.p, b, H, W, (const float*)nullptr, 0.f, (float*)nullptr,
+                       (float*)nullptr);
+    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  }
+  LEPIC_HIP(hipFree(dx));
+  if (t) {
+    LEPIC_HIP(hipMemcpy(N.Tt, t, (size_t)B * hw * 4, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpy(N.Mk, mk, (size_t)B * hw * 4, hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
+int loss_launch(lepic_ctx* ctx, Net& N, double nm) {
+  const int64_t n = (int64_t)N.B * N.H * N.W;
+  if (int rc = zero_async(ctx, N.d_loss, 8)) return rc;
+  hipLaunchKernelGGL(k_loss, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Tt,
+                     (const float*)N.Mk, (float)(1.0 / nm), N.dY, N.d_loss, n);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+int adam_step(lepic_ctx* ctx, Net& N, float lr) {
+  N.adam_t += 1;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - (float)std::pow(b1, (double)N.adam_t), c2 = 1.f - (float)std::pow(b2, (double)N.adam_t);
+  hipLaunchKernelGGL(k_adam, dim3(nblk((int64_t)N.nparam)), dim3(256), 0, ctx->stream, N.w, N.m, N.v,
+                     (const float*)N.g, (int64_t)N.nparam, lr, b1, b2, c1, c2, 1e-8f);
+  LEPIC_HIP(hipGetLastError());
+  return refresh_bf16(ctx, N);
+}
+
+double read_d(lepic_ctx* ctx, const double* d) {
+  double v = 0.0;
+  if (hipMemcpyAsync(ctx->h_pinned, d, 8, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess) return NAN;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return NAN;
+  std::memcpy(&v, ctx->h_pinned, 8);
+  return v;
+}
+
+}  // namespace
+
+namespace lepic {
+
+void net_destroy(lepic_ctx* ctx) {
+  Net* N = ctx->net;
+  if (!N) return;
+  void* ptrs[] = {N->w, N->m, N->v, N->g, N->wb, N->wt, N->arena, N->s_x, N->s_t, N->mask_img, N->rhs_last,
+                  N->phi_last, N->img_tmp, N->d_tmp, N->d_loss};
+  for (void* p : ptrs) if (p) (void)hipFree(p);
+  delete N;
+  ctx->net = nullptr;
+}
+
+// After a converged solve: record the sample (change since the last solve)
+// when recording, and remember this solve's rhs and phi.
+int net_after_solve(lepic_ctx* ctx) {
+  Net* N = ctx->net;
+  if (!N) return 0;
+  const int n1 = ctx->g.n1, n2 = ctx->g.n2;
+  const int64_t hw = (int64_t)N->Hs * N->Ws;
+  if (N->recording && N->have_last && N->cap > 0) {
+    const int slot = N->head;
+    float* sx = N->s_x + (size_t)slot * hw;
+    float* st = N->s_t + (size_t)slot * hw;
+    hipLaunchKernelGGL(k_diff_to_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const double*)ctx->rhs,
+                       (const double*)N->rhs_last, 0, n1, n2, N->Ws, N->Hs, sx);
+    hipLaunchKernelGGL(k_diff_to_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const double*)ctx->phi,
+                       (const double*)N->phi_last, 1, n1, n2, N->Ws, N->Hs, st);
+    if (int rc = zero_async(ctx, N->d_tmp, 16)) return rc;
+    hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)sx,
+                       (const float*)N->mask_img, hw, N->d_tmp);
+    hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)st,
+                       (const float*)N->mask_img, hw, N->d_tmp + 1);
+    LEPIC_HIP(hipGetLastError());
+    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, N->d_tmp, 16, hipMemcpyDeviceToHost, ctx->stream));
+    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    double s2[2];
+    std::memcpy(s2, ctx->h_pinned, 16);
+    const double rx = std::sqrt(s2[0] / N->nmask), rt = std::sqrt(s2[1] / N->nmask);
+    if (rx > 0.0 && rt > 0.0) {
+      N->s_sx[slot] = rx;
+      N->s_st[slot] = rt;
+      N->head = (N->head + 1) % N->cap;
+      N->count = std::min(N->count + 1, N->cap);
+    }
+  }
+  LEPIC_HIP(hipMemcpyAsync(N->rhs_last, ctx->rhs, ctx->rhs_n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  LEPIC_HIP(hipMemcpyAsync(N->phi_last, ctx->phi, ctx->nodes * 8, hipMemcpyDeviceToDevice, ctx->stream));
+  N->have_last = true;
+  return 0;
+}
+
+// Starting guess phi_last + ratio * rms(drhs) * net(drhs / rms(drhs), mask)
+// into `out`; returns 1 if no guess can be made (untrained or no change).
+int net_guess(lepic_ctx* ctx, double* out) {
+  Net* N = ctx->net;
+  if (!N || !N->trained || !N->have_last) return 1;
+  const int n1 = ctx->g.n1, n2 = ctx->g.n2;
+  const int64_t hw = (int64_t)N->Hs * N->Ws;
+  if (int rc = alloc_tensors(ctx, *N, 1, N->Hs, N->Ws)) return rc;
+  N->B = 1;
+  hipLaunchKernelGGL(k_diff_to_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const double*)ctx->rhs,
+                     (const double*)N->rhs_last, 0, n1, n2, N->Ws, N->Hs, N->img_tmp);
+  if (int rc = zero_async(ctx, N->d_tmp, 16)) return rc;
+  hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->img_tmp,
+                     (const float*)N->mask_img, hw, N->d_tmp);
+  LEPIC_HIP(hipGetLastError());
+  const double rx = std::sqrt(read_d(ctx, N->d_tmp) / N->nmask);
+  if (!(rx > 0.0)) return 1;
+  hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->img_tmp,
+                     (const float*)N->mask_img, (float)(1.0 / rx), N->X.p, 0, N->Hs, N->Ws, (const float*)nullptr,
+                     0.f, (float*)nullptr, (float*)nullptr);
+  LEPIC_HIP(hipGetLastError());
+  if (int rc = forward(ctx, *N)) return rc;
+  hipLaunchKernelGGL(k_apply_guess, dim3(nblk(ctx->nodes)), dim3(256), 0, ctx->stream, (const double*)N->phi_last,
+                     (const float*)N->Y, N->ratio * rx, (const int8_t*)ctx->bc, n1, n2, N->Ws, out);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace lepic
+
+extern "C" {
+
+int lepic_net_create(lepic_ctx* ctx, int32_t capacity, uint64_t seed) {
+  if (!ctx || capacity < 0) return LEPIC_ERR_ARG;
+  if (!ctx->boundary_set) return ctx->fail(LEPIC_ERR_STATE, "set the boundary before creating the network");
+  if (ctx->net) net_destroy(ctx);
+  Net* N = new Net();
+  ctx->net = N;
+  size_t off = 0;
+  for (int l = 0; l < 14; ++l) {
+    N->L[l] = kLayers[l];
+    N->L[l].off = off;
+    off += (size_t)N->L[l].cout * 9 * N->L[l].cin;
+  }
+  N->nconv = off;
+  N->nparam = off + 16;
+  LEPIC_HIP(hipMalloc(&N->w, N->nparam * 4));
+  LEPIC_HIP(hipMalloc(&N->m, N->nparam * 4));
+  LEPIC_HIP(hipMalloc(&N->v, N->nparam * 4));
+  LEPIC_HIP(hipMalloc(&N->g, N->nparam * 4));
+  LEPIC_HIP(hipMalloc(&N->wb, N->nconv * 2));
+  LEPIC_HIP(hipMalloc(&N->wt, N->nconv * 2));
+  LEPIC_HIP(hipMemset(N->m, 0, N->nparam * 4));
+  LEPIC_HIP(hipMemset(N->v, 0, N->nparam * 4));
+  LEPIC_HIP(hipMalloc(&N->d_loss, 8));
+  LEPIC_HIP(hipMalloc(&N->d_tmp, 64));
+  // He-normal initialisation on the logical fan-in
+  N->rng.seed(seed);
+  std::normal_distribution<float> nd(0.f, 1.f);
+  std::vector<float> flat(logical_size(*N));
+  size_t o = 0;
+  for (int l = 0; l < 14; ++l) {
+    const Layer& L = N->L[l];
+    const float s = std::sqrt(2.0f / (9.0f * L.cin_logical));
+    for (size_t k = 0; k < (size_t)L.cout * 9 * L.cin_logical; ++k) flat[o++] = s * nd(N->rng);
+  }
+  for (int c = 0; c < 16; ++c) flat[o++] = 0.25f * nd(N->rng);
+  std::vector<float> gpu;
+  logical_to_gpu(*N, flat.data(), gpu);
+  LEPIC_HIP(hipMemcpy(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice));
+  if (int rc = refresh_bf16(ctx, *N)) return rc;
+  // samples on the node grid image
+  N->Hs = round8(ctx->g.n2 + 1);
+  N->Ws = round8(ctx->g.n1 + 1);
+  const int64_t hw = (int64_t)N->Hs * N->Ws;
+  N->cap = capacity;
+  N->s_sx.assign(std::max(capacity, 1), 0.0);
+  N->s_st.assign(std::max(capacity, 1), 0.0);
+  if (capacity > 0) {
+    LEPIC_HIP(hipMalloc(&N->s_x, (size_t)capacity * hw * 4));
+    LEPIC_HIP(hipMalloc(&N->s_t, (size_t)capacity * hw * 4));
+  }
+  LEPIC_HIP(hipMalloc(&N->mask_img, hw * 4));
+  LEPIC_HIP(hipMalloc(&N->img_tmp, hw * 4));
+  LEPIC_HIP(hipMalloc(&N->rhs_last, ctx->rhs_n * 8));
+  LEPIC_HIP(hipMalloc(&N->phi_last, ctx->nodes * 8));
+  hipLaunchKernelGGL(k_mask_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const int8_t*)ctx->bc, ctx->g.n1,
+                     ctx->g.n2, N->Ws, N->Hs, N->mask_img);
+  LEPIC_HIP(hipGetLastError());
+  if (int rc = zero_async(ctx, N->d_tmp, 8)) return rc;
+  hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->mask_img,
+                     (const float*)N->mask_img, hw, N->d_tmp);
+  N->nmask = read_d(ctx, N->d_tmp);
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  return 0;
+}
+
+int lepic_net_recording(lepic_ctx* ctx, int32_t on) {
+  Net* N = need_net(ctx);
+  if (!N) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  N->recording = on != 0;
+  return 0;
+}
+
+int lepic_net_samples(lepic_ctx* ctx, int32_t* n) {
+  Net* N = need_net(ctx);
+  if (!N || !n) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  *n = N->count;
+  return 0;
+}
+
+int lepic_net_train(lepic_ctx* ctx, int32_t iters, int32_t batch, float lr, float* loss_first, float* loss_last) {
+  Net* N = need_net(ctx);
+  if (!N) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (iters < 0 || batch < 1 || batch > 64 || !(lr > 0.f)) return LEPIC_ERR_ARG;
+  if (N->count < 1) return ctx->fail(LEPIC_ERR_STATE, "no recorded samples");
+  // output scale: mean ratio rms(dphi)/rms(drhs) over the samples
+  double r = 0.0;
+  for (int s = 0; s < N->count; ++s) r += N->s_st[s] / N->s_sx[s];
+  N->ratio = r / N->count;
+  if (int rc = alloc_tensors(ctx, *N, batch, N->Hs, N->Ws)) return rc;
+  N->B = batch;
+  const int64_t hw = (int64_t)N->Hs * N->Ws;
+  std::uniform_int_distribution<int> pick(0, N->count - 1);
+  for (int it = 0; it < iters; ++it) {
+    for (int b = 0; b < batch; ++b) {
+      const int s = pick(N->rng);
+      hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream,
+                         (const float*)(N->s_x + (size_t)s * hw), (const float*)N->mask_img,
+                         (float)(1.0 / N->s_sx[s]), N->X.p, b, N->Hs, N->Ws, (const float*)(N->s_t + (size_t)s * hw),
+                         (float)(1.0 / (N->ratio * N->s_sx[s])), N->Tt, N->Mk);
+    }
+    LEPIC_HIP(hipGetLastError());
+    if (int rc = forward(ctx, *N)) return rc;
+    if (int rc = loss_launch(ctx, *N, N->nmask * batch)) return rc;
+    if (it == 0 && loss_first) *loss_first = (float)(read_d(ctx, N->d_loss) / (N->nmask * batch));
+    if (int rc = backward(ctx, *N)) return rc;
+    if (int rc = adam_step(ctx, *N, lr)) return rc;
+    if (it == iters - 1 && loss_last) {
+      // loss after the last update, on the same batch
+      if (int rc = forward(ctx, *N)) return rc;
+      if (int rc = loss_launch(ctx, *N, N->nmask * batch)) return rc;
+      *loss_last = (float)(read_d(ctx, N->d_loss) / (N->nmask * batch));
+    }
+  }
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  N->trained = iters > 0 || N->trained;
+  return 0;
+}
+
+int lepic_net_set_params(lepic_ctx* ctx, const float* flat, int64_t n) {
+  Net* N = need_net(ctx);
+  if (!N || !flat) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if ((size_t)n != logical_size(*N)) return ctx->fail(LEPIC_ERR_ARG, "parameter count mismatch");
+  std::vector<float> gpu;
+  logical_to_gpu(*N, flat, gpu);
+  LEPIC_HIP(hipMemcpy(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemset(N->m, 0, N->nparam * 4));
+  LEPIC_HIP(hipMemset(N->v, 0, N->nparam * 4));
+  N->adam_t = 0;
+  return refresh_bf16(ctx, *N);
+}
+
+int lepic_net_get_params(lepic_ctx* ctx, float* flat, int64_t n) {
+  Net* N = need_net(ctx);
+  if (!N || !flat) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if ((size_t)n != logical_size(*N)) return ctx->fail(LEPIC_ERR_ARG, "parameter count mismatch");
+  std::vector<float> gpu(N->nparam);
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  LEPIC_HIP(hipMemcpy(gpu.data(), N->w, N->nparam * 4, hipMemcpyDeviceToHost));
+  gpu_to_logical(*N, gpu.data(), flat);
+  return 0;
+}
+
+// forward pass on a host batch x (B,H,W,2); y (B,H,W)
+int lepic_net_forward(lepic_ctx* ctx, const float* x, int32_t B, int32_t H, int32_t W, float* y) {
+  Net* N = need_net(ctx);
+  if (!N || !x || !y) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (B < 1 || H < 8 || W < 8 || H % 8 || W % 8) return ctx->fail(LEPIC_ERR_ARG, "B >= 1, H and W multiples of 8");
+  if (int rc = load_batch(ctx, *N, x, nullptr, nullptr, B, H, W)) return rc;
+  if (int rc = forward(ctx, *N)) return rc;
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  LEPIC_HIP(hipMemcpy(y, N->Y, (size_t)B * H * W * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// loss and gradient (logical flat layout) of the masked MSE on a host batch
+int lepic_net_loss_grad(lepic_ctx* ctx, const float* x, const float* t, const float* mk, int32_t B, int32_t H,
+                        int32_t W, double* loss, float* grad, int64_t n) {
+  Net* N = need_net(ctx);
+  if (!N || !x || !t || !mk || !loss || !grad) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (B < 1 || H < 8 || W < 8 || H % 8 || W % 8) return ctx->fail(LEPIC_ERR_ARG, "B >= 1, H and W multiples of 8");
+  if ((size_t)n != logical_size(*N)) return ctx->fail(LEPIC_ERR_ARG, "parameter count mismatch");
+  if (int rc = load_batch(ctx, *N, x, t, mk, B, H, W)) return rc;
+  double nm = 0.0;
+  for (int64_t q = 0; q < (int64_t)B * H * W; ++q) nm += mk[q];
+  if (!(nm > 0.0)) return ctx->fail(LEPIC_ERR_ARG, "empty mask");
+  if (int rc = forward(ctx, *N)) return rc;
+  if (int rc = loss_launch(ctx, *N, nm)) return rc;
+  if (int rc = backward(ctx, *N)) return rc;
+  *loss = read_d(ctx, N->d_loss) / nm;
+  std::vector<float> gpu(N->nparam);
+  LEPIC_HIP(hipMemcpy(gpu.data(), N->g, N->nparam * 4, hipMemcpyDeviceToHost));
+  gpu_to_logical(*N, gpu.data(), grad);
+  return 0;
+}
+
+}  // extern "C"

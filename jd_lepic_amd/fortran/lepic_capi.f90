@@ -7,7 +7,8 @@ module lepic_capi
 
   integer(c_int32_t), parameter :: LEPIC_FP64 = 0, LEPIC_MIXED = 1
   integer(c_int32_t), parameter :: LEPIC_MCC_CHECK = 0, LEPIC_MCC_PROD = 1
-  integer(c_int32_t), parameter :: LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2
+  integer(c_int32_t), parameter :: LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2, &
+       LEPIC_START_NET = 3
   integer(c_int32_t), parameter :: LEPIC_GRID_PHI = 0, LEPIC_GRID_RHS = 1, LEPIC_GRID_E = 2, &
        LEPIC_GRID_NP = 3, LEPIC_GRID_NPRED = 4, LEPIC_GRID_SINK = 5
 
@@ -26,6 +27,7 @@ module lepic_capi
      integer(c_int32_t) :: vcycles, status
      real(c_double) :: res, ktot, res0
      integer(c_int32_t) :: start_used, nlog
+     real(c_double) :: res0_alt
   end type lepic_solve_info
 
   type, bind(C) :: lepic_mcc_info
