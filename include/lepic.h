@@ -232,10 +232,15 @@ int lepic_net_train(lepic_ctx *ctx, int32_t iters, int32_t batch, float lr, floa
 /* parameters in the flat layout [layer][cout][tap][cin] + 16 output weights */
 int lepic_net_set_params(lepic_ctx *ctx, const float *flat, int64_t n);
 int lepic_net_get_params(lepic_ctx *ctx, float *flat, int64_t n);
-/* forward pass and masked-MSE gradient on a host batch x(B,H,W,2) */
+/* forward pass, and the training loss with its gradient, on a host batch
+ * x(B,H,W,2): masked MSE against target plus lam times the masked squared
+ * Poisson residual of the prediction, A(mask*y) - x[..,0] with the 5-point
+ * stencil cw*(W+E) + cn*(S+N) - 2(cw+cn)*centre (lepic_net_train uses the
+ * finest level's coefficients in the network's units and lam = 1) */
 int lepic_net_forward(lepic_ctx *ctx, const float *x, int32_t B, int32_t H, int32_t W, float *y);
 int lepic_net_loss_grad(lepic_ctx *ctx, const float *x, const float *target, const float *mask, int32_t B,
-                        int32_t H, int32_t W, double *loss, float *grad, int64_t n);
+                        int32_t H, int32_t W, float lam, float cw, float cn, double *loss, float *grad,
+                        int64_t n);
 
 int lepic_step(lepic_ctx *ctx, int32_t it, lepic_step_args *a);
 

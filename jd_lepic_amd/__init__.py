@@ -238,7 +238,7 @@ class Context:
         self._check(self.lib.lepic_net_forward(self._ctx, x.ctypes.data_as(fp), B, H, W, y.ctypes.data_as(fp)))
         return y
 
-    def net_loss_grad(self, x, target, mask, nparam):
+    def net_loss_grad(self, x, target, mask, nparam, lam=0.0, cw=0.0, cn=0.0):
         x = np.ascontiguousarray(x, np.float32)
         t = np.ascontiguousarray(target, np.float32)
         m = np.ascontiguousarray(mask, np.float32)
@@ -247,7 +247,7 @@ class Context:
         loss = C.c_double()
         fp = C.POINTER(C.c_float)
         self._check(self.lib.lepic_net_loss_grad(self._ctx, x.ctypes.data_as(fp), t.ctypes.data_as(fp),
-                                                 m.ctypes.data_as(fp), B, H, W, C.byref(loss),
+                                                 m.ctypes.data_as(fp), B, H, W, lam, cw, cn, C.byref(loss),
                                                  g.ctypes.data_as(fp), nparam))
         return float(loss.value), g
 

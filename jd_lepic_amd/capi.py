@@ -87,7 +87,8 @@ SIGNATURES = {
     "lepic_net_forward": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int32, C.c_int32, C.c_int32,
                                     C.POINTER(C.c_float)]),
     "lepic_net_loss_grad": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
-                                      C.c_int32, C.c_int32, C.c_int32, _dp, C.POINTER(C.c_float), C.c_int64]),
+                                      C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_float, C.c_float, _dp,
+                                      C.POINTER(C.c_float), C.c_int64]),
     "lepic_get_kernel_ms": (C.c_int, [_ctxp, C.c_int32, C.POINTER(C.c_float)]),
     "lepic_set_boundary": (C.c_int, [_ctxp, _ip, _dp]),
     "lepic_set_bfield": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp]),

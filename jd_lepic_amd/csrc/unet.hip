@@ -266,6 +266,43 @@ __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
   if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
 }
 
+// Poisson residual of the prediction in the network's units:
+// r = mask * (A(mask*y) - x), A = cw*(W+E) + cn*(S+N) - 2(cw+cn)*centre;
+// adds lam * r^2 to the loss
+__device__ __forceinline__ float stencil_at(const float* u, const float* mk, int b, int x, int y, int W, int H,
+                                            float cw, float cn, bool masked) {
+  auto at = [&](int xx, int yy) -> float {
+    if (xx < 0 || yy < 0 || xx >= W || yy >= H) return 0.f;
+    const int64_t k = ((int64_t)b * H + yy) * W + xx;
+    return masked ? u[k] * mk[k] : u[k];
+  };
+  return cw * (at(x - 1, y) + at(x + 1, y)) + cn * (at(x, y - 1) + at(x, y + 1)) - 2.f * (cw + cn) * at(x, y);
+}
+
+__global__ void k_res_fwd(const float* __restrict__ y, const float* __restrict__ xf, const float* __restrict__ mk,
+                          float cw, float cn, float lam, int B, int H, int W, float* __restrict__ r,
+                          double* __restrict__ loss) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double l = 0.0;
+  if (q < (int64_t)B * H * W) {
+    const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
+    const float v = mk[q] != 0.f ? stencil_at(y, mk, b, x, yy, W, H, cw, cn, true) - xf[q] : 0.f;
+    r[q] = v;
+    l = (double)lam * (double)v * (double)v;
+  }
+  l = wave_sum(l);
+  if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+}
+
+// dL/dy += mask * A(r) * scale   (A is symmetric)
+__global__ void k_res_bwd(const float* __restrict__ r, const float* __restrict__ mk, float cw, float cn,
+                          float scale, int B, int H, int W, float* __restrict__ dy) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)B * H * W) return;
+  const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
+  if (mk[q] != 0.f) dy[q] += scale * stencil_at(r, mk, b, x, yy, W, H, cw, cn, false);
+}
+
 // gradient of the output layer: dw[c] += sum d0*dy; g_d0 = bf16(dy*w) * (d0 > 0)
 __global__ void k_out_bwd(const bf16_t* __restrict__ d0, const float* __restrict__ w, const float* __restrict__ dy,
                           float* __restrict__ dw, bf16_t* __restrict__ g, int B, int H, int W) {
@@ -358,17 +395,39 @@ __global__ void k_sumsq(const float* __restrict__ v, const float* __restrict__ m
 // target v * st (FP32)
 __global__ void k_fill_input(const float* __restrict__ xin, const float* __restrict__ mk, float sx,
                              bf16_t* __restrict__ X, int b, int H, int W, const float* __restrict__ tin, float st,
-                             float* __restrict__ T, float* __restrict__ M) {
+                             float* __restrict__ T, float* __restrict__ M, float* __restrict__ Xf) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (int64_t)H * W) return;
   const int x = (int)(q % W), y = (int)(q / W);
   bf16_t* p = X + px(b, y, x, H, W) * 16;
   p[0] = f2bf(xin[q] * sx);
   p[1] = f2bf(mk[q]);
+  if (Xf) Xf[(int64_t)b * H * W + q] = xin[q] * sx;
   if (T) {
     T[(int64_t)b * H * W + q] = tin[q] * st;
     M[(int64_t)b * H * W + q] = mk[q];
   }
+}
+
+// one pass of the 3x3 binomial filter [1 2 1]x[1 2 1]/16 over the image,
+// zero outside the mask (walls keep no correction)
+__global__ void k_smooth(const float* __restrict__ in, const float* __restrict__ mk, int W, int H,
+                         float* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)H * W) return;
+  const int x = (int)(q % W), y = (int)(q / W);
+  float s = 0.f;
+  if (mk[q] != 0.f) {
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = x + dx, yy = y + dy;
+        if (xx < 0 || yy < 0 || xx >= W || yy >= H) continue;
+        const int64_t k = (int64_t)yy * W + xx;
+        s += (float)((2 - (dx != 0 ? 1 : 0)) * (2 - (dy != 0 ? 1 : 0))) * in[k] * mk[k];
+      }
+    s *= 1.0f / 16.0f;
+  }
+  out[q] = s;
 }
 
 // guess = phi + scale * y on interior nodes
@@ -428,6 +487,8 @@ struct Net {
   float* Tt = nullptr;
   float* Mk = nullptr;
   float* dY = nullptr;
+  float* Xf = nullptr;               // FP32 normalized input channel 0
+  float* Rr = nullptr;               // Poisson residual of the prediction
   double* d_loss = nullptr;
   // samples on the node grid image (Hs x Ws)
   int Hs = 0, Ws = 0;
@@ -453,6 +514,8 @@ struct Net {
 namespace {
 
 int round8(int v) { return (v + 7) / 8 * 8; }
+constexpr int kSmoothPasses = 2;   // binomial low-pass passes on the predicted correction
+constexpr float kResLambda = 1.0f; // weight of the Poisson-residual term of the training loss
 
 int alloc_tensors(lepic_ctx* ctx, Net& N, int B, int H, int W) {
   if (N.arena && B <= N.B && H == N.H && W == N.W) return 0;
@@ -476,7 +539,7 @@ int alloc_tensors(lepic_ctx* ctx, Net& N, int B, int H, int W) {
   size_t total = 0;
   for (const Spec& s : specs) total += (elems(s.lvl, s.C) * 2 + 255) & ~(size_t)255;
   const size_t hw = (size_t)B * H * W;
-  const size_t fbytes = 4 * ((hw * 4 + 63) & ~(size_t)63);
+  const size_t fbytes = 6 * ((hw * 4 + 63) & ~(size_t)63);
   LEPIC_HIP(hipMalloc(&N.arena, total + fbytes + 256));
   LEPIC_HIP(hipMemset(N.arena, 0, total + fbytes + 256));
   char* p = (char*)N.arena;
@@ -487,6 +550,7 @@ int alloc_tensors(lepic_ctx* ctx, Net& N, int B, int H, int W) {
   }
   const size_t fstride = (hw * 4 + 63) & ~(size_t)63;
   N.Y = (float*)p; N.Tt = (float*)(p + fstride); N.Mk = (float*)(p + 2 * fstride); N.dY = (float*)(p + 3 * fstride);
+  N.Xf = (float*)(p + 4 * fstride); N.Rr = (float*)(p + 5 * fstride);
   N.B = B; N.H = H; N.W = W;
   return 0;
 }
@@ -706,7 +770,7 @@ int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const flo
     LEPIC_HIP(hipMemcpy(dx + hw, c.data(), hw * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)dx,
                        (const float*)(dx + hw), 1.0f, N.X.p, b, H, W, (const float*)nullptr, 0.f, (float*)nullptr,
-                       (float*)nullptr);
+                       (float*)nullptr, N.Xf);
     LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   LEPIC_HIP(hipFree(dx));
@@ -717,11 +781,18 @@ int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const flo
   return 0;
 }
 
-int loss_launch(lepic_ctx* ctx, Net& N, double nm) {
+// loss (sum over the batch; the caller divides by nm) and dL/dy in N.dY
+int loss_launch(lepic_ctx* ctx, Net& N, double nm, float lam, float cw, float cn) {
   const int64_t n = (int64_t)N.B * N.H * N.W;
   if (int rc = zero_async(ctx, N.d_loss, 8)) return rc;
   hipLaunchKernelGGL(k_loss, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Tt,
                      (const float*)N.Mk, (float)(1.0 / nm), N.dY, N.d_loss, n);
+  if (lam > 0.f) {
+    hipLaunchKernelGGL(k_res_fwd, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Xf,
+                       (const float*)N.Mk, cw, cn, lam, N.B, N.H, N.W, N.Rr, N.d_loss);
+    hipLaunchKernelGGL(k_res_bwd, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Rr, (const float*)N.Mk,
+                       cw, cn, (float)(2.0 * lam / nm), N.B, N.H, N.W, N.dY);
+  }
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -816,11 +887,22 @@ int net_guess(lepic_ctx* ctx, double* out) {
   if (!(rx > 0.0)) return 1;
   hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->img_tmp,
                      (const float*)N->mask_img, (float)(1.0 / rx), N->X.p, 0, N->Hs, N->Ws, (const float*)nullptr,
-                     0.f, (float*)nullptr, (float*)nullptr);
+                     0.f, (float*)nullptr, (float*)nullptr, (float*)nullptr);
   LEPIC_HIP(hipGetLastError());
   if (int rc = forward(ctx, *N)) return rc;
+  // the correction is low-passed before use: pixel-scale noise of the
+  // prediction would dominate the residual (the Laplacian weights it by
+  // 1/h^2) although the multigrid removes it in a sweep
+  for (int pass = 0; pass < kSmoothPasses; ++pass) {
+    const float* src = pass % 2 == 0 ? N->Y : N->img_tmp;
+    float* dst = pass % 2 == 0 ? N->img_tmp : N->Y;
+    hipLaunchKernelGGL(k_smooth, dim3(nblk(hw)), dim3(256), 0, ctx->stream, src, (const float*)N->mask_img, N->Ws,
+                       N->Hs, dst);
+  }
+  LEPIC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_apply_guess, dim3(nblk(ctx->nodes)), dim3(256), 0, ctx->stream, (const double*)N->phi_last,
-                     (const float*)N->Y, N->ratio * rx, (const int8_t*)ctx->bc, n1, n2, N->Ws, out);
+                     (const float*)(kSmoothPasses % 2 ? N->img_tmp : N->Y), N->ratio * rx, (const int8_t*)ctx->bc,
+                     n1, n2, N->Ws, out);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -920,6 +1002,8 @@ int lepic_net_train(lepic_ctx* ctx, int32_t iters, int32_t batch, float lr, floa
   if (int rc = alloc_tensors(ctx, *N, batch, N->Hs, N->Ws)) return rc;
   N->B = batch;
   const int64_t hw = (int64_t)N->Hs * N->Ws;
+  // the finest-level stencil in the network's units: ratio * A y = x
+  const float cw = (float)(N->ratio * ctx->lv[1].aw), cn = (float)(N->ratio * ctx->lv[1].an);
   std::uniform_int_distribution<int> pick(0, N->count - 1);
   for (int it = 0; it < iters; ++it) {
     for (int b = 0; b < batch; ++b) {
@@ -927,18 +1011,18 @@ int lepic_net_train(lepic_ctx* ctx, int32_t iters, int32_t batch, float lr, floa
       hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream,
                          (const float*)(N->s_x + (size_t)s * hw), (const float*)N->mask_img,
                          (float)(1.0 / N->s_sx[s]), N->X.p, b, N->Hs, N->Ws, (const float*)(N->s_t + (size_t)s * hw),
-                         (float)(1.0 / (N->ratio * N->s_sx[s])), N->Tt, N->Mk);
+                         (float)(1.0 / (N->ratio * N->s_sx[s])), N->Tt, N->Mk, N->Xf);
     }
     LEPIC_HIP(hipGetLastError());
     if (int rc = forward(ctx, *N)) return rc;
-    if (int rc = loss_launch(ctx, *N, N->nmask * batch)) return rc;
+    if (int rc = loss_launch(ctx, *N, N->nmask * batch, kResLambda, cw, cn)) return rc;
     if (it == 0 && loss_first) *loss_first = (float)(read_d(ctx, N->d_loss) / (N->nmask * batch));
     if (int rc = backward(ctx, *N)) return rc;
     if (int rc = adam_step(ctx, *N, lr)) return rc;
     if (it == iters - 1 && loss_last) {
       // loss after the last update, on the same batch
       if (int rc = forward(ctx, *N)) return rc;
-      if (int rc = loss_launch(ctx, *N, N->nmask * batch)) return rc;
+      if (int rc = loss_launch(ctx, *N, N->nmask * batch, kResLambda, cw, cn)) return rc;
       *loss_last = (float)(read_d(ctx, N->d_loss) / (N->nmask * batch));
     }
   }
@@ -985,7 +1069,7 @@ int lepic_net_forward(lepic_ctx* ctx, const float* x, int32_t B, int32_t H, int3
 
 // loss and gradient (logical flat layout) of the masked MSE on a host batch
 int lepic_net_loss_grad(lepic_ctx* ctx, const float* x, const float* t, const float* mk, int32_t B, int32_t H,
-                        int32_t W, double* loss, float* grad, int64_t n) {
+                        int32_t W, float lam, float cw, float cn, double* loss, float* grad, int64_t n) {
   Net* N = need_net(ctx);
   if (!N || !x || !t || !mk || !loss || !grad) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
   if (B < 1 || H < 8 || W < 8 || H % 8 || W % 8) return ctx->fail(LEPIC_ERR_ARG, "B >= 1, H and W multiples of 8");
@@ -995,7 +1079,7 @@ int lepic_net_loss_grad(lepic_ctx* ctx, const float* x, const float* t, const fl
   for (int64_t q = 0; q < (int64_t)B * H * W; ++q) nm += mk[q];
   if (!(nm > 0.0)) return ctx->fail(LEPIC_ERR_ARG, "empty mask");
   if (int rc = forward(ctx, *N)) return rc;
-  if (int rc = loss_launch(ctx, *N, nm)) return rc;
+  if (int rc = loss_launch(ctx, *N, nm, lam, cw, cn)) return rc;
   if (int rc = backward(ctx, *N)) return rc;
   *loss = read_d(ctx, N->d_loss) / nm;
   std::vector<float> gpu(N->nparam);

@@ -174,6 +174,26 @@ module lepic_capi
        integer(c_int32_t), intent(inout) :: iseed
        type(lepic_mcc_info), intent(out) :: info
      end function lepic_collisions
+     ! learned warm start (no counterpart in the reference)
+     integer(c_int) function lepic_net_create(ctx, capacity, seed) bind(C, name='lepic_net_create')
+       import :: c_int, c_ptr, c_int32_t, c_int64_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: capacity
+       integer(c_int64_t), value :: seed
+     end function lepic_net_create
+     integer(c_int) function lepic_net_recording(ctx, on) bind(C, name='lepic_net_recording')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: on
+     end function lepic_net_recording
+     integer(c_int) function lepic_net_train(ctx, iters, batch, lr, loss_first, loss_last) &
+          bind(C, name='lepic_net_train')
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: iters, batch
+       real(c_float), value :: lr
+       real(c_float), intent(out) :: loss_first, loss_last
+     end function lepic_net_train
   end interface
 
 contains

@@ -156,8 +156,19 @@ def forward(x, flat, rounding=True):
     return y, cache
 
 
-def loss_and_grad(x, target, mask, flat, rounding=True):
-    """MSE over mask nodes: L = sum(mask*(y-t)^2)/sum(mask).  Returns (L, dflat, y)."""
+def stencil(u, cw, cn):
+    """cw*(u[x-1]+u[x+1]) + cn*(u[y-1]+u[y+1]) - 2*(cw+cn)*u, zero outside (B,H,W)."""
+    p = np.zeros((u.shape[0], u.shape[1] + 2, u.shape[2] + 2), _DT)
+    p[:, 1:-1, 1:-1] = u
+    return (cw * (p[:, 1:-1, :-2] + p[:, 1:-1, 2:]) + cn * (p[:, :-2, 1:-1] + p[:, 2:, 1:-1])
+            - 2.0 * (cw + cn) * u).astype(_DT)
+
+
+def loss_and_grad(x, target, mask, flat, rounding=True, lam=0.0, cw=0.0, cn=0.0):
+    """Loss over mask nodes: L = sum(mask*(y-t)^2)/sum(mask) plus, when lam > 0,
+    lam * sum(r^2)/sum(mask) with the Poisson residual r = mask*(A(mask*y) -
+    x[...,0]) of the prediction (A: the 5-point stencil with coefficients cw,
+    cn in the network's units).  Returns (L, dflat, y)."""
     R = bf16 if rounding else (lambda a: np.asarray(a, _DT))
     y, c = forward(x, flat, rounding)
     P = split_params(flat)
@@ -166,6 +177,10 @@ def loss_and_grad(x, target, mask, flat, rounding=True):
     diff = (y - target) * mask
     L = float((diff.astype(np.float64) ** 2).sum() / nm)
     dy = (2.0 * diff / nm).astype(_DT)
+    if lam > 0.0:
+        r = (stencil(y * mask, cw, cn) - x[..., 0]) * mask
+        L += lam * float((r.astype(np.float64) ** 2).sum() / nm)
+        dy = (dy + mask * stencil(r, cw, cn) * _DT(2.0 * lam / nm)).astype(_DT)
     g = {"out": np.einsum("bhwc,bhw->c", c["d0"], dy, dtype=_DT)}
     dd0 = R(dy[..., None] * P["out"].astype(_DT)[None, None, None, :]) * (c["d0"] > 0)
 
@@ -224,7 +239,8 @@ def _fd_check(H, W, B, n_probe, seed):
     t = rng.standard_normal((B, H, W))
     m = (rng.uniform(size=(B, H, W)) > 0.2).astype(np.float64)
     flat = init_params(seed).astype(np.float64)
-    _, g, _ = loss_and_grad(x, t, m, flat, rounding=False)
+    kw = dict(rounding=False, lam=0.7, cw=0.3, cn=0.5)   # both loss terms
+    _, g, _ = loss_and_grad(x, t, m, flat, **kw)
     idx = rng.choice(flat.size, n_probe, replace=False)
     idx = np.concatenate([idx, [flat.size - 1, 0]])
     num, ana = [], []
@@ -233,8 +249,8 @@ def _fd_check(H, W, B, n_probe, seed):
         fp, fm = flat.copy(), flat.copy()
         fp[i] += h
         fm[i] -= h
-        Lp, _, _ = loss_and_grad(x, t, m, fp, rounding=False)
-        Lm, _, _ = loss_and_grad(x, t, m, fm, rounding=False)
+        Lp, _, _ = loss_and_grad(x, t, m, fp, **kw)
+        Lm, _, _ = loss_and_grad(x, t, m, fm, **kw)
         num.append((Lp - Lm) / (fp[i] - fm[i]))
         ana.append(g[i])
     num, ana = np.array(num), np.array(ana)

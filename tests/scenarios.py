@@ -59,3 +59,47 @@ def particles(dom, N, vt, seed=2, edge_frac=0.2):
     v[k, 0] *= 20.0  # fast toward/away from the walls
     z = rng.uniform(0, 1.0, N)
     return np.column_stack([x, y, z, v])
+
+
+def h2_source_domain(n1=64, n2=32, h=5e-5, dt=5e-12, Bpeak=0.01):
+    """Negative-ion-source-like geometry (the 1024x512 four-species workload at
+    reduced size): grounded box (label 1), a grid electrode of three bars a
+    few cells before the extraction wall (label 2), the extraction wall
+    itself (label 3), and the Gaussian Bz filter field.  Electrode edges sit
+    on odd node indices so MG injection keeps them on coarse levels."""
+    bcnd = O.box_bcnd(n1, n2, 1)
+    bcnd[:, n1 + 1:] = 3                   # extraction wall and its ghost column
+    c0 = n1 - 11 if (n1 - 11) % 2 == 1 else n1 - 10
+    for r0 in range(5, n2 - 3, 8):
+        bcnd[r0:r0 + 5, c0:c0 + 3] = 2
+    dom = O.Domain(n1=n1, n2=n2, h1=h, h2=h, dt=dt, bcnd=bcnd, ngrid=3, flag_B_pos=1)
+    L = n1 * h
+    x = (np.arange(n1 + 3) - 1) * h
+    Bi = np.zeros((n2 + 3, n1 + 3, 4))
+    Bi[..., 2] = Bpeak * np.exp(-(x - L / 2) ** 2 / (2 * (L / 8) ** 2))[None, :]
+    Bi[..., 3] = np.abs(Bi[..., 2])
+    dom.Bi, dom.nB, dom.hB = Bi, (n1, n2), (h, h)
+    return dom
+
+
+def exp_profile_particles(dom, N, vt, n_hi=1e17, n_lo=1e15, seed=3):
+    """Particles with density falling exponentially from n_hi at x=0 to n_lo
+    at the extraction wall, uniform in y, none inside electrode cells."""
+    rng = np.random.default_rng(seed)
+    L1, L2 = dom.n1 * dom.h1, dom.n2 * dom.h2
+    lam = L1 / np.log(n_hi / n_lo)
+    out = []
+    while sum(len(a) for a in out) < N:
+        u = rng.uniform(size=2 * N)
+        x = -lam * np.log(1.0 - u * (1.0 - np.exp(-L1 / lam)))
+        y = rng.uniform(0, L2, 2 * N)
+        ix = (x / dom.h1).astype(int) + 1
+        iy = (y / dom.h2).astype(int) + 1
+        b = dom.bcnd
+        walls = (b[iy, ix] >= 1) & (b[iy, ix + 1] >= 1) & (b[iy + 1, ix] >= 1) & (b[iy + 1, ix + 1] >= 1)
+        keep = ~walls
+        out.append(np.column_stack([x[keep], y[keep]]))
+    xy = np.concatenate(out)[:N]
+    v = rng.normal(0, vt, (N, 3))
+    z = rng.uniform(0, 1.0, N)
+    return np.column_stack([xy, z, v])

@@ -54,17 +54,18 @@ def test_forward_matches_oracle(B, H, W):
     assert np.abs(y - ref).mean() <= 2e-3 * scale
 
 
-@pytest.mark.parametrize("B,H,W", [(1, 8, 8), (2, 16, 24)])
-def test_gradient_matches_oracle(B, H, W):
+@pytest.mark.parametrize("B,H,W,lam", [(1, 8, 8, 0.0), (2, 16, 24, 0.0), (2, 16, 24, 0.7)])
+def test_gradient_matches_oracle(B, H, W, lam):
     ctx, _ = box_ctx(16)
     n = U.n_params()
+    kw = dict(lam=lam, cw=0.3, cn=0.5)
     with ctx:
         ctx.net_create(capacity=0, seed=1)
         flat = U.init_params(3)
         ctx.net_set_params(flat)
         x, t, m = rand_batch(B, H, W, 5)
-        loss, g = ctx.net_loss_grad(x, t, m, n)
-    ref_loss, ref_g, _ = U.loss_and_grad(x, t, m, flat)
+        loss, g = ctx.net_loss_grad(x, t, m, n, **kw)
+    ref_loss, ref_g, _ = U.loss_and_grad(x, t, m, flat, **kw)
     assert abs(loss - ref_loss) <= 1e-2 * ref_loss
     rel = np.linalg.norm(g - ref_g) / np.linalg.norm(ref_g)
     cos = float(g @ ref_g / (np.linalg.norm(g) * np.linalg.norm(ref_g)))
@@ -107,11 +108,14 @@ def test_training_and_warm_start_gain():
         assert ctx.net_samples() == 40
         ctx.net_recording(False)
         l0, l1 = ctx.net_train(300, batch=4, lr=2e-3)
+        print(f"training loss {l0:.4g} -> {l1:.4g}")
         assert l1 < 0.3 * l0, (l0, l1)
         gains, used = [], 0
         for it in range(41, 51):
             ctx.set_grid(L.GRID_RHS, moving_rhs(n, it, phase))
             info = ctx.iterative_solver(eps=1e-10, start=L.START_NET)
+            print(f"step {it}: start {info['start_used']} res0 {info['res0']:.3g} other {info['res0_alt']:.3g} "
+                  f"vcycles {info['vcycles']}")
             assert info["res"] <= 1e-10
             if info["start_used"] == L.START_NET:
                 used += 1
