@@ -38,6 +38,9 @@ import jd_lepic_amd as L  # noqa: E402
 from jd_lepic_amd import reactions as RX, tables  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per mover launch measured with rocprofv3 PMC counters (FETCH_SIZE
+# and WRITE_SIZE, scaled by a plain copy's reading); None until measured
+TRAFFIC = None
 
 
 class Hip:
@@ -111,14 +114,15 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
     """One mode of the step loop.  With learned=True the first learn_steps
     (untimed) solves record (drhs, dphi) samples, the network is trained on
     them (its time reported apart), and every later solve starts from the
-    network's guess when that lowers the initial residual."""
+    network's guess when that lowers the initial residual (the library skips
+    the network for a growing back-off after a guess that gains too little)."""
     ctx = build_ctx(W, precision)
     ctx.kernel_timing(True)      # mover kernels bracketed by events inside the library
     stream = C.c_void_p()
     ctx.lib.lepic_stream(ctx._ctx, C.byref(stream))
     ev = [hip.event() for _ in range(12)]
     acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[], []],
-               net_used=0, res0_gain=[], train_s=0.0, loss=None)
+               net_used=0, net_probed=0, res0_gain=[], train_s=0.0, loss=None)
     pre = learn_steps if learned else 0
     if learned:
         ctx.net_create(capacity=64, seed=seed)
@@ -160,9 +164,13 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
             acc["mcc"] += hip.elapsed(ev[2], ev[3])
             acc["push"] += hip.elapsed(ev[3], ev[8])
             acc["vcycles"] += info["vcycles"]
-            if learned and info["start_used"] == L.START_NET:
-                acc["net_used"] += 1
-                acc["res0_gain"].append(info["res0_alt"] / info["res0"])
+            if learned and info["res0_alt"] >= 0.0:
+                # the network was evaluated: gain = residual of the previous-phi
+                # start over that of the network's guess
+                acc["net_probed"] += 1
+                used = info["start_used"] == L.START_NET
+                acc["net_used"] += used
+                acc["res0_gain"].append(info["res0_alt"] / info["res0"] if used else info["res0"] / info["res0_alt"])
             for p in (0, 1):
                 acc["push_kernel_ms"][p].append(ctx.kernel_ms(p + 1))
     hip.sync()
@@ -256,7 +264,7 @@ def main():
                    ion_mover_ms=float(np.mean(a["push_kernel_ms"][1])))
         if a["loss"] is not None:
             out.update(train_s=a["train_s"], train_loss_first=a["loss"][0], train_loss_last=a["loss"][1],
-                       net_start_used=a["net_used"] / K,
+                       net_probed=a["net_probed"] / K, net_start_used=a["net_used"] / K,
                        res0_gain_median=float(np.median(a["res0_gain"])) if a["res0_gain"] else None)
         return out
 
@@ -277,14 +285,16 @@ def main():
         res["fp64"] = (r, summary(*r))
     head = "mixed_learned" if "mixed_learned" in res else "mixed"
     (acc_h, _, cnt_h), s_h = res[head]
-    # roofline of the dominant kernel (the electron mover, mixed precision):
-    # algorithmic bytes = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities,
-    # read and written) per particle and launch; the launch duration comes from
-    # HIP events around that kernel alone, on the library's stream
+    # roofline of the dominant kernel, the mixed-precision mover with fused
+    # deposition (one instantiation serves electrons and ions): algorithmic
+    # bytes = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities, read
+    # and written) per particle and launch; launch durations come from HIP
+    # events around that kernel alone, on the library's stream
     bpp = 48.0
     ke = np.array(acc_h["push_kernel_ms"][0])
-    alg = cnt_h[0] * bpp
-    achieved = alg / (ke.mean() * 1e-3) / 1e9
+    ki = np.array(acc_h["push_kernel_ms"][1])
+    alg_e, alg_i = cnt_h[0] * bpp, cnt_h[1] * bpp
+    achieved = (alg_e + alg_i) / ((ke.mean() + ki.mean()) * 1e-3) / 1e9
     cpu = None if args.skip_cpu else cpu_baseline(W, args.cpu_budget)
     out = {
         "metric": "ms/step & particles/s (push+deposit+MCC), mixed+learned vs FP64 GPU",
@@ -301,9 +311,11 @@ def main():
                                   else "mixed precision"),
                    "global_batch": N, "seq_len": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_mover_mixed (electrons)", "alg_bytes_per_launch": alg,
-                     "avg_launch_ms": float(ke.mean())},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC,
+                     "kernel": "k_mover_mixed<256> (electron and ion launches)",
+                     "alg_bytes_per_launch": (alg_e + alg_i) / 2,
+                     "avg_launch_ms": float((ke.mean() + ki.mean()) / 2),
+                     "avg_launch_ms_electrons": float(ke.mean()), "avg_launch_ms_ions": float(ki.mean())},
         "cpu_baseline": cpu,
     }
     for k, (_, sm) in res.items():

@@ -223,10 +223,15 @@ typedef struct {
  * starts from the network's guess only when it lowers the initial residual
  * (else from the previous phi) and always iterates to the same tolerance.
  * Samples are the (drhs, dphi) pairs of consecutive converged solves,
- * recorded while recording is on. */
+ * recorded while recording is on.  A guess that does not lower the initial
+ * residual by at least min_gain (default 2) makes the next 8, 16, ...
+ * max_backoff (default 256) START_NET solves start from the previous phi
+ * without evaluating the network; a guess that does resets the back-off.
+ * max_backoff = 0 evaluates the network on every START_NET solve. */
 int lepic_net_create(lepic_ctx *ctx, int32_t capacity, uint64_t seed);
 int lepic_net_recording(lepic_ctx *ctx, int32_t on);
 int lepic_net_samples(lepic_ctx *ctx, int32_t *n);
+int lepic_net_policy(lepic_ctx *ctx, float min_gain, int32_t max_backoff);
 int lepic_net_train(lepic_ctx *ctx, int32_t iters, int32_t batch, float lr, float *loss_first,
                     float *loss_last);
 /* parameters in the flat layout [layer][cout][tap][cin] + 16 output weights */

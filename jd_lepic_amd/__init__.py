@@ -216,6 +216,9 @@ class Context:
         self._check(self.lib.lepic_net_samples(self._ctx, C.byref(n)))
         return n.value
 
+    def net_policy(self, min_gain=2.0, max_backoff=256):
+        self._check(self.lib.lepic_net_policy(self._ctx, min_gain, max_backoff))
+
     def net_train(self, iters, batch=4, lr=1e-3):
         a, b = C.c_float(), C.c_float()
         self._check(self.lib.lepic_net_train(self._ctx, iters, batch, lr, C.byref(a), C.byref(b)))

@@ -80,6 +80,7 @@ SIGNATURES = {
     "lepic_net_create": (C.c_int, [_ctxp, C.c_int32, C.c_uint64]),
     "lepic_net_recording": (C.c_int, [_ctxp, C.c_int32]),
     "lepic_net_samples": (C.c_int, [_ctxp, _ip]),
+    "lepic_net_policy": (C.c_int, [_ctxp, C.c_float, C.c_int32]),
     "lepic_net_train": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float),
                                   C.POINTER(C.c_float)]),
     "lepic_net_set_params": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int64]),

@@ -260,7 +260,9 @@ namespace {
 
 int alloc_species(lepic_ctx* ctx, int p) {
   Species& s = ctx->sp[p];
-  const int64_t cap = std::max<int64_t>(s.cap, 1);
+  // arrays hold whole segments, so a kernel may load a segment's slots as
+  // 16-byte vectors past the species' count
+  const int64_t cap = (std::max<int64_t>(s.cap, 1) + kChunk - 1) / kChunk * kChunk;
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
   for (int b = 0; b < 2; ++b) {
     size_t bytes = mixed ? (size_t)cap * (4 + 6 * 4) : (size_t)cap * 48;
@@ -275,11 +277,11 @@ int alloc_species(lepic_ctx* ctx, int p) {
     }
   }
   LEPIC_HIP(hipMalloc(&s.flag_dead, cap));
-  LEPIC_HIP(hipMemset(s.flag_dead, 0, cap));
+  LEPIC_HIP(hipMemsetAsync(s.flag_dead, 0, cap, ctx->stream));
   s.nseg_cap = (cap + kChunk - 1) / kChunk + 1;
   for (int b = 0; b < 2; ++b) {
     LEPIC_HIP(hipMalloc(&s.seg[b], s.nseg_cap * 4));
-    LEPIC_HIP(hipMemset(s.seg[b], 0, s.nseg_cap * 4));
+    LEPIC_HIP(hipMemsetAsync(s.seg[b], 0, s.nseg_cap * 4, ctx->stream));
   }
   if (s.nseg_cap > ctx->segoff_cap) {
     if (ctx->d_segoff) LEPIC_HIP(hipFree(ctx->d_segoff));
@@ -288,11 +290,11 @@ int alloc_species(lepic_ctx* ctx, int p) {
   }
   s.packed = true;
   LEPIC_HIP(hipMalloc(&s.np, ctx->nodes * 8));
-  LEPIC_HIP(hipMemset(s.np, 0, ctx->nodes * 8));
+  LEPIC_HIP(hipMemsetAsync(s.np, 0, ctx->nodes * 8, ctx->stream));
   LEPIC_HIP(hipMalloc(&s.np_red, ctx->nodes * 8));
-  LEPIC_HIP(hipMemset(s.np_red, 0, ctx->nodes * 8));
+  LEPIC_HIP(hipMemsetAsync(s.np_red, 0, ctx->nodes * 8, ctx->stream));
   LEPIC_HIP(hipMalloc(&s.sink, ctx->nodes * 8));
-  LEPIC_HIP(hipMemset(s.sink, 0, ctx->nodes * 8));
+  LEPIC_HIP(hipMemsetAsync(s.sink, 0, ctx->nodes * 8, ctx->stream));
   return 0;
 }
 
@@ -423,7 +425,8 @@ int lepic_sync(lepic_ctx* ctx) {
   if (!ctx) return LEPIC_ERR_ARG;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   int32_t e = 0;
-  LEPIC_HIP(hipMemcpy(&e, ctx->d_error, 4, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(&e, ctx->d_error, 4, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   if (e) return ctx->fail(LEPIC_ERR_HIP, "device-side wait bound exceeded (look-back)");
   return 0;
 }
@@ -486,8 +489,10 @@ int lepic_set_boundary(lepic_ctx* ctx, const int32_t* bcnd, const double* phi) {
   ctx->bc_host.assign(bcnd, bcnd + ctx->nodes);
   std::vector<int8_t> b8(ctx->nodes);
   for (int64_t i = 0; i < ctx->nodes; ++i) b8[i] = (int8_t)bcnd[i];
-  LEPIC_HIP(hipMemcpy(ctx->bc, b8.data(), ctx->nodes, hipMemcpyHostToDevice));
-  LEPIC_HIP(hipMemcpy(ctx->phi, phi, ctx->nodes * 8, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(ctx->bc, b8.data(), ctx->nodes, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  LEPIC_HIP(hipMemcpyAsync(ctx->phi, phi, ctx->nodes * 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   // coarse-level masks by injection (mg.f90:343-348); corners copied from
   // the finer corners (the reference never writes them and never reads them)
   lepic::Level* lv = ctx->lv;
@@ -510,7 +515,8 @@ int lepic_set_boundary(lepic_ctx* ctx, const int32_t* bcnd, const double* phi) {
       }
     std::vector<int8_t> c8(c.size());
     for (size_t i = 0; i < c.size(); ++i) c8[i] = (int8_t)c[i];
-    LEPIC_HIP(hipMemcpy(lv[l].bc, c8.data(), c8.size(), hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(lv[l].bc, c8.data(), c8.size(), hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
     fine.swap(c);
   }
   ctx->boundary_set = true;
@@ -527,12 +533,14 @@ int lepic_set_bfield(lepic_ctx* ctx, int32_t nB1, int32_t nB2, double hB1, doubl
   if (ctx->Bf) LEPIC_HIP(hipFree(ctx->Bf));
   ctx->Bf = nullptr;
   LEPIC_HIP(hipMalloc(&ctx->Bi, bn * 32));
-  LEPIC_HIP(hipMemcpy(ctx->Bi, Bi, bn * 32, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(ctx->Bi, Bi, bn * 32, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   if (ctx->g.precision == LEPIC_MIXED) {
     std::vector<float> f(bn * 4);
     for (int64_t i = 0; i < bn * 4; ++i) f[i] = (float)Bi[i];
     LEPIC_HIP(hipMalloc(&ctx->Bf, bn * 16));
-    LEPIC_HIP(hipMemcpy(ctx->Bf, f.data(), bn * 16, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(ctx->Bf, f.data(), bn * 16, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   ctx->nB1 = nB1; ctx->nB2 = nB2; ctx->hB1 = hB1; ctx->hB2 = hB2;
   ctx->flag_B = 1;
@@ -569,7 +577,8 @@ int lepic_upload_particles(lepic_ctx* ctx, int32_t ptype, const double* vxp, int
   }
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipFree(stage));
-  LEPIC_HIP(hipMemcpy(ctx->d_counts + (ptype - 1), &np, 8, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(ctx->d_counts + (ptype - 1), &np, 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   s.ub = np;
   return set_packed_counts(ctx, ptype - 1);
 }
@@ -623,7 +632,8 @@ int lepic_set_flag_dead(lepic_ctx* ctx, int32_t ptype, const int8_t* flags, int6
   if (np > ctx->sp[ptype - 1].cap) return ctx->fail(LEPIC_ERR_CAPACITY, "np exceeds capacity");
   if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(ctx->sp[ptype - 1].flag_dead, flags, np, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(ctx->sp[ptype - 1].flag_dead, flags, np, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -632,7 +642,8 @@ int lepic_get_flag_dead(lepic_ctx* ctx, int32_t ptype, int8_t* flags, int64_t np
   if (np > ctx->sp[ptype - 1].cap) return ctx->fail(LEPIC_ERR_CAPACITY, "np exceeds capacity");
   if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(flags, ctx->sp[ptype - 1].flag_dead, np, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(flags, ctx->sp[ptype - 1].flag_dead, np, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -658,7 +669,8 @@ int lepic_get_grid(lepic_ctx* ctx, int32_t which, int32_t ptype, double* out) {
   double* p; int64_t n;
   if (grid_ptr(ctx, which, ptype, &p, &n)) return ctx->fail(LEPIC_ERR_ARG, "bad grid id/ptype");
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(out, p, n * 8, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(out, p, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -667,12 +679,15 @@ int lepic_set_grid(lepic_ctx* ctx, int32_t which, int32_t ptype, const double* i
   double* p; int64_t n;
   if (grid_ptr(ctx, which, ptype, &p, &n)) return ctx->fail(LEPIC_ERR_ARG, "bad grid id/ptype");
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(p, in, n * 8, hipMemcpyHostToDevice));
-  if (which == LEPIC_GRID_PHI) LEPIC_HIP(hipMemcpy(ctx->phi_given, in, n * 8, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(p, in, n * 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  if (which == LEPIC_GRID_PHI) LEPIC_HIP(hipMemcpyAsync(ctx->phi_given, in, n * 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   if (which == LEPIC_GRID_E && ctx->Ef) {
     std::vector<float> f(n);
     for (int64_t i = 0; i < n; ++i) f[i] = (float)in[i];
-    LEPIC_HIP(hipMemcpy(ctx->Ef, f.data(), n * 4, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(ctx->Ef, f.data(), n * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
 }
@@ -693,7 +708,8 @@ int lepic_get_mover_tally(lepic_ctx* ctx, int32_t ptype, double* P_loss, double*
   if (!ctx || !valid_ptype(ctx, ptype)) return LEPIC_ERR_ARG;
   Tally t;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(&t, ctx->d_tally + (ptype - 1), sizeof(Tally), hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(&t, ctx->d_tally + (ptype - 1), sizeof(Tally), hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   if (P_loss) std::memcpy(P_loss, t.P_loss, 32);
   if (p_mac)
     for (int k = 0; k < 2; ++k)

@@ -216,6 +216,8 @@ int load_uniform_launch(lepic_ctx* ctx, int ptype, int ppc, double vt, double xl
 void net_destroy(lepic_ctx* ctx);
 int net_after_solve(lepic_ctx* ctx);
 int net_guess(lepic_ctx* ctx, double* out);   // 1: no guess available
+bool net_probe_due(lepic_ctx* ctx);
+void net_probe_result(lepic_ctx* ctx, double gain);
 
 // ---- device helpers ----
 __device__ __forceinline__ int64_t node_index(int ix, int iy, int n1) {

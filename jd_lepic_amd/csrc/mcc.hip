@@ -870,12 +870,14 @@ int lepic_set_reactions(lepic_ctx* ctx, const lepic_reactions* r) {
   if (R.d_sig) LEPIC_HIP(hipFree(R.d_sig));
   if (R.d_cnt_col) LEPIC_HIP(hipFree(R.d_cnt_col));
   LEPIC_HIP(hipMalloc(&R.d_sig_Er, (size_t)r->npt * 8));
-  LEPIC_HIP(hipMemcpy(R.d_sig_Er, r->sig_Er, (size_t)r->npt * 8, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(R.d_sig_Er, r->sig_Er, (size_t)r->npt * 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipMalloc(&R.d_sig, (size_t)r->npt * std::max(r->ncol, 1) * 8));
-  LEPIC_HIP(hipMemcpy(R.d_sig, r->sig, (size_t)r->npt * r->ncol * 8, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(R.d_sig, r->sig, (size_t)r->npt * r->ncol * 8, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   const size_t ncc = 3 * (size_t)std::max(r->ncol, 1) * std::max(r->n_rg, 1);
   LEPIC_HIP(hipMalloc(&R.d_cnt_col, ncc * 8));
-  LEPIC_HIP(hipMemset(R.d_cnt_col, 0, ncc * 8));
+  LEPIC_HIP(hipMemsetAsync(R.d_cnt_col, 0, ncc * 8, ctx->stream));
   R.set = true;
   return 0;
 }
@@ -890,15 +892,16 @@ int lepic_get_cnt_col(lepic_ctx* ctx, double* cnt_col) {
   if (!ctx || !cnt_col) return LEPIC_ERR_ARG;
   if (!ctx->rx.set) return ctx->fail(LEPIC_ERR_STATE, "no reactions");
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(cnt_col, ctx->rx.d_cnt_col, 3 * (size_t)ctx->rx.ncol * ctx->rx.n_rg * 8,
-                      hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(cnt_col, ctx->rx.d_cnt_col, 3 * (size_t)ctx->rx.ncol * ctx->rx.n_rg * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
 int lepic_get_mcc_ploss(lepic_ctx* ctx, double* P_loss3) {
   if (!ctx || !P_loss3) return LEPIC_ERR_ARG;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(P_loss3, ctx->d_mcc_f64, (kMaxSpecies + 1) * 8, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(P_loss3, ctx->d_mcc_f64, (kMaxSpecies + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 

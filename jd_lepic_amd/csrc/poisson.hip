@@ -942,17 +942,17 @@ int poisson_setup(lepic_ctx* ctx) {
       L.bc = ctx->bc;
     } else {
       LEPIC_HIP(hipMalloc(&L.u, nn * 8));
-      LEPIC_HIP(hipMemset(L.u, 0, nn * 8));
+      LEPIC_HIP(hipMemsetAsync(L.u, 0, nn * 8, ctx->stream));
       LEPIC_HIP(hipMalloc(&L.b, nr * 8));
-      LEPIC_HIP(hipMemset(L.b, 0, nr * 8));
+      LEPIC_HIP(hipMemsetAsync(L.b, 0, nr * 8, ctx->stream));
       LEPIC_HIP(hipMalloc(&L.bc, nn));
-      LEPIC_HIP(hipMemset(L.bc, 1, nn));
+      LEPIC_HIP(hipMemsetAsync(L.bc, 1, nn, ctx->stream));
     }
     if (mixed) {
       LEPIC_HIP(hipMalloc(&L.uf, nn * 4));
-      LEPIC_HIP(hipMemset(L.uf, 0, nn * 4));
+      LEPIC_HIP(hipMemsetAsync(L.uf, 0, nn * 4, ctx->stream));
       LEPIC_HIP(hipMalloc(&L.bf, nr * 4));
-      LEPIC_HIP(hipMemset(L.bf, 0, nr * 4));
+      LEPIC_HIP(hipMemsetAsync(L.bf, 0, nr * 4, ctx->stream));
     }
   }
   // levels with more nodes than the tail threshold run multi-block; the
@@ -973,30 +973,32 @@ int poisson_setup(lepic_ctx* ctx) {
     L.rot[0] = mixed ? (void*)L.uf : (void*)L.u;
     for (int r = 1; r < kRot; ++r) {
       LEPIC_HIP(hipMalloc(&L.rot[r], nn * esz));
-      LEPIC_HIP(hipMemset(L.rot[r], 0, nn * esz));
+      LEPIC_HIP(hipMemsetAsync(L.rot[r], 0, nn * esz, ctx->stream));
     }
     L.nbx = (L.n1 + 1 + kTX - 1) / kTX;
     L.nby = (L.n2 + 1 + ty - 1) / ty;
     const size_t np = (size_t)kRing * L.nbx * L.nby;
     LEPIC_HIP(hipMalloc(&L.partials, np * 8));
-    LEPIC_HIP(hipMemset(L.partials, 0, np * 8));
+    LEPIC_HIP(hipMemsetAsync(L.partials, 0, np * 8, ctx->stream));
   }
   if (mixed && ctx->lv[1].mb) {
     LEPIC_HIP(hipMalloc(&ctx->d_erot, kRot * sizeof(float*)));
-    LEPIC_HIP(hipMemcpy(ctx->d_erot, ctx->lv[1].rot, kRot * sizeof(float*), hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(ctx->d_erot, ctx->lv[1].rot, kRot * sizeof(float*), hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   } else if (mixed) {
     std::vector<float*> e(kRot, ctx->lv[1].uf);
     LEPIC_HIP(hipMalloc(&ctx->d_erot, kRot * sizeof(float*)));
-    LEPIC_HIP(hipMemcpy(ctx->d_erot, e.data(), kRot * sizeof(float*), hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(ctx->d_erot, e.data(), kRot * sizeof(float*), hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   LEPIC_HIP(hipMalloc(&ctx->d_ctrl, (kMaxLevels + 1) * sizeof(LevelCtrl)));
-  LEPIC_HIP(hipMemset(ctx->d_ctrl, 0, (kMaxLevels + 1) * sizeof(LevelCtrl)));
+  LEPIC_HIP(hipMemsetAsync(ctx->d_ctrl, 0, (kMaxLevels + 1) * sizeof(LevelCtrl), ctx->stream));
   ctx->mg_epoch = 0;
   LEPIC_HIP(hipMalloc(&ctx->d_partials, kResidBlocks * 8));
-  LEPIC_HIP(hipMemset(ctx->d_partials, 0, kResidBlocks * 8));
+  LEPIC_HIP(hipMemsetAsync(ctx->d_partials, 0, kResidBlocks * 8, ctx->stream));
   ctx->partials_cap = kResidBlocks;
   LEPIC_HIP(hipMalloc(&ctx->d_scalar, 64 * 8 + (kVisitLog + 1) * sizeof(int)));
-  LEPIC_HIP(hipMemset(ctx->d_scalar, 0, 64 * 8 + (kVisitLog + 1) * sizeof(int)));
+  LEPIC_HIP(hipMemsetAsync(ctx->d_scalar, 0, 64 * 8 + (kVisitLog + 1) * sizeof(int), ctx->stream));
   return 0;
 }
 
@@ -1028,7 +1030,7 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
     // the network's guess replaces the previous phi only if it lowers the
     // initial residual; otherwise the solve starts from the previous phi
     info->start_used = LEPIC_START_PREV;
-    const int g = net_guess(ctx, ctx->phi_given);
+    const int g = net_probe_due(ctx) ? net_guess(ctx, ctx->phi_given) : 1;
     if (g < 0) return g;
     if (g == 0) {
       double r_prev = 0.0, r_net = 0.0;
@@ -1036,6 +1038,7 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
       if (int rc = read_back(ctx, visit, vres, &r_prev)) return rc;
       if (int rc = launch_resid64(ctx, false, ctx->phi_given)) return rc;
       if (int rc = read_back(ctx, visit, vres, &r_net)) return rc;
+      net_probe_result(ctx, r_net > 0.0 ? r_prev / r_net : HUGE_VAL);
       if (r_net < r_prev) {
         hipLaunchKernelGGL(k_copy_interior, g1, dim3(256), 0, ctx->stream, ctx->phi,
                            (const double*)ctx->phi_given, (const int8_t*)ctx->bc, n);

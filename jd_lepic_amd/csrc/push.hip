@@ -175,6 +175,124 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cn
   }
 }
 
+// Offsets for thread-major items (thread t holds slots kItems*t + k): the
+// same order-preserving compaction as compact_offsets.  Returns the number
+// of survivors of the segment.
+template <int BT>
+__device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t bid, int cnt_in,
+                                                  const bool surv[kItems], uint32_t off[kItems]) {
+  __shared__ uint32_t s_wave[BT / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t mine = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) mine += surv[k] ? 1u : 0u;
+  // inclusive scan of the per-thread counts over the wave
+  uint32_t incl = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) s_wave[w] = incl;
+  __syncthreads();
+  uint32_t before = incl - mine, total = 0;
+#pragma unroll
+  for (int v = 0; v < BT / 64; ++v) {
+    const uint32_t c = s_wave[v];
+    if (v < w) before += c;
+    total += c;
+  }
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    off[k] = before;
+    before += surv[k] ? 1u : 0u;
+  }
+  if (threadIdx.x == 0) {
+    a.seg_out[bid] = (int32_t)total;
+    const int lost = cnt_in - (int)total;
+    if (lost > 0) atomicAdd(a.count, (unsigned long long)(-(long long)lost));
+  }
+  return total;
+}
+
+// DPP lane move of a float (see dpp_move for the controls)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_f32(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+}
+
+// Segmented inclusive sum of w[0..3] over runs of equal key in adjacent
+// lanes, in a fixed order: afterwards the last lane of each run holds the
+// run's sums.  s = first lane of this lane's run.  Every lane must be active.
+__device__ __forceinline__ void seg_sum4(float w[4], int s) {
+  const int lane = threadIdx.x & 63;
+  const bool a1 = lane - 1 >= s, a2 = lane - 2 >= s, a4 = lane - 4 >= s, a8 = lane - 8 >= s;
+  const bool b15 = (lane & 16) && (lane & ~15) - 1 >= s;   // rows 1 and 3
+  const bool b31 = lane >= 32 && s <= 31;                  // rows 2 and 3
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    float v = w[c], o;
+    o = dpp_f32<0x111, 0xf>(v); if (a1) v += o;
+    o = dpp_f32<0x112, 0xf>(v); if (a2) v += o;
+    o = dpp_f32<0x114, 0xf>(v); if (a4) v += o;
+    o = dpp_f32<0x118, 0xf>(v); if (a8) v += o;
+    o = dpp_f32<0x142, 0xa>(v); if (b15) v += o;
+    o = dpp_f32<0x143, 0xc>(v); if (b31) v += o;
+    w[c] = v;
+  }
+}
+
+// CIC deposition of thread-major items (consecutive particles per thread):
+// equal cells within a thread are summed first, then runs of equal cells
+// across lanes per item slot; slots that hold nothing in the whole wave are
+// skipped.  Run sums go to the LDS window (or to global atomics outside it).
+__device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win, int wx0, int wy0,
+                                               const int C[kItems], const float FX[kItems],
+                                               const float FY[kItems], const bool ok[kItems]) {
+  const int lane = threadIdx.x & 63;
+  int key[kItems];
+  float w[kItems][4];
+  int prev = -1;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    key[k] = -1;
+    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0.f;
+    if (!ok[k]) continue;
+    int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
+    clamp_cell(ix, iy, a.n1, a.n2);
+    const int kk = (int)node_index(ix, iy, a.n1);
+    const float px = 1.0f - FX[k], py = 1.0f - FY[k];
+    const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
+    if (kk == prev) {
+      // same cell as this thread's previous particle: add into its slot
+#pragma unroll
+      for (int j = k - 1; j >= 0; --j)
+        if (key[j] == kk) { w[j][0] += q0; w[j][1] += q1; w[j][2] += q2; w[j][3] += q3; break; }
+    } else {
+      key[k] = kk;
+      w[k][0] = q0; w[k][1] = q1; w[k][2] = q2; w[k][3] = q3;
+      prev = kk;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t act = __ballot(key[k] >= 0);
+    if (act == 0) continue;                       // wave-uniform
+    // inactive lanes take a key no active lane has, distinct from neighbours
+    const int kk = key[k] >= 0 ? key[k] : -2 - lane;
+    const int kp = __shfl_up(kk, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || kp != kk);
+    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+    float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
+    seg_sum4(v, s);
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (tail && key[k] >= 0) {
+      const int ix = key[k] % (a.n1 + 3), iy = key[k] / (a.n1 + 3);
+      deposit_cell(a, win, wx0, wy0, ix, iy, v, a.k1dep);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // FP64 mover
 // ---------------------------------------------------------------------------
@@ -357,24 +475,42 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
       s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
     }
   }
+  // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
+  // the arrays are padded to whole segments, so the 16-byte loads stay inside
+  // them also past the segment's count
+  const int64_t i0 = base + (int64_t)kItems * t;
+  const int nv = min(max(cnt_in - kItems * t, 0), kItems);
   int C[kItems];
   float FX[kItems], FY[kItems], Z[kItems], VX[kItems], VY[kItems], VZ[kItems];
   bool surv[kItems];
+  uint32_t dead4 = 0;
+  if (nv > 0) {
+    const int4 c4 = *(const int4*)(in.cell + i0);
+    const float4 fx4 = *(const float4*)(in.fx + i0), fy4 = *(const float4*)(in.fy + i0);
+    const float4 vx4 = *(const float4*)(in.vx + i0), vy4 = *(const float4*)(in.vy + i0);
+    const float4 vz4 = *(const float4*)(in.vz + i0);
+    const float4 z4 = in.z ? *(const float4*)(in.z + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
+    dead4 = *(const uint32_t*)(flag_dead + i0);
+    C[0] = c4.x; C[1] = c4.y; C[2] = c4.z; C[3] = c4.w;
+    FX[0] = fx4.x; FX[1] = fx4.y; FX[2] = fx4.z; FX[3] = fx4.w;
+    FY[0] = fy4.x; FY[1] = fy4.y; FY[2] = fy4.z; FY[3] = fy4.w;
+    VX[0] = vx4.x; VX[1] = vx4.y; VX[2] = vx4.z; VX[3] = vx4.w;
+    VY[0] = vy4.x; VY[1] = vy4.y; VY[2] = vy4.z; VY[3] = vy4.w;
+    VZ[0] = vz4.x; VZ[1] = vz4.y; VZ[2] = vz4.z; VZ[3] = vz4.w;
+    Z[0] = z4.x; Z[1] = z4.y; Z[2] = z4.z; Z[3] = z4.w;
+  }
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
-    if (k * BT + t >= cnt_in) continue;
-    int c = in.cell[i];
-    float fx = in.fx[i], fy = in.fy[i];
-    float vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-    float zp = in.z ? in.z[i] : 0.0f;
-    if (flag_dead[i] == 1) {
+    if (k >= nv) continue;
+    const int64_t i = i0 + k;
+    float fx = FX[k], fy = FY[k], vx = VX[k], vy = VY[k], vz = VZ[k], zp = Z[k];
+    if (((dead4 >> (8 * k)) & 0xFF) == 1) {
       flag_dead[i] = 0;
       tally_dead(a, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
       continue;
     }
-    int ix = c & 0xFFFF, iy = c >> 16;
+    int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
     clamp_cell(ix, iy, n1, n2);
     const float px = 1.0f - fx, py = 1.0f - fy;
     const float w0 = px * py, w1 = fx * py, w2 = fx * fy, w3 = px * fy;
@@ -453,34 +589,30 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     surv[k] = true;
   }
   uint32_t off[kItems];
-  compact_offsets<BT>(a, bid, cnt_in, surv, off);
-  const int64_t gbase = base;
+  const uint32_t kept = compact_offsets_tmajor<BT>(a, bid, cnt_in, surv, off);
+  if (kept == (uint32_t)cnt_in && nv == kItems) {
+    // nothing of the segment was lost: every survivor keeps its slot
+    *(int4*)(out.cell + i0) = make_int4(C[0], C[1], C[2], C[3]);
+    *(float4*)(out.fx + i0) = make_float4(FX[0], FX[1], FX[2], FX[3]);
+    *(float4*)(out.fy + i0) = make_float4(FY[0], FY[1], FY[2], FY[3]);
+    if (out.z) *(float4*)(out.z + i0) = make_float4(Z[0], Z[1], Z[2], Z[3]);
+    *(float4*)(out.vx + i0) = make_float4(VX[0], VX[1], VX[2], VX[3]);
+    *(float4*)(out.vy + i0) = make_float4(VY[0], VY[1], VY[2], VY[3]);
+    *(float4*)(out.vz + i0) = make_float4(VZ[0], VZ[1], VZ[2], VZ[3]);
+  } else {
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    if (!surv[k]) continue;
-    const int64_t o = gbase + off[k];
-    out.cell[o] = C[k]; out.fx[o] = FX[k]; out.fy[o] = FY[k];
-    if (out.z) out.z[o] = Z[k];
-    out.vx[o] = VX[k]; out.vy[o] = VY[k]; out.vz[o] = VZ[k];
+    for (int k = 0; k < kItems; ++k) {
+      if (!surv[k]) continue;
+      const int64_t o = base + off[k];
+      out.cell[o] = C[k]; out.fx[o] = FX[k]; out.fy[o] = FY[k];
+      if (out.z) out.z[o] = Z[k];
+      out.vx[o] = VX[k]; out.vy[o] = VY[k]; out.vz[o] = VZ[k];
+    }
   }
   if (!a.deposit) return;
-  const int wx0 = s_wx0, wy0 = s_wy0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    int64_t key = -2 - (int64_t)t;
-    float w[4] = {0.f, 0.f, 0.f, 0.f};
-    int ix = 0, iy = 0;
-    if (surv[k]) {
-      ix = C[k] & 0xFFFF; iy = C[k] >> 16;
-      float px = 1.0f - FX[k], py = 1.0f - FY[k];
-      w[0] = px * py; w[1] = FX[k] * py; w[2] = FX[k] * FY[k]; w[3] = px * FY[k];
-      key = node_index(ix, iy, n1);
-    }
-    bool head = run_reduce(key, w);
-    if (head && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, a.k1dep);
-  }
+  deposit_tmajor(a, win, s_wx0, s_wy0, C, FX, FY, surv);
   __syncthreads();
-  flush_window(a, win, wx0, wy0, a.k1dep);
+  flush_window(a, win, s_wx0, s_wy0, a.k1dep);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,28 +672,24 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx i
     s_wx0 = (c0 & 0xFFFF) - 2 < 0 ? 0 : (c0 & 0xFFFF) - 2;
     s_wy0 = (c0 >> 16) - 1 < 0 ? 0 : (c0 >> 16) - 1;
   }
-  __syncthreads();
-  const int wx0 = s_wx0, wy0 = s_wy0;
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int64_t i = base + (int64_t)k * kBlock + t;
-    int64_t key = -2 - (int64_t)t;
-    float w[4] = {0.f, 0.f, 0.f, 0.f};
-    int ix = 0, iy = 0;
-    bool ok = k * kBlock + t < cnt_in;
-    if (ok) {
-      int c = in.cell[i];
-      ix = c & 0xFFFF; iy = c >> 16;
-      clamp_cell(ix, iy, a.n1, a.n2);
-      float fx = in.fx[i], fy = in.fy[i], px = 1.0f - fx, py = 1.0f - fy;
-      w[0] = px * py; w[1] = fx * py; w[2] = fx * fy; w[3] = px * fy;
-      key = node_index(ix, iy, a.n1);
-    }
-    bool head = run_reduce(key, w);
-    if (head && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, a.k1dep);
+  const int64_t i0 = base + (int64_t)kItems * t;
+  const int nv = min(max(cnt_in - kItems * t, 0), kItems);
+  int C[kItems] = {0, 0, 0, 0};
+  float FX[kItems] = {0.f, 0.f, 0.f, 0.f}, FY[kItems] = {0.f, 0.f, 0.f, 0.f};
+  bool ok[kItems];
+  if (nv > 0) {
+    const int4 c4 = *(const int4*)(in.cell + i0);
+    const float4 fx4 = *(const float4*)(in.fx + i0), fy4 = *(const float4*)(in.fy + i0);
+    C[0] = c4.x; C[1] = c4.y; C[2] = c4.z; C[3] = c4.w;
+    FX[0] = fx4.x; FX[1] = fx4.y; FX[2] = fx4.z; FX[3] = fx4.w;
+    FY[0] = fy4.x; FY[1] = fy4.y; FY[2] = fy4.z; FY[3] = fy4.w;
   }
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) ok[k] = k < nv;
   __syncthreads();
-  flush_window(a, win, wx0, wy0, a.k1dep);
+  deposit_tmajor(a, win, s_wx0, s_wy0, C, FX, FY, ok);
+  __syncthreads();
+  flush_window(a, win, s_wx0, s_wy0, a.k1dep);
 }
 
 MoverCommon common_args(lepic_ctx* ctx, int p) {

@@ -252,18 +252,31 @@ __global__ void k_out(const bf16_t* __restrict__ d0, const float* __restrict__ w
   y[q] = s;
 }
 
+// sum of v over the block into *out (one atomic per block)
+__device__ inline void block_atomic_sum(double v, double* out) {
+  __shared__ double ws[4];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+    if (t != 0.0) atomicAdd(out, t);
+  }
+}
+
+constexpr int kRedBlocks = 256;   // grid of the reduction kernels (grid-stride)
+
 // loss sum and dL/dy = 2 mask (y - t) / nm
-__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, const float* __restrict__ mk,
-                       float inv_nm, float* __restrict__ dy, double* __restrict__ loss, int64_t n) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_loss(const float* __restrict__ y, const float* __restrict__ t,
+                                              const float* __restrict__ mk, float inv_nm, float* __restrict__ dy,
+                                              double* __restrict__ loss, int64_t n) {
   double l = 0.0;
-  if (q < n) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const float d = (y[q] - t[q]) * mk[q];
     dy[q] = 2.0f * d * inv_nm;
-    l = (double)d * (double)d;
+    l += (double)d * (double)d;
   }
-  l = wave_sum(l);
-  if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+  block_atomic_sum(l, loss);
 }
 
 // Poisson residual of the prediction in the network's units:
@@ -279,19 +292,18 @@ __device__ __forceinline__ float stencil_at(const float* u, const float* mk, int
   return cw * (at(x - 1, y) + at(x + 1, y)) + cn * (at(x, y - 1) + at(x, y + 1)) - 2.f * (cw + cn) * at(x, y);
 }
 
-__global__ void k_res_fwd(const float* __restrict__ y, const float* __restrict__ xf, const float* __restrict__ mk,
-                          float cw, float cn, float lam, int B, int H, int W, float* __restrict__ r,
-                          double* __restrict__ loss) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_res_fwd(const float* __restrict__ y, const float* __restrict__ xf,
+                                                 const float* __restrict__ mk, float cw, float cn, float lam, int B,
+                                                 int H, int W, float* __restrict__ r, double* __restrict__ loss) {
   double l = 0.0;
-  if (q < (int64_t)B * H * W) {
+  const int64_t n = (int64_t)B * H * W;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
     const float v = mk[q] != 0.f ? stencil_at(y, mk, b, x, yy, W, H, cw, cn, true) - xf[q] : 0.f;
     r[q] = v;
-    l = (double)lam * (double)v * (double)v;
+    l += (double)lam * (double)v * (double)v;
   }
-  l = wave_sum(l);
-  if ((threadIdx.x & 63) == 0 && l != 0.0) atomicAdd(loss, l);
+  block_atomic_sum(l, loss);
 }
 
 // dL/dy += mask * A(r) * scale   (A is symmetric)
@@ -304,27 +316,31 @@ __global__ void k_res_bwd(const float* __restrict__ r, const float* __restrict__
 }
 
 // gradient of the output layer: dw[c] += sum d0*dy; g_d0 = bf16(dy*w) * (d0 > 0)
-__global__ void k_out_bwd(const bf16_t* __restrict__ d0, const float* __restrict__ w, const float* __restrict__ dy,
-                          float* __restrict__ dw, bf16_t* __restrict__ g, int B, int H, int W) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = q < (int64_t)B * H * W;
-  float part[16];
-  if (ok) {
+__global__ __launch_bounds__(256) void k_out_bwd(const bf16_t* __restrict__ d0, const float* __restrict__ w,
+                                                 const float* __restrict__ dy, float* __restrict__ dw,
+                                                 bf16_t* __restrict__ g, int B, int H, int W) {
+  __shared__ double ws[4][16];
+  double part[16];
+  for (int c = 0; c < 16; ++c) part[c] = 0.0;
+  const int64_t n = (int64_t)B * H * W;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
     const int x = (int)(q % W), yy = (int)((q / W) % H), b = (int)(q / ((int64_t)W * H));
     const size_t p = px(b, yy, x, H, W) * 16;
     const float d = dy[q];
     for (int c = 0; c < 16; ++c) {
       const bf16_t a = d0[p + c];
-      part[c] = bf2f(a) * d;
+      part[c] += (double)(bf2f(a) * d);
       g[p + c] = bf_pos(a) ? f2bf(d * w[c]) : (bf16_t)0;
     }
-  } else {
-    for (int c = 0; c < 16; ++c) part[c] = 0.f;
   }
   for (int c = 0; c < 16; ++c) {
-    float v = part[c];
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v != 0.f) atomicAdd(&dw[c], v);
+    const double v = wave_sum(part[c]);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const double t = (ws[0][threadIdx.x] + ws[1][threadIdx.x]) + (ws[2][threadIdx.x] + ws[3][threadIdx.x]);
+    if (t != 0.0) atomicAdd(&dw[threadIdx.x], (float)t);
   }
 }
 
@@ -382,13 +398,13 @@ __global__ void k_mask_image(const int8_t* __restrict__ bc, int n1, int n2, int 
   mk[q] = (x <= n1 && y <= n2 && bc[(int64_t)(y + 1) * (n1 + 3) + (x + 1)] == -1) ? 1.f : 0.f;
 }
 
-// sum of squares over mask (double, atomics)
-__global__ void k_sumsq(const float* __restrict__ v, const float* __restrict__ mk, int64_t n, double* out) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// sum of squares over mask (double, one atomic per block)
+__global__ __launch_bounds__(256) void k_sumsq(const float* __restrict__ v, const float* __restrict__ mk, int64_t n,
+                                               double* out) {
   double s = 0.0;
-  if (q < n) s = (double)v[q] * (double)v[q] * (double)mk[q];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0 && s != 0.0) atomicAdd(out, s);
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    s += (double)v[q] * (double)v[q] * (double)mk[q];
+  block_atomic_sum(s, out);
 }
 
 // batch slot: x channels (v * sx, mask) into the 16-channel input tensor,
@@ -502,6 +518,12 @@ struct Net {
   bool trained = false;
   bool recording = false;
   bool have_last = false;
+  // when to ask the network: a probe whose guess does not lower the initial
+  // residual by min_gain skips the next `backoff` solves and doubles the
+  // back-off (up to max_backoff); a good guess resets it
+  float min_gain = 2.0f;
+  int max_backoff = 256;
+  int backoff = 0, skip_left = 0;
   double* rhs_last = nullptr;
   double* phi_last = nullptr;
   float* img_tmp = nullptr;
@@ -541,7 +563,7 @@ int alloc_tensors(lepic_ctx* ctx, Net& N, int B, int H, int W) {
   const size_t hw = (size_t)B * H * W;
   const size_t fbytes = 6 * ((hw * 4 + 63) & ~(size_t)63);
   LEPIC_HIP(hipMalloc(&N.arena, total + fbytes + 256));
-  LEPIC_HIP(hipMemset(N.arena, 0, total + fbytes + 256));
+  LEPIC_HIP(hipMemsetAsync(N.arena, 0, total + fbytes + 256, ctx->stream));
   char* p = (char*)N.arena;
   for (const Spec& s : specs) {
     s.t->p = (bf16_t*)p;
@@ -680,7 +702,7 @@ int backward(lepic_ctx* ctx, Net& N) {
   int rc = zero_async(ctx, N.g, N.nparam * 4);
   if (rc) return rc;
   const int64_t hw = (int64_t)N.B * N.H * N.W;
-  hipLaunchKernelGGL(k_out_bwd, dim3(nblk(hw)), dim3(256), 0, ctx->stream, N.D0.p, (const float*)(N.w + N.nconv),
+  hipLaunchKernelGGL(k_out_bwd, dim3(std::min<unsigned>(nblk(hw), kRedBlocks)), dim3(256), 0, ctx->stream, N.D0.p, (const float*)(N.w + N.nconv),
                      (const float*)N.dY, N.g + N.nconv, N.G_D0.p, N.B, N.H, N.W);
   LEPIC_HIP(hipGetLastError());
   auto WG = [&](int l, const T4& x, int xo, const T4& dz, int dzo, int lvl) {
@@ -766,8 +788,10 @@ int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const flo
   LEPIC_HIP(hipMalloc(&dx, 2 * hw * 4));
   for (int b = 0; b < B; ++b) {
     for (size_t q = 0; q < hw; ++q) { a[q] = x[((size_t)b * hw + q) * 2]; c[q] = x[((size_t)b * hw + q) * 2 + 1]; }
-    LEPIC_HIP(hipMemcpy(dx, a.data(), hw * 4, hipMemcpyHostToDevice));
-    LEPIC_HIP(hipMemcpy(dx + hw, c.data(), hw * 4, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(dx, a.data(), hw * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    LEPIC_HIP(hipMemcpyAsync(dx + hw, c.data(), hw * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
     hipLaunchKernelGGL(k_fill_input, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)dx,
                        (const float*)(dx + hw), 1.0f, N.X.p, b, H, W, (const float*)nullptr, 0.f, (float*)nullptr,
                        (float*)nullptr, N.Xf);
@@ -775,8 +799,10 @@ int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const flo
   }
   LEPIC_HIP(hipFree(dx));
   if (t) {
-    LEPIC_HIP(hipMemcpy(N.Tt, t, (size_t)B * hw * 4, hipMemcpyHostToDevice));
-    LEPIC_HIP(hipMemcpy(N.Mk, mk, (size_t)B * hw * 4, hipMemcpyHostToDevice));
+    LEPIC_HIP(hipMemcpyAsync(N.Tt, t, (size_t)B * hw * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    LEPIC_HIP(hipMemcpyAsync(N.Mk, mk, (size_t)B * hw * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
 }
@@ -785,10 +811,10 @@ int load_batch(lepic_ctx* ctx, Net& N, const float* x, const float* t, const flo
 int loss_launch(lepic_ctx* ctx, Net& N, double nm, float lam, float cw, float cn) {
   const int64_t n = (int64_t)N.B * N.H * N.W;
   if (int rc = zero_async(ctx, N.d_loss, 8)) return rc;
-  hipLaunchKernelGGL(k_loss, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Tt,
+  hipLaunchKernelGGL(k_loss, dim3(std::min<unsigned>(nblk(n), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Tt,
                      (const float*)N.Mk, (float)(1.0 / nm), N.dY, N.d_loss, n);
   if (lam > 0.f) {
-    hipLaunchKernelGGL(k_res_fwd, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Xf,
+    hipLaunchKernelGGL(k_res_fwd, dim3(std::min<unsigned>(nblk(n), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)N.Y, (const float*)N.Xf,
                        (const float*)N.Mk, cw, cn, lam, N.B, N.H, N.W, N.Rr, N.d_loss);
     hipLaunchKernelGGL(k_res_bwd, dim3(nblk(n)), dim3(256), 0, ctx->stream, (const float*)N.Rr, (const float*)N.Mk,
                        cw, cn, (float)(2.0 * lam / nm), N.B, N.H, N.W, N.dY);
@@ -845,9 +871,9 @@ int net_after_solve(lepic_ctx* ctx) {
     hipLaunchKernelGGL(k_diff_to_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const double*)ctx->phi,
                        (const double*)N->phi_last, 1, n1, n2, N->Ws, N->Hs, st);
     if (int rc = zero_async(ctx, N->d_tmp, 16)) return rc;
-    hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)sx,
+    hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(nblk(hw), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)sx,
                        (const float*)N->mask_img, hw, N->d_tmp);
-    hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)st,
+    hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(nblk(hw), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)st,
                        (const float*)N->mask_img, hw, N->d_tmp + 1);
     LEPIC_HIP(hipGetLastError());
     LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, N->d_tmp, 16, hipMemcpyDeviceToHost, ctx->stream));
@@ -880,7 +906,7 @@ int net_guess(lepic_ctx* ctx, double* out) {
   hipLaunchKernelGGL(k_diff_to_image, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const double*)ctx->rhs,
                      (const double*)N->rhs_last, 0, n1, n2, N->Ws, N->Hs, N->img_tmp);
   if (int rc = zero_async(ctx, N->d_tmp, 16)) return rc;
-  hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->img_tmp,
+  hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(nblk(hw), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)N->img_tmp,
                      (const float*)N->mask_img, hw, N->d_tmp);
   LEPIC_HIP(hipGetLastError());
   const double rx = std::sqrt(read_d(ctx, N->d_tmp) / N->nmask);
@@ -907,6 +933,29 @@ int net_guess(lepic_ctx* ctx, double* out) {
   return 0;
 }
 
+// Whether this START_NET solve evaluates the network (false while backing off).
+bool net_probe_due(lepic_ctx* ctx) {
+  Net* N = ctx->net;
+  if (!N) return false;
+  if (N->skip_left > 0) {
+    --N->skip_left;
+    return false;
+  }
+  return true;
+}
+
+// Outcome of a probe: residual of the previous-phi start over that of the guess.
+void net_probe_result(lepic_ctx* ctx, double gain) {
+  Net* N = ctx->net;
+  if (!N) return;
+  if (gain >= (double)N->min_gain || N->max_backoff <= 0) {
+    N->backoff = 0;
+    return;
+  }
+  N->backoff = std::min(std::max(2 * N->backoff, 8), N->max_backoff);
+  N->skip_left = N->backoff;
+}
+
 }  // namespace lepic
 
 extern "C" {
@@ -931,8 +980,8 @@ int lepic_net_create(lepic_ctx* ctx, int32_t capacity, uint64_t seed) {
   LEPIC_HIP(hipMalloc(&N->g, N->nparam * 4));
   LEPIC_HIP(hipMalloc(&N->wb, N->nconv * 2));
   LEPIC_HIP(hipMalloc(&N->wt, N->nconv * 2));
-  LEPIC_HIP(hipMemset(N->m, 0, N->nparam * 4));
-  LEPIC_HIP(hipMemset(N->v, 0, N->nparam * 4));
+  LEPIC_HIP(hipMemsetAsync(N->m, 0, N->nparam * 4, ctx->stream));
+  LEPIC_HIP(hipMemsetAsync(N->v, 0, N->nparam * 4, ctx->stream));
   LEPIC_HIP(hipMalloc(&N->d_loss, 8));
   LEPIC_HIP(hipMalloc(&N->d_tmp, 64));
   // He-normal initialisation on the logical fan-in
@@ -948,7 +997,8 @@ int lepic_net_create(lepic_ctx* ctx, int32_t capacity, uint64_t seed) {
   for (int c = 0; c < 16; ++c) flat[o++] = 0.25f * nd(N->rng);
   std::vector<float> gpu;
   logical_to_gpu(*N, flat.data(), gpu);
-  LEPIC_HIP(hipMemcpy(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice));
+  LEPIC_HIP(hipMemcpyAsync(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   if (int rc = refresh_bf16(ctx, *N)) return rc;
   // samples on the node grid image
   N->Hs = round8(ctx->g.n2 + 1);
@@ -969,7 +1019,7 @@ int lepic_net_create(lepic_ctx* ctx, int32_t capacity, uint64_t seed) {
                      ctx->g.n2, N->Ws, N->Hs, N->mask_img);
   LEPIC_HIP(hipGetLastError());
   if (int rc = zero_async(ctx, N->d_tmp, 8)) return rc;
-  hipLaunchKernelGGL(k_sumsq, dim3(nblk(hw)), dim3(256), 0, ctx->stream, (const float*)N->mask_img,
+  hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(nblk(hw), kRedBlocks)), dim3(256), 0, ctx->stream, (const float*)N->mask_img,
                      (const float*)N->mask_img, hw, N->d_tmp);
   N->nmask = read_d(ctx, N->d_tmp);
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
@@ -980,6 +1030,16 @@ int lepic_net_recording(lepic_ctx* ctx, int32_t on) {
   Net* N = need_net(ctx);
   if (!N) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
   N->recording = on != 0;
+  return 0;
+}
+
+int lepic_net_policy(lepic_ctx* ctx, float min_gain, int32_t max_backoff) {
+  Net* N = need_net(ctx);
+  if (!N) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (!(min_gain >= 0.f) || max_backoff < 0) return LEPIC_ERR_ARG;
+  N->min_gain = min_gain;
+  N->max_backoff = max_backoff;
+  N->backoff = N->skip_left = 0;
   return 0;
 }
 
@@ -1037,9 +1097,10 @@ int lepic_net_set_params(lepic_ctx* ctx, const float* flat, int64_t n) {
   if ((size_t)n != logical_size(*N)) return ctx->fail(LEPIC_ERR_ARG, "parameter count mismatch");
   std::vector<float> gpu;
   logical_to_gpu(*N, flat, gpu);
-  LEPIC_HIP(hipMemcpy(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice));
-  LEPIC_HIP(hipMemset(N->m, 0, N->nparam * 4));
-  LEPIC_HIP(hipMemset(N->v, 0, N->nparam * 4));
+  LEPIC_HIP(hipMemcpyAsync(N->w, gpu.data(), N->nparam * 4, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  LEPIC_HIP(hipMemsetAsync(N->m, 0, N->nparam * 4, ctx->stream));
+  LEPIC_HIP(hipMemsetAsync(N->v, 0, N->nparam * 4, ctx->stream));
   N->adam_t = 0;
   return refresh_bf16(ctx, *N);
 }
@@ -1050,7 +1111,8 @@ int lepic_net_get_params(lepic_ctx* ctx, float* flat, int64_t n) {
   if ((size_t)n != logical_size(*N)) return ctx->fail(LEPIC_ERR_ARG, "parameter count mismatch");
   std::vector<float> gpu(N->nparam);
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(gpu.data(), N->w, N->nparam * 4, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(gpu.data(), N->w, N->nparam * 4, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   gpu_to_logical(*N, gpu.data(), flat);
   return 0;
 }
@@ -1063,7 +1125,8 @@ int lepic_net_forward(lepic_ctx* ctx, const float* x, int32_t B, int32_t H, int3
   if (int rc = load_batch(ctx, *N, x, nullptr, nullptr, B, H, W)) return rc;
   if (int rc = forward(ctx, *N)) return rc;
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpy(y, N->Y, (size_t)B * H * W * 4, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(y, N->Y, (size_t)B * H * W * 4, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   return 0;
 }
 
@@ -1083,7 +1146,8 @@ int lepic_net_loss_grad(lepic_ctx* ctx, const float* x, const float* t, const fl
   if (int rc = backward(ctx, *N)) return rc;
   *loss = read_d(ctx, N->d_loss) / nm;
   std::vector<float> gpu(N->nparam);
-  LEPIC_HIP(hipMemcpy(gpu.data(), N->g, N->nparam * 4, hipMemcpyDeviceToHost));
+  LEPIC_HIP(hipMemcpyAsync(gpu.data(), N->g, N->nparam * 4, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   gpu_to_logical(*N, gpu.data(), grad);
   return 0;
 }

@@ -186,6 +186,12 @@ module lepic_capi
        type(c_ptr), value :: ctx
        integer(c_int32_t), value :: on
      end function lepic_net_recording
+     integer(c_int) function lepic_net_policy(ctx, min_gain, max_backoff) bind(C, name='lepic_net_policy')
+       import :: c_int, c_ptr, c_int32_t, c_float
+       type(c_ptr), value :: ctx
+       real(c_float), value :: min_gain
+       integer(c_int32_t), value :: max_backoff
+     end function lepic_net_policy
      integer(c_int) function lepic_net_train(ctx, iters, batch, lr, loss_first, loss_last) &
           bind(C, name='lepic_net_train')
        import :: c_int, c_ptr, c_int32_t, c_float

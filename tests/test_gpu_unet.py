@@ -107,6 +107,7 @@ def test_training_and_warm_start_gain():
             ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
         assert ctx.net_samples() == 40
         ctx.net_recording(False)
+        ctx.net_policy(min_gain=0.0, max_backoff=0)    # evaluate the network on every solve
         l0, l1 = ctx.net_train(300, batch=4, lr=2e-3)
         print(f"training loss {l0:.4g} -> {l1:.4g}")
         assert l1 < 0.3 * l0, (l0, l1)
@@ -125,3 +126,38 @@ def test_training_and_warm_start_gain():
         assert used >= 8, used
         assert np.median(gains) > 2.0, gains
         assert np.isfinite(ref).all()
+
+
+def test_probe_backoff():
+    """A guess that misses min_gain makes the next 8, 16, ... max_backoff
+    START_NET solves start from the previous phi without the network."""
+    n = 32
+    ctx, _ = box_ctx(n)
+    phase = [0.4, 2.0]
+    with ctx:
+        ctx.net_create(capacity=8, seed=3)
+        ctx.net_recording(True)
+        ctx.set_grid(L.GRID_RHS, moving_rhs(n, 0, phase))
+        ctx.iterative_solver(eps=1e-10, start=L.START_COLD)
+        for it in range(1, 5):
+            ctx.set_grid(L.GRID_RHS, moving_rhs(n, it, phase))
+            ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
+        ctx.net_recording(False)
+        ctx.net_train(1, batch=2, lr=1e-4)
+        ctx.net_policy(min_gain=1e30, max_backoff=16)   # no guess can pass
+        probed = []
+        for it in range(5, 35):
+            ctx.set_grid(L.GRID_RHS, moving_rhs(n, it, phase))
+            info = ctx.iterative_solver(eps=1e-10, start=L.START_NET)
+            assert info["res"] <= 1e-10
+            if info["res0_alt"] >= 0.0:
+                probed.append(it - 5)
+            else:
+                assert info["start_used"] == L.START_PREV
+        print("probed", probed)
+        assert probed == [0, 9, 26], probed
+        ctx.net_policy(min_gain=0.0, max_backoff=256)   # every guess passes
+        for it in range(35, 38):
+            ctx.set_grid(L.GRID_RHS, moving_rhs(n, it, phase))
+            info = ctx.iterative_solver(eps=1e-10, start=L.START_NET)
+            assert info["res0_alt"] >= 0.0
