@@ -47,6 +47,7 @@ constexpr int kSweepThreads = 1024;
 constexpr int kDefaultTailNodes = 5000;
 constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
+constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 128;               // row-loop blocks of the grid-transfer kernels
 
 template <typename T> struct TileY;
@@ -628,21 +629,203 @@ __device__ void t_smooth(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T 
   *res_out = res;
 }
 
-template <typename T>
-__device__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps, int dig,
+// Red-black sweeps of one tail level with the residual folded into the
+// sweeps.  Interior node q = (j-1)(n1+1) + (i-1) is red (i+j odd) exactly
+// when q is odd (n1+1 is odd), so a participant handles red nodes 2m+1 and
+// black nodes 2m for m = tid + P*r, r < R, with the indices computed once.
+// The residual after sweep k is the black nodes' residual, taken right after
+// their update (their red neighbours are final), plus the red nodes'
+// residual, taken in the red pass of sweep k+1 from the same loads as that
+// pass's update; the update is stored only if the exit test (sors.f90:371-377)
+// lets sweep k+1 run.  Expressions and exit tests are those of t_smooth.
+// WAVE: wave 0 alone (P = 64, no block barriers); else the whole block.
+template <bool WAVE>
+__device__ __forceinline__ void tail_sync() {
+  if (WAVE) {
+    // a level may live in global memory: wait for the wave's stores and
+    // order the later loads after them, as a block barrier would
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
+// sum over the participants, the same value in every participant; ws holds
+// two banks of 16 wave sums used alternately (par), so a bank is rewritten
+// only after the barriers of a whole sweep
+template <bool WAVE>
+__device__ __forceinline__ double tail_sum(double v, double* ws, int par) {
+  v = wave_sum(v);
+  if (WAVE) return v;
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double* bank = ws + 16 * par;
+  if ((threadIdx.x & 63) == 0) bank[w] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int q = 0; q < nw; ++q) t += bank[q];
+  return t;
+}
+
+template <typename T, int R, bool WAVE, typename PU, typename PB, typename PC>
+__device__ __forceinline__ void t_smooth_fused(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega, double eps,
+                               int dig, int nr, double* ws, int* ksor_out, double* res_out) {
+  const int W = n1 + 3, n1p = n1 + 1;
+  const int nint = n1p * (n2 + 1);
+  const int P = WAVE ? 64 : (int)blockDim.x;
+  const int tid = WAVE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  int xr[R], qr[R], xk[R], qk[R];
+  bool ur[R], rr[R], uk[R], rk[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = tid + P * r;
+    int q = 2 * m + 1;
+    bool ok = q < nint;
+    q = ok ? q : 0;
+    int x = (q / n1p + 1) * W + q % n1p + 1;
+    int8_t mq = bc[x];
+    xr[r] = x; qr[r] = q; ur[r] = ok && mq <= 0; rr[r] = ok && mq == -1;
+    q = 2 * m;
+    ok = q < nint;
+    q = ok ? q : 0;
+    x = (q / n1p + 1) * W + q % n1p + 1;
+    mq = bc[x];
+    xk[r] = x; qk[r] = q; uk[r] = ok && mq <= 0; rk[r] = ok && mq == -1;
+  }
+  T cand[R];
+  auto red_pass = [&](bool want_res) {
+    double rs = 0.0;
+    T c[R], w[R], e[R], nn[R], so[R], bb[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      c[r] = u[xr[r]]; w[r] = u[xr[r] - 1]; e[r] = u[xr[r] + 1];
+      nn[r] = u[xr[r] + W]; so[r] = u[xr[r] - W]; bb[r] = b[qr[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (want_res && rr[r]) rs += fabs(sor_resid(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac));
+      cand[r] = sor_update(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac, omega);
+    }
+    return rs;
+  };
+  auto store_red = [&]() {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (ur[r]) u[xr[r]] = cand[r];
+  };
+  auto black_pass = [&]() {
+    double rs = 0.0;
+    T c[R], w[R], e[R], nn[R], so[R], bb[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      c[r] = u[xk[r]]; w[r] = u[xk[r] - 1]; e[r] = u[xk[r] + 1];
+      nn[r] = u[xk[r] + W]; so[r] = u[xk[r] - W]; bb[r] = b[qk[r]];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      T v = c[r];
+      if (uk[r]) {
+        v = sor_update(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac, omega);
+        u[xk[r]] = v;
+      }
+      if (rk[r]) rs += fabs(sor_resid(v, bb[r], w[r], nn[r], so[r], e[r], aw, an, ac));
+    }
+    return rs;
+  };
+  // sweep 1
+  red_pass(false);
+  store_red();
+  tail_sync<WAVE>();
+  double rb = black_pass();
+  tail_sync<WAVE>();
+  int k = 1;
+  double res_s = 0.0, res = 0.0;
+  for (;;) {
+    const double rs = red_pass(true);
+    res = tail_sum<WAVE>(rs + rb, ws, k & 1);
+    bool stop;
+    if (dig == 0) stop = res <= eps;
+    else stop = res_s > 0.0 && (res / res_s) <= 0.6;
+    if (stop || k >= nr) break;
+    res_s = res;
+    store_red();
+    tail_sync<WAVE>();
+    rb = black_pass();
+    tail_sync<WAVE>();
+    ++k;
+  }
+  // the last red pass only read; the next phase may overwrite nothing it read
+  tail_sync<WAVE>();
+  *ksor_out = k;
+  *res_out = res;
+}
+
+// items per participant and colour of a level, 0 if above kFusedMaxR
+constexpr int kFusedMaxR = 3;
+static_assert((kWaveNodes + 1) / 2 <= 64 * kFusedMaxR, "wave-mode levels must fit the fused smoother");
+__device__ __forceinline__ int fused_items(int n1, int n2, int P) {
+  const int half = ((n1 + 1) * (n2 + 1) + 1) / 2;
+  const int R = (half + P - 1) / P;
+  return R <= kFusedMaxR ? R : 0;
+}
+
+template <typename T, bool WAVE, typename PU, typename PB, typename PC>
+__device__ __forceinline__ bool t_smooth_fused_any(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega, double eps,
+                                   int dig, int nr, double* ws, int* ks, double* res) {
+  switch (fused_items(n1, n2, WAVE ? 64 : (int)blockDim.x)) {
+    case 1: t_smooth_fused<T, 1, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
+    case 2: t_smooth_fused<T, 2, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
+    case 3: t_smooth_fused<T, 3, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
+    default: return false;
+  }
+}
+
+template <typename T, bool WAVE = false>
+__device__ __forceinline__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps, int dig,
                              int nr, double* ws, int* ks, double* res) {
-  if (L.lds)
-    t_smooth<T>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
-                eps, dig, nr, ws, ks, res);
-  else
-    t_smooth<T>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
-                eps, dig, nr, ws, ks, res);
+  if (L.lds) {
+    if (t_smooth_fused_any<T, WAVE>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw,
+                                    L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
+      return;
+    if (!WAVE)
+      t_smooth<T>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
+                  eps, dig, nr, ws, ks, res);
+  } else {
+    if (t_smooth_fused_any<T, WAVE>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw,
+                                    L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
+      return;
+    if (!WAVE)
+      t_smooth<T>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
+                  eps, dig, nr, ws, ks, res);
+  }
+}
+
+// restriction of level l onto l+1 and prolongation of l+1 onto l by the
+// participants (tid, P)
+template <typename T>
+__device__ __forceinline__ void tail_restrict(const TLevel<T>& F, const TLevel<T>& C, T* uf, T* bf, const int8_t* bcf, T* e2h,
+                              T* r2h, const int8_t* bcc, int tid, int P) {
+  for (int q = tid; q < (C.n1 + 3) * (C.n2 + 3); q += P) e2h[q] = (T)0;
+  for (int q = tid; q < (C.n1 + 1) * (C.n2 + 1); q += P) {
+    const int i = q % (C.n1 + 1) + 1, j = q / (C.n1 + 1) + 1;
+    T out = (T)0;
+    if (bcc[j * (C.n1 + 3) + i] <= 0) out = restrict_node(uf, bf, bcf, i, j, F.n1, F.n2, F.aw, F.an, F.ac);
+    r2h[q] = out;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_prolong(const TLevel<T>& F, const TLevel<T>& C, T* uf, const int8_t* bcf, const T* uc,
+                             int tid, int P) {
+  for (int q = tid; q < (C.n1 + 1) * (C.n2 + 1); q += P)
+    prolong_node(uf, bcf, F.n1, uc, C.n1, q % (C.n1 + 1) + 1, q / (C.n1 + 1) + 1);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ double ws[16];
+  __shared__ double ws[32];
   __shared__ T* sU[kMaxLevels + 1];
   __shared__ T* sB[kMaxLevels + 1];
   __shared__ const int8_t* sBC[kMaxLevels + 1];
@@ -669,37 +852,49 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
   __syncthreads();
   int ks;
   double res = 0.0;
-  for (int l = Ls; l < ng; ++l) {  // down leg: ig = l
+  // levels from Lw down are run by wave 0 alone (no block barriers)
+  int Lw = ng + 1;
+  for (int l = ng; l >= Ls && (A.lv[l].n1 + 1) * (A.lv[l].n2 + 1) <= kWaveNodes; --l) Lw = l;
+  const int tid = threadIdx.x, P = blockDim.x;
+  for (int l = Ls; l < ng && l < Lw; ++l) {  // down leg: ig = l
     const TLevel<T>& F = A.lv[l];
-    const TLevel<T>& C = A.lv[l + 1];
-    smooth_level(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
-    if (threadIdx.x == 0) { A.visit[l] = ks; A.vres[l] = res; }
-    // restriction: coarse rhs and zero coarse correction
-    T* e2h = sU[l + 1];
-    T* r2h = sB[l + 1];
-    for (int q = threadIdx.x; q < (C.n1 + 3) * (C.n2 + 3); q += blockDim.x) e2h[q] = (T)0;
-    for (int q = threadIdx.x; q < (C.n1 + 1) * (C.n2 + 1); q += blockDim.x) {
-      const int i = q % (C.n1 + 1) + 1, j = q / (C.n1 + 1) + 1;
-      T out = (T)0;
-      if (sBC[l + 1][j * (C.n1 + 3) + i] <= 0)
-        out = restrict_node(sU[l], sB[l], sBC[l], i, j, F.n1, F.n2, F.aw, F.an, F.ac);
-      r2h[q] = out;
+    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
+    if (tid == 0) { A.visit[l] = ks; A.vres[l] = res; }
+    tail_restrict<T>(F, A.lv[l + 1], sU[l], sB[l], sBC[l], sU[l + 1], sB[l + 1], sBC[l + 1], tid, P);
+    __syncthreads();
+  }
+  if (Lw <= ng) {
+    if (tid < 64) {
+      for (int l = Lw; l < ng; ++l) {
+        const TLevel<T>& F = A.lv[l];
+        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
+        if (tid == 0) { A.visit[l] = ks; A.vres[l] = res; }
+        tail_restrict<T>(F, A.lv[l + 1], sU[l], sB[l], sBC[l], sU[l + 1], sB[l + 1], sBC[l + 1], tid, 64);
+        tail_sync<true>();
+      }
+      smooth_level<T, true>(A.lv[ng], sU[ng], sB[ng], sBC[ng], A.omega, A.eps, 0, A.k == 1 ? 1000000 : kNr, ws,
+                            &ks, &res);
+      if (tid == 0) { A.visit[ng] = ks; A.vres[ng] = res; }
+      for (int l = ng - 1; l >= Lw; --l) {
+        const TLevel<T>& F = A.lv[l];
+        tail_prolong<T>(F, A.lv[l + 1], sU[l], sBC[l], (const T*)sU[l + 1], tid, 64);
+        tail_sync<true>();
+        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
+        if (tid == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
+      }
     }
     __syncthreads();
+  } else {  // coarsest: ig = ng; the first V-cycle iterates to eps (sors.f90:63-64)
+    smooth_level<T>(A.lv[ng], sU[ng], sB[ng], sBC[ng], A.omega, A.eps, 0, A.k == 1 ? 1000000 : kNr, ws, &ks,
+                    &res);
+    if (tid == 0) { A.visit[ng] = ks; A.vres[ng] = res; }
   }
-  {  // coarsest: ig = ng; the first V-cycle iterates to eps (sors.f90:63-64)
-    const TLevel<T>& L = A.lv[ng];
-    smooth_level(L, sU[ng], sB[ng], sBC[ng], A.omega, A.eps, 0, A.k == 1 ? 1000000 : kNr, ws, &ks, &res);
-    if (threadIdx.x == 0) { A.visit[ng] = ks; A.vres[ng] = res; }
-  }
-  for (int l = ng - 1; l >= Ls; --l) {  // up leg: ig = 2ng - l
+  for (int l = std::min(Lw, ng) - 1; l >= Ls; --l) {  // up leg: ig = 2ng - l
     const TLevel<T>& F = A.lv[l];
-    const TLevel<T>& C = A.lv[l + 1];
-    for (int q = threadIdx.x; q < (C.n1 + 1) * (C.n2 + 1); q += blockDim.x)
-      prolong_node(sU[l], sBC[l], F.n1, (const T*)sU[l + 1], C.n1, q % (C.n1 + 1) + 1, q / (C.n1 + 1) + 1);
+    tail_prolong<T>(F, A.lv[l + 1], sU[l], sBC[l], (const T*)sU[l + 1], tid, P);
     __syncthreads();
-    smooth_level(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
-    if (threadIdx.x == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
+    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
+    if (tid == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
   }
   const TLevel<T>& L = A.lv[Ls];
   if (L.lds)
