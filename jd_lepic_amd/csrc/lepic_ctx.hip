@@ -520,6 +520,7 @@ int lepic_set_boundary(lepic_ctx* ctx, const int32_t* bcnd, const double* phi) {
     fine.swap(c);
   }
   ctx->boundary_set = true;
+  ctx->tail_bc_key = -1;   // coarse bc changed: rebuild the tail's LDS image
   return 0;
 }
 

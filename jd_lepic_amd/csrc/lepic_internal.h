@@ -139,6 +139,9 @@ struct lepic_ctx {
   int ng = 1;
   lepic::Level lv[lepic::kMaxLevels + 1];
   int Lc = 1;                        // first level handled by the single-workgroup kernel
+  int8_t* d_tail_bc = nullptr;       // LDS image of the tail levels' bc arrays
+  size_t tail_bc_cap = 0;
+  int tail_bc_key = -1;              // layout the image was built for (-1: rebuild)
   int32_t mg_epoch = 0;              // epoch of the last multi-block level visit
   lepic::LevelCtrl* d_ctrl = nullptr;   // [kMaxLevels+1]
   double* d_partials = nullptr;      // FP64 residual partial sums of the finest level

@@ -38,7 +38,7 @@ using namespace lepic;
 namespace {
 
 constexpr int kVisitLog = 2 * kMaxLevels;     // ig = 1 .. 2ng-1
-constexpr size_t kTailLds = 64 * 1024;
+constexpr size_t kTailLds = 160 * 1024 - 2048;  // dynamic LDS of k_tail (static arrays take < 2 KB)
 constexpr int kTailThreads = 1024;
 constexpr int kS = kSweepsPerLaunch;
 constexpr int kHalo = 2 * kS + 1;             // validity shrinks by one node per colour pass
@@ -55,9 +55,14 @@ template <> struct TileY<float> { static constexpr int v = 32; };
 template <> struct TileY<double> { static constexpr int v = 16; };
 
 // ---- SOR update of one node (sors.f90:108-110 / 238-240) ----
+// FP64 divides like the reference; the FP32 smoother of the mixed form
+// multiplies by the reciprocal (the FP64 refinement sets the final accuracy)
 template <typename T>
 __device__ __forceinline__ T sor_update(T u, T b, T uW, T uN, T uS, T uE, T aw, T an, T ac, T omega) {
-  return ((T)1.0 - omega) * u + omega * (b - aw * uW - an * uN - an * uS - aw * uE) / ac;
+  if constexpr (sizeof(T) == 4)
+    return (1.0f - omega) * u + omega * (b - aw * uW - an * uN - an * uS - aw * uE) * (1.0f / ac);
+  else
+    return ((T)1.0 - omega) * u + omega * (b - aw * uW - an * uN - an * uS - aw * uE) / ac;
 }
 
 // ---- residual of one node, sors.f90:348-349 order ----
@@ -67,8 +72,8 @@ __device__ __forceinline__ double sor_resid(T u, T b, T uW, T uN, T uS, T uE, T 
 }
 
 // ---- getres, mg.f90:512-547 order (aw, ae, an, as, ac) ----
-template <typename T>
-__device__ __forceinline__ T getres(const T* u, const T* b, int ir, int jr, int n1, T aw, T an, T ac) {
+template <typename T, typename PU, typename PB>
+__device__ __forceinline__ T getres(PU u, PB b, int ir, int jr, int n1, T aw, T an, T ac) {
   const int64_t W = n1 + 3;
   const int64_t c = (int64_t)jr * W + ir;
   T uij = u[c], un = u[c + W], us = u[c - W], ue = u[c + 1], uw = u[c - 1];
@@ -77,9 +82,8 @@ __device__ __forceinline__ T getres(const T* u, const T* b, int ir, int jr, int 
 }
 
 // full-weighted residual of coarse node (i,j), mg.f90:386-414
-template <typename T>
-__device__ __forceinline__ T restrict_node(const T* u, const T* b, const int8_t* bc, int i, int j, int n1,
-                                           int n2, T aw, T an, T ac) {
+template <typename T, typename PU, typename PB, typename PC>
+__device__ __forceinline__ T restrict_node(PU u, PB b, PC bc, int i, int j, int n1, int n2, T aw, T an, T ac) {
   const int ir = 2 * i - 1, jr = 2 * j - 1;
   const int64_t W = n1 + 3;
   auto B = [&](int a, int c) { return bc[(int64_t)c * W + a]; };
@@ -98,9 +102,8 @@ __device__ __forceinline__ T restrict_node(const T* u, const T* b, const int8_t*
 
 // bilinear add of coarse node (i,j)'s correction to its four fine nodes,
 // mg.f90:585-611
-template <typename T>
-__device__ __forceinline__ void prolong_node(T* u, const int8_t* bc, int n1, const T* e2h, int m1, int i,
-                                             int j) {
+template <typename T, typename PU, typename PC, typename PE>
+__device__ __forceinline__ void prolong_node(PU u, PC bc, int n1, PE e2h, int m1, int i, int j) {
   const int64_t Wc = m1 + 3, W = n1 + 3;
   T r1 = e2h[(int64_t)j * Wc + i], r2 = e2h[(int64_t)(j - 1) * Wc + i];
   T r3 = e2h[(int64_t)(j - 1) * Wc + i - 1], r4 = e2h[(int64_t)j * Wc + i - 1];
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(128) void k_restrict(RestrictArgs<T> A) {
     if (i < 1 || i > m1 + 1 || j < 1 || j > m2 + 1) continue;
     T out = (T)0;
     if (A.bc2h[(int64_t)j * (m1 + 3) + i] <= 0)
-      out = restrict_node(u, A.fb, A.fbc, i, j, A.n1, A.n2, A.aw, A.an, A.ac);
+      out = restrict_node<T>(u, A.fb, A.fbc, i, j, A.n1, A.n2, A.aw, A.an, A.ac);
     A.r2h[(int64_t)(j - 1) * (m1 + 1) + (i - 1)] = out;
   }
 }
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(128) void k_prolong(ProlongArgs<T> A) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (i > m1 + 1) return;
   for (int j = blockIdx.y + 1; j <= m2 + 1; j += gridDim.y)
-    prolong_node(A.frot[s_fb], A.bc, A.n1, (const T*)A.crot[s_cb], m1, i, j);
+    prolong_node<T>(A.frot[s_fb], A.bc, A.n1, (const T*)A.crot[s_cb], m1, i, j);
 }
 
 // FP64 mode, end of a V-cycle on a multi-block finest level: resolve the up
@@ -539,6 +542,8 @@ struct TailArgs {
   int Ls, ng, k;
   T omega;
   double eps;
+  const int8_t* bc_img;    // bc arrays of the LDS-resident levels, in their LDS layout at offset 0
+  int bc_bytes;
   int* visit;
   double* vres;
   TLevel<T> lv[kMaxLevels + 1];
@@ -569,7 +574,7 @@ __device__ void t_smooth(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T 
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double res_s = 0.0, res = 0.0;
   int ksor = 0;
-  constexpr int NB = sizeof(T) == 8 ? 2 : 4;  // rows per wave whose loads are issued together
+  constexpr int NB = sizeof(T) == 8 ? 1 : 2;  // rows per wave whose loads are issued together
   for (int k = 1; k <= nr; ++k) {
     for (int color = 1; color >= 0; --color) {  // red (i+j odd) then black
       for (int j0 = 1 + wv; j0 <= n2 + 1; j0 += NB * nw) {
@@ -770,56 +775,147 @@ __device__ __forceinline__ int fused_items(int n1, int n2, int P) {
   return R <= kFusedMaxR ? R : 0;
 }
 
+// LDS-resident levels run the fused smoother with one or three items per
+// participant (two-item levels take the three-item form with one masked);
+// levels in global memory, or larger than three items, run t_smooth.  Few
+// instantiations keep the kernel's code small: it is fetched anew by the CU
+// that runs each launch.
 template <typename T, bool WAVE, typename PU, typename PB, typename PC>
-__device__ __forceinline__ bool t_smooth_fused_any(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega, double eps,
-                                   int dig, int nr, double* ws, int* ks, double* res) {
-  switch (fused_items(n1, n2, WAVE ? 64 : (int)blockDim.x)) {
-    case 1: t_smooth_fused<T, 1, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
-    case 2: t_smooth_fused<T, 2, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
-    case 3: t_smooth_fused<T, 3, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res); return true;
-    default: return false;
+__device__ __forceinline__ bool t_smooth_fused_any(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega,
+                                                   double eps, int dig, int nr, double* ws, int* ks, double* res) {
+  const int R = fused_items(n1, n2, WAVE ? 64 : (int)blockDim.x);
+  if (R == 1) {
+    t_smooth_fused<T, 1, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res);
+    return true;
   }
+  if (R == 2 || R == 3) {
+    t_smooth_fused<T, 3, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res);
+    return true;
+  }
+  return false;
 }
 
 template <typename T, bool WAVE = false>
-__device__ __forceinline__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps, int dig,
-                             int nr, double* ws, int* ks, double* res) {
-  if (L.lds) {
-    if (t_smooth_fused_any<T, WAVE>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw,
-                                    L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
-      return;
-    if (!WAVE)
+__device__ __forceinline__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps,
+                                             int dig, int nr, double* ws, int* ks, double* res) {
+  if (L.lds && t_smooth_fused_any<T, WAVE>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2,
+                                           L.aw, L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
+    return;
+  if constexpr (!WAVE) {
+    if (L.lds)
       t_smooth<T>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
                   eps, dig, nr, ws, ks, res);
-  } else {
-    if (t_smooth_fused_any<T, WAVE>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw,
-                                    L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
-      return;
-    if (!WAVE)
+    else
       t_smooth<T>((glb_p<T>)u, (glb_p<const T>)b, (glb_p<const int8_t>)bc, L.n1, L.n2, L.aw, L.an, L.ac, omega,
                   eps, dig, nr, ws, ks, res);
   }
 }
 
 // restriction of level l onto l+1 and prolongation of l+1 onto l by the
-// participants (tid, P)
-template <typename T>
-__device__ __forceinline__ void tail_restrict(const TLevel<T>& F, const TLevel<T>& C, T* uf, T* bf, const int8_t* bcf, T* e2h,
-                              T* r2h, const int8_t* bcc, int tid, int P) {
+// participants (tid, P); LDS-resident pairs of levels use LDS-typed pointers
+template <typename T, typename PU, typename PB, typename PC, typename PUC, typename PBC, typename PCC>
+__device__ __forceinline__ void tail_restrict_p(const TLevel<T>& F, const TLevel<T>& C, PU uf, PB bf, PC bcf,
+                                                PUC e2h, PBC r2h, PCC bcc, int tid, int P) {
   for (int q = tid; q < (C.n1 + 3) * (C.n2 + 3); q += P) e2h[q] = (T)0;
   for (int q = tid; q < (C.n1 + 1) * (C.n2 + 1); q += P) {
     const int i = q % (C.n1 + 1) + 1, j = q / (C.n1 + 1) + 1;
     T out = (T)0;
-    if (bcc[j * (C.n1 + 3) + i] <= 0) out = restrict_node(uf, bf, bcf, i, j, F.n1, F.n2, F.aw, F.an, F.ac);
+    if (bcc[j * (C.n1 + 3) + i] <= 0) out = restrict_node<T>(uf, bf, bcf, i, j, F.n1, F.n2, F.aw, F.an, F.ac);
     r2h[q] = out;
   }
 }
 
 template <typename T>
-__device__ __forceinline__ void tail_prolong(const TLevel<T>& F, const TLevel<T>& C, T* uf, const int8_t* bcf, const T* uc,
-                             int tid, int P) {
+__device__ __forceinline__ void tail_restrict(const TLevel<T>& F, const TLevel<T>& C, T* uf, T* bf,
+                                              const int8_t* bcf, T* e2h, T* r2h, const int8_t* bcc, int tid,
+                                              int P) {
+  if (F.lds && C.lds)
+    tail_restrict_p<T>(F, C, (lds_p<const T>)uf, (lds_p<const T>)bf, (lds_p<const int8_t>)bcf, (lds_p<T>)e2h,
+                       (lds_p<T>)r2h, (lds_p<const int8_t>)bcc, tid, P);
+  else
+    tail_restrict_p<T>(F, C, (const T*)uf, (const T*)bf, bcf, e2h, r2h, bcc, tid, P);
+}
+
+template <typename T, typename PU, typename PC, typename PE>
+__device__ __forceinline__ void tail_prolong_p(const TLevel<T>& F, const TLevel<T>& C, PU uf, PC bcf, PE uc,
+                                               int tid, int P) {
   for (int q = tid; q < (C.n1 + 1) * (C.n2 + 1); q += P)
-    prolong_node(uf, bcf, F.n1, uc, C.n1, q % (C.n1 + 1) + 1, q / (C.n1 + 1) + 1);
+    prolong_node<T>(uf, bcf, F.n1, uc, C.n1, q % (C.n1 + 1) + 1, q / (C.n1 + 1) + 1);
+}
+
+template <typename T>
+__device__ __forceinline__ void tail_prolong(const TLevel<T>& F, const TLevel<T>& C, T* uf, const int8_t* bcf,
+                                             const T* uc, int tid, int P) {
+  if (F.lds && C.lds)
+    tail_prolong_p<T>(F, C, (lds_p<T>)uf, (lds_p<const int8_t>)bcf, (lds_p<const T>)uc, tid, P);
+  else
+    tail_prolong_p<T>(F, C, uf, bcf, uc, tid, P);
+}
+
+// Block copies of `bytes` bytes between 16-byte aligned global and LDS
+// buffers: every thread issues its 16-byte loads (up to four) before it
+// stores, so a copy costs one memory latency instead of one per loop trip.
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+template <typename VD, typename VS, typename BD, typename BS>
+__device__ __forceinline__ void copy_block(VD d4, VS s4, BD d1, BS s1, int bytes) {
+  const int nv = bytes >> 4;
+  for (int q0 = threadIdx.x; q0 < nv; q0 += 4 * (int)blockDim.x) {
+    v4i v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * (int)blockDim.x;
+      if (q < nv) v[k] = s4[q];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = q0 + k * (int)blockDim.x;
+      if (q < nv) d4[q] = v[k];
+    }
+  }
+  for (int q = (nv << 4) + (int)threadIdx.x; q < bytes; q += blockDim.x) d1[q] = s1[q];
+}
+
+__device__ __forceinline__ void copy_g2l(char* lds_dst, const void* src, int bytes) {
+  copy_block((lds_p<v4i>)lds_dst, (glb_p<const v4i>)src, (lds_p<char>)lds_dst, (glb_p<const char>)src, bytes);
+}
+
+__device__ __forceinline__ void copy_l2g(void* dst, const char* lds_src, int bytes) {
+  copy_block((glb_p<v4i>)dst, (lds_p<const v4i>)lds_src, (glb_p<char>)dst, (lds_p<const char>)lds_src, bytes);
+}
+
+// three global-to-LDS copies whose loads are all issued before any store
+// (segments of up to 4 x 16 x blockDim bytes take one memory latency)
+__device__ __forceinline__ void copy3_g2l(char* d0, const void* s0, int n0, char* d1, const void* s1, int n1,
+                                          char* d2, const void* s2, int n2) {
+  lds_p<v4i> d[3] = {(lds_p<v4i>)d0, (lds_p<v4i>)d1, (lds_p<v4i>)d2};
+  glb_p<const v4i> sv[3] = {(glb_p<const v4i>)s0, (glb_p<const v4i>)s1, (glb_p<const v4i>)s2};
+  const int nb[3] = {n0, n1, n2};
+  const int P = blockDim.x, t = threadIdx.x;
+  v4i v[3][4];
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = t + k * P;
+      if (q < (nb[g] >> 4)) v[g][k] = sv[g][q];
+    }
+#pragma unroll
+  for (int g = 0; g < 3; ++g)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = t + k * P;
+      if (q < (nb[g] >> 4)) d[g][q] = v[g][k];
+    }
+  // what is left: vectors beyond 4 per thread, then the last bytes
+  char* dc[3] = {d0, d1, d2};
+  const void* sc[3] = {s0, s1, s2};
+#pragma unroll
+  for (int g = 0; g < 3; ++g) {
+    for (int q = t + 4 * P; q < (nb[g] >> 4); q += P) d[g][q] = sv[g][q];
+    for (int q = (nb[g] & ~15) + t; q < nb[g]; q += P)
+      ((lds_p<char>)dc[g])[q] = ((glb_p<const char>)sc[g])[q];
+  }
 }
 
 template <typename T>
@@ -839,22 +935,20 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
     }
   }
   __syncthreads();
-  for (int l = Ls; l <= ng; ++l) {
-    const TLevel<T>& L = A.lv[l];
-    if (!L.lds) continue;
-    int8_t* bc = (int8_t*)(smem + L.obc);
-    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) bc[q] = L.gbc[q];
-    if (l == Ls) {
-      for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) sU[l][q] = L.gu[q];
-      for (int q = threadIdx.x; q < (L.n1 + 1) * (L.n2 + 1); q += blockDim.x) sB[l][q] = L.gb[q];
-    }
+  {
+    // one memory latency for all of it: the bc image of the resident levels
+    // and, when resident, u and b of the first level
+    const TLevel<T>& L = A.lv[Ls];
+    const int ub = L.lds ? (L.n1 + 3) * (L.n2 + 3) * (int)sizeof(T) : 0;
+    const int bb = L.lds ? (L.n1 + 1) * (L.n2 + 1) * (int)sizeof(T) : 0;
+    copy3_g2l(smem, A.bc_img, A.bc_bytes, smem + L.ou, L.gu, ub, smem + L.ob, L.gb, bb);
   }
   __syncthreads();
   int ks;
   double res = 0.0;
   // levels from Lw down are run by wave 0 alone (no block barriers)
   int Lw = ng + 1;
-  for (int l = ng; l >= Ls && (A.lv[l].n1 + 1) * (A.lv[l].n2 + 1) <= kWaveNodes; --l) Lw = l;
+  for (int l = ng; l >= Ls && A.lv[l].lds && (A.lv[l].n1 + 1) * (A.lv[l].n2 + 1) <= kWaveNodes; --l) Lw = l;
   const int tid = threadIdx.x, P = blockDim.x;
   for (int l = Ls; l < ng && l < Lw; ++l) {  // down leg: ig = l
     const TLevel<T>& F = A.lv[l];
@@ -897,8 +991,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
     if (tid == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
   }
   const TLevel<T>& L = A.lv[Ls];
-  if (L.lds)
-    for (int q = threadIdx.x; q < (L.n1 + 3) * (L.n2 + 3); q += blockDim.x) L.gu[q] = sU[Ls][q];
+  if (L.lds) copy_l2g(L.gu, smem + L.ou, (L.n1 + 3) * (L.n2 + 3) * (int)sizeof(T));
 }
 
 // ---------------------------------------------------------------------------
@@ -1004,24 +1097,58 @@ int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
   A.visit = d_visit_of(ctx);
   A.vres = d_vres_of(ctx);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
+  // residency, coarsest first, until the LDS budget is spent; the LDS holds
+  // the bc arrays of the resident levels first, then their u and b
+  int Lr = ctx->ng + 1;   // finest resident level
+  size_t need_all = 0;
+  for (int l = ctx->ng; l >= ctx->Lc; --l) {
+    const Level& L = ctx->lv[l];
+    const size_t need = al((size_t)(L.n1 + 3) * (L.n2 + 3) * sizeof(T)) +
+                        al((size_t)(L.n1 + 1) * (L.n2 + 1) * sizeof(T)) + al((size_t)(L.n1 + 3) * (L.n2 + 3));
+    if (need_all + need > kTailLds) break;
+    need_all += need;
+    Lr = l;
+  }
   size_t off = 0;
-  bool fits = true;
-  for (int l = ctx->ng; l >= ctx->Lc; --l) {  // coarsest first into LDS
+  for (int l = ctx->ng; l >= Lr; --l) {
+    const Level& L = ctx->lv[l];
+    A.lv[l].obc = (uint32_t)off;
+    off = al(off + (size_t)(L.n1 + 3) * (L.n2 + 3));
+  }
+  const size_t bc_bytes = off;
+  for (int l = ctx->ng; l >= ctx->Lc; --l) {
     Level& L = ctx->lv[l];
     TLevel<T>& C = A.lv[l];
     C.n1 = L.n1; C.n2 = L.n2; C.aw = (T)L.aw; C.an = (T)L.an; C.ac = (T)L.ac;
     C.gu = u_of<T>(L, f32); C.gb = b_of<T>(L, f32); C.gbc = L.bc;
-    const size_t need = al((size_t)(L.n1 + 3) * (L.n2 + 3) * sizeof(T)) +
-                        al((size_t)(L.n1 + 1) * (L.n2 + 1) * sizeof(T)) + al((size_t)(L.n1 + 3) * (L.n2 + 3));
-    C.lds = 0;
-    if (fits && off + need <= kTailLds) {
-      C.lds = 1;
+    C.lds = l >= Lr ? 1 : 0;
+    if (C.lds) {
       C.ou = (uint32_t)off; off = al(off + (size_t)(L.n1 + 3) * (L.n2 + 3) * sizeof(T));
       C.ob = (uint32_t)off; off = al(off + (size_t)(L.n1 + 1) * (L.n2 + 1) * sizeof(T));
-      C.obc = (uint32_t)off; off = al(off + (size_t)(L.n1 + 3) * (L.n2 + 3));
-    } else {
-      fits = false;  // finer levels stay in global memory
     }
+  }
+  // the bc image, rebuilt when the boundary or the layout changes
+  const int key = Lr * 2 + (f32 ? 1 : 0);
+  if (ctx->tail_bc_key != key) {
+    if (bc_bytes > ctx->tail_bc_cap) {
+      if (ctx->d_tail_bc) LEPIC_HIP(hipFree(ctx->d_tail_bc));
+      LEPIC_HIP(hipMalloc(&ctx->d_tail_bc, bc_bytes));
+      ctx->tail_bc_cap = bc_bytes;
+    }
+    for (int l = ctx->ng; l >= Lr; --l) {
+      const Level& L = ctx->lv[l];
+      LEPIC_HIP(hipMemcpyAsync(ctx->d_tail_bc + A.lv[l].obc, L.bc, (size_t)(L.n1 + 3) * (L.n2 + 3),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    ctx->tail_bc_key = key;
+  }
+  A.bc_img = ctx->d_tail_bc;
+  A.bc_bytes = (int)bc_bytes;
+  static bool attr_set = false;   // per instantiation; the attribute is per kernel
+  if (!attr_set) {
+    LEPIC_HIP(hipFuncSetAttribute((const void*)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)kTailLds));
+    attr_set = true;
   }
   hipLaunchKernelGGL(k_tail<T>, dim3(1), dim3(kTailThreads), off, ctx->stream, A);
   LEPIC_HIP(hipGetLastError());
@@ -1205,6 +1332,10 @@ void poisson_teardown(lepic_ctx* ctx) {
     if (L.partials) (void)hipFree(L.partials);
   }
   if (ctx->d_erot) (void)hipFree(ctx->d_erot);
+  if (ctx->d_tail_bc) (void)hipFree(ctx->d_tail_bc);
+  ctx->d_tail_bc = nullptr;
+  ctx->tail_bc_cap = 0;
+  ctx->tail_bc_key = -1;
 }
 
 int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_info* info,
