@@ -247,6 +247,39 @@ int lepic_net_loss_grad(lepic_ctx *ctx, const float *x, const float *target, con
                         int32_t H, int32_t W, float lam, float cw, float cn, double *loss, float *grad,
                         int64_t n);
 
+/* ---- averaged moments (part_moments utils.f90:1-136, calc_avg utils.f90:138-265,
+ *      phi/E averages main.f90:933-944) ----
+ * Each call adds, on the grid coarsened by size_na (na = n/size_na), every
+ * species' density, temperature, current densities and sink map to
+ * data_pavg(7,na1+1,na2+1,ntype) = [n, Tp, jx, jy, jz, source, sink], phi
+ * and E to phi_avg(3,0:n1+2,0:n2+2), counts the call (cnt_avg(1)) and
+ * clears the sink maps, as the reference does every navg steps.  Moments
+ * use the particles' half-step-back positions and the density np of the
+ * last deposition.  The source map stays zero (nothing on this path
+ * fills it).  size_na must divide n1 and n2 and stay the same until a reset. */
+int lepic_moments_accumulate(lepic_ctx *ctx, int32_t size_na);
+int lepic_moments_get(lepic_ctx *ctx, double *data_pavg, double *phi_avg, int32_t *count);
+int lepic_moments_reset(lepic_ctx *ctx);
+
+/* ---- particle backup files (main.f90:1102-1132 writer, restart.f90:56-113 reader) ----
+ * The reference's particlesNNN.bak layout: Fortran unformatted sequential
+ * records (4-byte length markers, subrecords with signed markers above
+ * 2^31-9 bytes) holding (time, gtype, Pabs_cor), np_tot(ntype,nproc), then
+ * vxp(6,np_tot) per (iproc, ptype).  The host functions need no device;
+ * vxp[ptype-1 + ntype*iproc] points at 6*np_tot doubles. */
+int lepic_backup_write(const char *path, double time, int32_t gtype, double pabs_cor, int32_t ntype,
+                       int32_t nproc, const int32_t *np_tot, const double *const *vxp);
+int lepic_backup_header(const char *path, int32_t ntype, double *time, int32_t *gtype, double *pabs_cor,
+                        int32_t *nproc, int32_t *np_tot, int32_t cap);
+/* the particles of ptype from every partition of the file, in partition
+ * order (restart with another thread count, restart.f90:66-113) */
+int lepic_backup_read_species(const char *path, int32_t ntype, int32_t ptype, double *vxp, int64_t cap,
+                              int64_t *np);
+/* the context's species as one partition; loading replaces every species'
+ * particles (the caller then redeposits, as restart_OMP does) */
+int lepic_save_particles(lepic_ctx *ctx, const char *path, double time, int32_t gtype, double pabs_cor);
+int lepic_load_particles(lepic_ctx *ctx, const char *path, double *time, int32_t *gtype, double *pabs_cor);
+
 int lepic_step(lepic_ctx *ctx, int32_t it, lepic_step_args *a);
 
 #ifdef __cplusplus

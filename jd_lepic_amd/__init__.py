@@ -22,7 +22,8 @@ EPS0 = 8.854187817e-12   # main.f90:122
 
 __all__ = ["Context", "LepicError", "Species", "FP64", "MIXED", "MCC_CHECK", "MCC_PROD",
            "START_PREV", "START_COLD", "START_GIVEN", "START_NET", "GRID_PHI", "GRID_RHS", "GRID_E", "GRID_NP",
-           "GRID_NPRED", "GRID_SINK", "QE", "AMU", "ME", "EPS0"]
+           "GRID_NPRED", "GRID_SINK", "QE", "AMU", "ME", "EPS0", "backup_write", "backup_header",
+           "backup_read_species"]
 
 
 class LepicError(RuntimeError):
@@ -139,6 +140,33 @@ class Context:
         self._check(self.lib.lepic_download_particles(self._ctx, ptype, _dp(out), out.shape[0],
                                                       C.byref(got)))
         return out[:got.value]
+
+    # ---- averaged moments (part_moments / calc_avg) ----
+    def moments_accumulate(self, size_na=1):
+        self._check(self.lib.lepic_moments_accumulate(self._ctx, size_na))
+        self._size_na = size_na
+
+    def moments(self):
+        """(data_pavg (ntype, na2+1, na1+1, 7), phi_avg (n2+3, n1+3, 3), count)."""
+        na = getattr(self, "_size_na", 1)
+        d = np.zeros((self.nsp, self.n2 // na + 1, self.n1 // na + 1, 7))
+        ph = np.zeros((self.n2 + 3, self.n1 + 3, 3))
+        c = C.c_int32()
+        self._check(self.lib.lepic_moments_get(self._ctx, _dp(d), _dp(ph), C.byref(c)))
+        return d, ph, c.value
+
+    def moments_reset(self):
+        self._check(self.lib.lepic_moments_reset(self._ctx))
+
+    # ---- backup files in the reference's particlesNNN.bak format ----
+    def save_particles(self, path, time=0.0, gtype=0, pabs_cor=1.0):
+        self._check(self.lib.lepic_save_particles(self._ctx, str(path).encode(), time, gtype, pabs_cor))
+
+    def load_particles(self, path):
+        t, g, p = C.c_double(), C.c_int32(), C.c_double()
+        self._check(self.lib.lepic_load_particles(self._ctx, str(path).encode(), C.byref(t), C.byref(g),
+                                                  C.byref(p)))
+        return t.value, g.value, p.value
 
     def set_flag_dead(self, ptype, flags):
         f = np.ascontiguousarray(flags, dtype=np.int8)
@@ -334,3 +362,40 @@ class Context:
         self._check(self.lib.lepic_step(self._ctx, it, C.byref(a)))
         return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res,
                     ms=dict(field=a.ms_rhs_poisson_e, sort=a.ms_sort, mcc=a.ms_mcc, push=a.ms_push))
+
+
+# ---- backup files without a device (restart.f90:56-113, main.f90:1125-1131) ----
+def backup_write(path, time, gtype, pabs_cor, parts):
+    """parts[iproc][ptype-1]: (np, 6) arrays in vxp order."""
+    lib = capi.load()
+    nproc, ntype = len(parts), len(parts[0])
+    arrs = [np.ascontiguousarray(parts[ip][p], dtype=np.float64).reshape(-1, 6)
+            for ip in range(nproc) for p in range(ntype)]
+    np_tot = np.array([a.shape[0] for a in arrs], np.int32)
+    ptrs = (C.POINTER(C.c_double) * len(arrs))(*[_dp(a) for a in arrs])
+    rc = lib.lepic_backup_write(str(path).encode(), time, gtype, pabs_cor, ntype, nproc, _ip(np_tot), ptrs)
+    if rc:
+        raise LepicError(rc, f"cannot write {path}")
+
+
+def backup_header(path, ntype):
+    lib = capi.load()
+    t, g, pc, nproc = C.c_double(), C.c_int32(), C.c_double(), C.c_int32()
+    cnt = np.zeros(4096, np.int32)
+    rc = lib.lepic_backup_header(str(path).encode(), ntype, C.byref(t), C.byref(g), C.byref(pc), C.byref(nproc),
+                                 _ip(cnt), cnt.size)
+    if rc:
+        raise LepicError(rc, f"cannot read {path}")
+    return t.value, g.value, pc.value, cnt[:ntype * nproc.value].reshape(nproc.value, ntype)
+
+
+def backup_read_species(path, ntype, ptype):
+    lib = capi.load()
+    counts = backup_header(path, ntype)[3]
+    total = int(counts[:, ptype - 1].sum())
+    out = np.zeros((max(total, 1), 6))
+    n = C.c_int64()
+    rc = lib.lepic_backup_read_species(str(path).encode(), ntype, ptype, _dp(out), out.shape[0], C.byref(n))
+    if rc:
+        raise LepicError(rc, f"cannot read {path}")
+    return out[:n.value]

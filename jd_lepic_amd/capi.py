@@ -81,6 +81,16 @@ SIGNATURES = {
     "lepic_net_recording": (C.c_int, [_ctxp, C.c_int32]),
     "lepic_net_samples": (C.c_int, [_ctxp, _ip]),
     "lepic_net_policy": (C.c_int, [_ctxp, C.c_float, C.c_int32]),
+    "lepic_backup_write": (C.c_int, [C.c_char_p, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32, _ip,
+                                     C.POINTER(_dp)]),
+    "lepic_backup_header": (C.c_int, [C.c_char_p, C.c_int32, _dp, _ip, _dp, _ip, _ip, C.c_int32]),
+    "lepic_backup_read_species": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _dp, C.c_int64,
+                                            C.POINTER(C.c_int64)]),
+    "lepic_moments_accumulate": (C.c_int, [_ctxp, C.c_int32]),
+    "lepic_moments_get": (C.c_int, [_ctxp, _dp, _dp, _ip]),
+    "lepic_moments_reset": (C.c_int, [_ctxp]),
+    "lepic_save_particles": (C.c_int, [_ctxp, C.c_char_p, C.c_double, C.c_int32, C.c_double]),
+    "lepic_load_particles": (C.c_int, [_ctxp, C.c_char_p, _dp, _ip, _dp]),
     "lepic_net_train": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float),
                                   C.POINTER(C.c_float)]),
     "lepic_net_set_params": (C.c_int, [_ctxp, C.POINTER(C.c_float), C.c_int64]),

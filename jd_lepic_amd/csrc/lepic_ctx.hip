@@ -387,6 +387,7 @@ int lepic_destroy(lepic_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   poisson_teardown(ctx);
   net_destroy(ctx);
+  lepic_moments_reset(ctx);
   for (int p = 0; p < kMaxSpecies; ++p)
     for (int k = 0; k < 2; ++k)
       if (ctx->ev_mover[p][k]) (void)hipEventDestroy(ctx->ev_mover[p][k]);

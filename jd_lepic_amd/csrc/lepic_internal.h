@@ -111,6 +111,17 @@ struct Reactions {
 
 }  // namespace lepic
 
+// Averaged moments (calc_avg, utils.f90:138-265): data_pavg(7,na1+1,na2+1,
+// ntype), phi_avg(3,0:n1+2,0:n2+2), the number of accumulations, and the
+// per-species part_moments scratch p_mts(4,na1+1,na2+1).
+struct Moments {
+  int size_na = 0;
+  int count = 0;
+  double* data = nullptr;
+  double* phi = nullptr;
+  double* pm = nullptr;
+};
+
 struct lepic_ctx {
   lepic_grid_desc g{};
   int device = 0;
@@ -185,6 +196,7 @@ struct lepic_ctx {
 
   // learned warm start (unet.hip)
   lepic::Net* net = nullptr;
+  Moments mom;
 
   int fail(int code, const std::string& msg) {
     err = msg;

@@ -174,6 +174,34 @@ module lepic_capi
        integer(c_int32_t), intent(inout) :: iseed
        type(lepic_mcc_info), intent(out) :: info
      end function lepic_collisions
+     ! averaged moments (utils.f90:1-265) and backup files (main.f90:1102-1132, restart.f90)
+     integer(c_int) function lepic_moments_accumulate(ctx, size_na) bind(C, name='lepic_moments_accumulate')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: size_na
+     end function lepic_moments_accumulate
+     integer(c_int) function lepic_moments_get(ctx, data_pavg, phi_avg, cnt) bind(C, name='lepic_moments_get')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: data_pavg(*), phi_avg(*)
+       integer(c_int32_t), intent(out) :: cnt
+     end function lepic_moments_get
+     integer(c_int) function lepic_save_particles(ctx, path, time, gtype, pabs_cor) &
+          bind(C, name='lepic_save_particles')
+       import :: c_int, c_ptr, c_char, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_double), value :: time, pabs_cor
+       integer(c_int32_t), value :: gtype
+     end function lepic_save_particles
+     integer(c_int) function lepic_load_particles(ctx, path, time, gtype, pabs_cor) &
+          bind(C, name='lepic_load_particles')
+       import :: c_int, c_ptr, c_char, c_int32_t, c_double
+       type(c_ptr), value :: ctx
+       character(kind=c_char), intent(in) :: path(*)
+       real(c_double), intent(out) :: time, pabs_cor
+       integer(c_int32_t), intent(out) :: gtype
+     end function lepic_load_particles
      ! learned warm start (no counterpart in the reference)
      integer(c_int) function lepic_net_create(ctx, capacity, seed) bind(C, name='lepic_net_create')
        import :: c_int, c_ptr, c_int32_t, c_int64_t

@@ -5,19 +5,24 @@
 ! cross sections), then runs the reference's per-step order (main.f90:689-
 ! 1090): rhs -> MG Poisson -> E -> sort every nsort steps -> density
 ! reduction + MCC every ns_coll steps -> push + deposition of each species,
-! each phase through the C ABI.  Prints the phase timers in the style of the
-! reference's "<t> (ms)" lines (main.f90:1318-1322) and the particle counts.
+! each phase through the C ABI; averaged moments every navg steps on the grid
+! coarsened by size_na (main.f90:903-944), particle backups every nbak steps
+! into DATA.BAK/particles0.bak (main.f90:1102-1132; the directory must
+! exist, as for the reference), and restart = 1 starts
+! from that file (restart.f90:56-64) instead of a uniform load.  Prints the
+! phase timers in the style of the reference's "<t> (ms)" lines
+! (main.f90:1318-1322), the particle counts and the averaged Te at the centre.
 program lepic_driver
   use iso_c_binding
   use lepic_capi
   implicit none
   ! ---- namelist ----
   integer :: n = 128, ppc = 20, nsteps = 20, ng = 7, precision = 0, nsort = 10, ns_coll = 10
-  integer :: mcc_mode = 1
+  integer :: mcc_mode = 1, navg = 0, size_na = 1, nbak = 0, restart = 0
   real(8) :: dx = 5.d-5, dt = 5.d-12, Te = 3.d0, Ti = 0.03d0, Tgas = 0.02585d0, n0 = 1.d16
   real(8) :: B0 = 0.01d0, pgas = 0.3d0, Tgas_K = 300.d0, eps = 1.d-10
   namelist /lepic/ n, ppc, nsteps, ng, precision, nsort, ns_coll, mcc_mode, dx, dt, Te, Ti, Tgas, &
-       n0, B0, pgas, Tgas_K, eps
+       n0, B0, pgas, Tgas_K, eps, navg, size_na, nbak, restart
   ! ---- constants (main.f90:120-124) ----
   real(8) :: qe, amu, eps0, pi, me, mAr, kB
   ! ---- state ----
@@ -38,6 +43,10 @@ program lepic_driver
   integer :: it, i, ix, u, ios, npt, c, rc, vcyc
   integer(c_int32_t) :: iseed
   integer(8) :: c0, c1, crate
+  real(8), allocatable :: pavg(:,:,:,:), phiavg(:,:,:)
+  real(8) :: time, pabs_cor
+  integer(c_int32_t) :: gtype, cnt_avg
+  character(len=*), parameter :: bakfile = 'DATA.BAK/particles0.bak'
 
   open(newunit=u, file='lepic.nml', status='old', iostat=ios)
   if (ios == 0) then
@@ -77,8 +86,13 @@ program lepic_driver
   end do
   Bi(4, :, :) = dabs(Bi(3, :, :))
   call lepic_check(ctx, lepic_set_bfield(ctx, n, n, dx, dx, Bi), 'set_bfield')
-  call lepic_check(ctx, lepic_load_uniform(ctx, 1, ppc, vte, 0.d0, L, 20261202_8), 'load e')
-  call lepic_check(ctx, lepic_load_uniform(ctx, 2, ppc, vti, 0.d0, L, 20261202_8), 'load Ar+')
+  time = 0.d0; gtype = 0; pabs_cor = 1.d0
+  if (restart == 1) then
+     call lepic_check(ctx, lepic_load_particles(ctx, bakfile//c_null_char, time, gtype, pabs_cor), 'restart')
+  else
+     call lepic_check(ctx, lepic_load_uniform(ctx, 1, ppc, vte, 0.d0, L, 20261202_8), 'load e')
+     call lepic_check(ctx, lepic_load_uniform(ctx, 2, ppc, vti, 0.d0, L, 20261202_8), 'load Ar+')
+  end if
 
   ! analytic Ar table on one energy grid (union grid = the grid itself)
   npt = 242
@@ -159,6 +173,9 @@ program lepic_driver
         call lepic_check(ctx, lepic_sync(ctx), 'sync')
      end if
      call system_clock(c1); t_sort = t_sort + real(c1 - c0, 8)/crate; c0 = c1
+     if (navg > 0) then
+        if (mod(it, navg) == 1) call lepic_check(ctx, lepic_moments_accumulate(ctx, size_na), 'moments')
+     end if
      if (it > 1 .and. mod(it, ns_coll) == 1) then
         call lepic_check(ctx, lepic_dens_red(ctx, np_mx_c), 'dens_red')
         call lepic_check(ctx, lepic_collisions(ctx, it, mcc_mode, 7_8, iseed, minfo), 'collisions')
@@ -169,6 +186,13 @@ program lepic_driver
      end do
      call lepic_check(ctx, lepic_sync(ctx), 'sync')
      call system_clock(c1); t_move = t_move + real(c1 - c0, 8)/crate
+     time = time + dt
+     if (nbak > 0 .and. it > 1) then
+        if (mod(it, nbak) == 1) then
+           call lepic_check(ctx, lepic_save_particles(ctx, bakfile//c_null_char, time, gtype, pabs_cor), 'backup')
+           print '(a,i8)', ' backup written at step ', it
+        end if
+     end if
   end do
   call lepic_check(ctx, lepic_get_count(ctx, 1, ne), 'count')
   call lepic_check(ctx, lepic_get_count(ctx, 2, ni), 'count')
@@ -179,6 +203,12 @@ program lepic_driver
        1.d3*t_move/nsteps
   call lepic_check(ctx, lepic_get_grid(ctx, LEPIC_GRID_PHI, 0, phi), 'get_grid')
   print '(a,es14.6)', ' phi(centre) (V): ', phi(n/2 + 1, n/2 + 1)
+  if (navg > 0) then
+     allocate(pavg(7, n/size_na + 1, n/size_na + 1, 2), phiavg(3, 0:n+2, 0:n+2))
+     call lepic_check(ctx, lepic_moments_get(ctx, pavg, phiavg, cnt_avg), 'moments_get')
+     if (cnt_avg > 0) print '(a,i6,a,es14.6)', ' averages: ', cnt_avg, '  Te(centre) (eV): ', &
+          pavg(2, n/(2*size_na) + 1, n/(2*size_na) + 1, 1)/cnt_avg
+  end if
   rc = lepic_destroy(ctx)
 
 contains

@@ -220,6 +220,77 @@ void lo_charge_deposition(const lo_domain *d, const lo_species *sp,
   }
 }
 
+/* part_moments, utils.f90:1-136, one species and partition, no Neumann or
+ * periodic walls (flag_nmn = flag_pbc = 0), 2D.  p_mts(4,na1+1,na2+1) with
+ * index 0 = sum k v^2, 1..3 = sum k v; the coarse node (ix,iy) is
+ * p_mts[4*((iy-1)*(na1+1)+(ix-1)) + q].  Writes that would land outside
+ * p_mts (a particle exactly on xmax or ymax) are dropped. */
+void lo_part_moments(const lo_domain *d, const lo_species *sp, const double *vxp,
+                     long long np_tot, const double *kq, int size_na, double *p_mts) {
+  const int n1 = d->n1;
+  const int na1 = d->n1 / size_na, na2 = d->n2 / size_na;
+  const double ha1 = size_na * d->h1, ha2 = size_na * d->h2;
+  const double k1 = sp->Nm / (ha1 * ha2 * d->zmax);
+  for (long i = 0; i < (long)(na1 + 1) * (na2 + 1) * 4; ++i) p_mts[i] = 0.0;
+  for (long long i = 0; i < np_tot; ++i) {
+    const double vx = vxp[6 * i + 3], vy = vxp[6 * i + 4], vz = vxp[6 * i + 5];
+    const double xp = vxp[6 * i + 0] - vx * d->dt / 2.0;
+    const double yp = vxp[6 * i + 1] - vy * d->dt / 2.0;
+    if (xp > d->xmax || xp < 0.0 || yp > d->ymax || yp < 0.0) continue;
+    const double Eki = vx * vx + vy * vy + vz * vz;
+    const int ix = (int)(xp / ha1) + 1;
+    const double px = (ix * ha1 - xp) / ha1;
+    const int iy = (int)(yp / ha2) + 1;
+    const double py = (iy * ha2 - yp) / ha2;
+    const double ki[4] = {k1 * px * py, k1 * (1.0 - px) * py, k1 * (1.0 - px) * (1.0 - py), k1 * px * (1.0 - py)};
+    const int ix_f = size_na * ix - (size_na - 1), iy_f = size_na * iy - (size_na - 1);
+    const int cx[4] = {ix, ix + 1, ix + 1, ix}, cy[4] = {iy, iy, iy + 1, iy + 1};
+    const int fx[4] = {ix_f, ix_f + 1, ix_f + 1, ix_f}, fy[4] = {iy_f, iy_f, iy_f + 1, iy_f + 1};
+    for (int c = 0; c < 4; ++c) {
+      if (cx[c] > na1 + 1 || cy[c] > na2 + 1) continue;
+      const double k_tmp = kq[NODE(fx[c], fy[c], n1)] * ki[c];
+      double *m = p_mts + 4 * ((long)(cy[c] - 1) * (na1 + 1) + (cx[c] - 1));
+      m[1] = m[1] + k_tmp * vx;
+      m[2] = m[2] + k_tmp * vy;
+      m[3] = m[3] + k_tmp * vz;
+      m[0] = m[0] + k_tmp * Eki;
+    }
+  }
+}
+
+/* calc_avg, utils.f90:138-265, one partition and rank, plt_src = 0.
+ * np: node arrays per species; sink: ss2D(2,...) node arrays per species
+ * (ss2D(1,...), the source map, is zero: nothing on this path fills it);
+ * data_pavg(7,na1+1,na2+1,ntype) accumulates:
+ * [(p*(na2+1) + iy-1)*(na1+1) + ix-1]*7 + {0 n, 1 Tp, 2..4 j, 5 source, 6 sink}. */
+void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const double *const *np,
+                 const double *const *p_mts, const double *const *sink, int size_na, double *data_pavg) {
+  const int n1 = d->n1;
+  const int na1 = d->n1 / size_na, na2 = d->n2 / size_na;
+  const double Vc = d->h1 * d->h2 * d->zmax;
+  for (int p = 0; p < ntype; ++p) {
+    const double k = sp[p].Nm / Vc;
+    for (int iy = 1; iy <= na2 + 1; ++iy)
+      for (int ix = 1; ix <= na1 + 1; ++ix) {
+        const int ix_f = size_na * ix - (size_na - 1), iy_f = size_na * iy - (size_na - 1);
+        const double *m = p_mts[p] + 4 * ((long)(iy - 1) * (na1 + 1) + (ix - 1));
+        const double np_tmp = np[p][NODE(ix_f, iy_f, n1)];
+        const double jx = m[1], jy = m[2], jz = m[3];
+        double *o = data_pavg + 7 * (((long)p * (na2 + 1) + (iy - 1)) * (na1 + 1) + (ix - 1));
+        o[0] = o[0] + np_tmp;
+        o[2] = o[2] + jx;
+        o[3] = o[3] + jy;
+        o[4] = o[4] + jz;
+        double Tp = 0.0;
+        if (np_tmp > 0)
+          Tp = sp[p].mass / d->qe * (1.0 / 3.0) * (m[0] / np_tmp - (jx * jx + jy * jy + jz * jz) / (np_tmp * np_tmp));
+        if (Tp > 0) o[1] = o[1] + Tp;
+        /* o[5], the source map, gets nothing here */
+        o[6] = o[6] + (sink ? sink[p][NODE(ix_f, iy_f, n1)] * k : 0.0);
+      }
+  }
+}
+
 /* calc_rho.f90:37-77, one partition (nproc=1, nproc_mpi=1). */
 void lo_calc_rhs(int n1, int n2, int ntype, const double *charge,
                  const double *const *np, double *rhs) {

@@ -20,6 +20,11 @@
  *   lo_calc_efield     oracle/_ref calc_Efield (bit-exact); quadratic phi.
  *   lo_part_sorting    oracle/_ref part_sorting (bit-exact); stable sort.
  *   lo_calc_rhs, lo_dens_red   closed forms (sums / maxima).
+ *   lo_part_moments, lo_calc_avg   closed forms: single-particle weights,
+ *                      cold beams (Tp = 0, j/n = beam velocity), Maxwellian
+ *                      temperature within its sampling error, drift
+ *                      invariance of Tp (utils.f90 includes mpif.h, so the
+ *                      reference file cannot be compiled here).
  *   lo_sor_rb, lo_mg, lo_iterative_solver, lo_restriction, lo_prolongation:
  *                      converged phi pinned by a direct sparse solve; single
  *                      red/black sweep pinned by an independent Gauss-Seidel
@@ -98,6 +103,10 @@ void lo_charge_deposition(const lo_domain *d, const lo_species *sp,
 void lo_kq(const lo_domain *d, double *kq);
 void lo_calc_rhs(int n1, int n2, int ntype, const double *charge,
                  const double *const *np, double *rhs);
+void lo_part_moments(const lo_domain *d, const lo_species *sp, const double *vxp,
+                     long long np_tot, const double *kq, int size_na, double *p_mts);
+void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const double *const *np,
+                 const double *const *p_mts, const double *const *sink, int size_na, double *data_pavg);
 void lo_dens_red(int n1, int n2, int ntype, const int32_t *bcnd,
                  const double *const *np, double *const *np_red, double *np_mx);
 void lo_calc_efield(int n1, int n2, const double h[2], const double *phi,

@@ -116,6 +116,9 @@ def _declare(L):
     L.lo_ran2.restype = C.c_double
     L.lo_ran2.argtypes = [_ip]
     L.lo_load_gauss.argtypes = [_dp, _dp, C.c_double, _dp, C.c_double]
+    L.lo_part_moments.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, C.c_longlong, _dp, C.c_int, _dp]
+    L.lo_calc_avg.argtypes = [C.POINTER(LoDomain), C.c_int, C.POINTER(LoSpecies), C.POINTER(_dp), C.POINTER(_dp),
+                              C.POINTER(_dp), C.c_int, _dp]
     L.lo_part_mover.restype = C.c_int
     L.lo_part_mover.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, _dp, _bp,
                                 C.POINTER(C.c_longlong), C.POINTER(LoMoverTally), _dp, _dp]
@@ -241,6 +244,34 @@ def charge_deposition(dom: Domain, Nm, vxp, kq_arr=None):
     d = dom.struct()
     sp = LoSpecies(0.0, 1.0, Nm, 0)
     lib().lo_charge_deposition(C.byref(d), C.byref(sp), _p(vxp), vxp.shape[0], _p(k), _p(out))
+    return out
+
+
+def part_moments(dom: Domain, Nm, vxp, size_na=1, kq_arr=None):
+    """utils.f90:1-136 (lo_part_moments): p_mts as (na2+1, na1+1, 4) with
+    [..., 0] = sum k v^2 and [..., 1:4] = sum k v."""
+    vxp = np.ascontiguousarray(vxp, dtype=np.float64).reshape(-1, 6)
+    k = kq(dom) if kq_arr is None else np.ascontiguousarray(kq_arr, dtype=np.float64)
+    na1, na2 = dom.n1 // size_na, dom.n2 // size_na
+    out = np.zeros((na2 + 1, na1 + 1, 4))
+    d = dom.struct()
+    sp = LoSpecies(0.0, 1.0, Nm, 0)
+    lib().lo_part_moments(C.byref(d), C.byref(sp), _p(vxp), vxp.shape[0], _p(k), size_na, _p(out))
+    return out
+
+
+def calc_avg(dom: Domain, masses, Nms, nps, p_mts, sinks, size_na=1, data_pavg=None):
+    """utils.f90:138-265 (lo_calc_avg): adds into data_pavg, shaped
+    (ntype, na2+1, na1+1, 7) = [n, Tp, jx, jy, jz, source, sink]."""
+    ntype = len(nps)
+    na1, na2 = dom.n1 // size_na, dom.n2 // size_na
+    out = np.zeros((ntype, na2 + 1, na1 + 1, 7)) if data_pavg is None else data_pavg
+    sps = (LoSpecies * ntype)(*[LoSpecies(0.0, m, w, 0) for m, w in zip(masses, Nms)])
+    arr = lambda xs: [np.ascontiguousarray(a, dtype=np.float64) for a in xs]
+    a_np, a_m, a_s = arr(nps), arr(p_mts), arr(sinks)
+    ptr = lambda xs: (_dp * ntype)(*[_p(a) for a in xs])
+    d = dom.struct()
+    lib().lo_calc_avg(C.byref(d), ntype, sps, ptr(a_np), ptr(a_m), ptr(a_s), size_na, _p(out))
     return out
 
 

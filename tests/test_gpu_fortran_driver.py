@@ -4,6 +4,7 @@ conserve the particle count up to wall losses and ionization, and print the
 reference-style phase timers."""
 import os
 import subprocess
+import sys
 
 import pytest
 
@@ -23,3 +24,26 @@ def test_fortran_driver_runs(tmp_path):
     loaded = 64 * 64 * 20
     assert 0.9 * loaded < ne < 1.1 * loaded and 0.9 * loaded < ni < 1.1 * loaded
     assert any("<t> (ms):" in l for l in lines)
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+def test_fortran_driver_backup_restart_and_moments(tmp_path):
+    """navg/nbak/restart: the run averages moments, writes a backup in the
+    reference's format at step 7 and a second run restarts from it."""
+    (tmp_path / "DATA.BAK").mkdir()
+    (tmp_path / "lepic.nml").write_text("&lepic n=64, ppc=10, nsteps=8, ng=6, precision=1, navg=2, nbak=6 /\n")
+    out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "backup written at step        7" in out.stdout
+    te = [l for l in out.stdout.splitlines() if "Te(centre)" in l][0].split()
+    assert int(te[1]) == 4 and 1.0 < float(te[-1]) < 6.0      # 3 eV electrons, 4 averages
+    bak = tmp_path / "DATA.BAK" / "particles0.bak"
+    sys.path.insert(0, ROOT)
+    import jd_lepic_amd as L
+    t, g, pc, counts = L.backup_header(bak, 2)
+    assert counts.shape == (1, 2) and 0.9 * 64 * 64 * 10 < counts[0, 0] < 1.1 * 64 * 64 * 10
+    (tmp_path / "lepic.nml").write_text("&lepic n=64, ppc=10, nsteps=2, ng=6, precision=1, restart=1 /\n")
+    out2 = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out2.returncode == 0, out2.stdout + out2.stderr
+    counts2 = [l for l in out2.stdout.splitlines() if "particles:" in l][0].split()
+    assert abs(int(counts2[2]) - counts[0, 0]) < 0.01 * counts[0, 0]
