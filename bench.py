@@ -38,9 +38,11 @@ import jd_lepic_amd as L  # noqa: E402
 from jd_lepic_amd import reactions as RX, tables  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per mover launch measured with rocprofv3 PMC counters (FETCH_SIZE
-# and WRITE_SIZE, scaled by a plain copy's reading); None until measured
-TRAFFIC = None
+# HBM bytes per particle of one mixed mover launch at this workload, measured
+# with rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE, each scaled by what
+# it reads for a 1 GiB device copy): 51.4 B against 48 algorithmic bytes
+# (profiles/mover_mixed_traffic_512x512_100ppc.md)
+TRAFFIC_BYTES_PER_PARTICLE = 51.4
 
 
 class Hip:
@@ -70,6 +72,19 @@ class Hip:
 
     def sync(self):
         assert self.lib.hipDeviceSynchronize() == 0
+
+    def copy_once(self, nbytes):
+        """One device-to-device copy of nbytes (hipMemcpy's copy kernel)."""
+        a, b = C.c_void_p(), C.c_void_p()
+        self.lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.lib.hipFree.argtypes = [C.c_void_p]
+        assert self.lib.hipMalloc(C.byref(a), nbytes) == 0 and self.lib.hipMalloc(C.byref(b), nbytes) == 0
+        assert self.lib.hipMemcpy(b, a, nbytes, 3) == 0          # hipMemcpyDeviceToDevice
+        self.sync()
+        self.lib.hipFree(a)
+        self.lib.hipFree(b)
+        return nbytes
 
 
 def workload(n=512, ppc=100):
@@ -244,6 +259,9 @@ def main():
     ap.add_argument("--train-iters", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="after the runs, one device-to-device copy of GIB GiB (a plain copy kernel "
+                         "whose HBM counters calibrate those of the mover under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
@@ -311,7 +329,8 @@ def main():
                                   else "mixed precision"),
                    "global_batch": N, "seq_len": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_PARTICLE * (cnt_h[0] + cnt_h[1]) / 2,
                      "kernel": "k_mover_mixed<256> (electron and ion launches)",
                      "alg_bytes_per_launch": (alg_e + alg_i) / 2,
                      "avg_launch_ms": float((ke.mean() + ki.mean()) / 2),
@@ -326,6 +345,8 @@ def main():
         out["fp64"]["roofline_push_GBs"] = cnt_f[0] * 96.0 / (kf.mean() * 1e-3) / 1e9
         out["speedup_vs_fp64_step"] = s_f["ms_per_step"] / s_h["ms_per_step"]
         out["speedup_vs_fp64_pdm"] = s_h["particles_per_s"] / s_f["particles_per_s"]
+    if args.copy_calibration > 0:
+        out["copy_calibration_bytes"] = hip.copy_once(int(args.copy_calibration * (1 << 30)))
     print(json.dumps(out))
 
 
