@@ -1,6 +1,10 @@
 // sort.hip -- cell sort of one species (sorting.f90:92-251): a stable LSD
-// radix sort of the y-major cell index with 8-bit digits, then a gather of
-// the particle payload into the other buffer, and the cell-end list Plist.
+// radix sort of the y-major cell index in two passes (digits of up to 11
+// bits, 2048 buckets), then a gather of the particle payload into the other
+// buffer, and the cell-end list Plist.  The sort reads the species as the
+// mover leaves it, segment by segment: slots past a segment's count carry a
+// key above every cell and sort behind the particles, so the output is
+// packed without a separate packing pass.
 //
 // With one thread the reference's counting sort is stable (particles keep
 // their order inside a cell); LSD radix sort with stable per-digit scatter
@@ -20,6 +24,7 @@ namespace {
 
 constexpr int kSortItems = 16;
 constexpr int kTile = 256 * kSortItems;
+constexpr int kMaxRadix = 2048;   // two passes cover cell indices below 2^22 (2048 x 2048 cells)
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
@@ -40,21 +45,11 @@ __device__ __forceinline__ uint32_t wave_match(bool valid, uint32_t d, uint32_t*
   return rank;
 }
 
-// keys (0-based cell index) and per-cell counts
-__global__ void k_keys64(const double* __restrict__ x, const double* __restrict__ y, const int64_t* nptr,
-                         double h1, double h2, int n1, int n2, uint32_t* __restrict__ key,
-                         uint32_t* __restrict__ cnt) {
-  const int64_t N = *nptr;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < N;
-  uint32_t k = 0;
-  if (valid) {
-    int ix = (int)(x[i] / h1), iy = (int)(y[i] / h2);  // INT(x/h) (sorting.f90:147-151)
-    ix = ix < 0 ? 0 : (ix > n1 - 1 ? n1 - 1 : ix);
-    iy = iy < 0 ? 0 : (iy > n2 - 1 ? n2 - 1 : iy);
-    k = (uint32_t)ix + (uint32_t)n1 * (uint32_t)iy;
-    key[i] = k;
-  }
+// keys (0-based cell index, kEmpty for an empty slot) and per-cell counts;
+// slot i is a particle when i % kChunk < seg[i / kChunk]
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ void count_keys(bool valid, uint32_t k, uint32_t* cnt) {
   // aggregate equal keys of the wave, one atomic per distinct key
   const int lane = threadIdx.x & 63;
   uint64_t remaining = __ballot(valid);
@@ -67,69 +62,78 @@ __global__ void k_keys64(const double* __restrict__ x, const double* __restrict_
   }
 }
 
-__global__ void k_keys_mixed(const int32_t* __restrict__ cell, const int64_t* nptr, int n1, int n2,
-                             uint32_t* __restrict__ key, uint32_t* __restrict__ cnt) {
-  const int64_t N = *nptr;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < N;
-  uint32_t k = 0;
+__global__ void k_keys64(const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ seg,
+                         int64_t nslots, double h1, double h2, int n1, int n2, uint32_t* __restrict__ key,
+                         uint32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < nslots && (int)(i & (kChunk - 1)) < seg[i / kChunk];
+  uint32_t k = kEmpty;
+  if (valid) {
+    int ix = (int)(x[i] / h1), iy = (int)(y[i] / h2);  // INT(x/h) (sorting.f90:147-151)
+    ix = ix < 0 ? 0 : (ix > n1 - 1 ? n1 - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > n2 - 1 ? n2 - 1 : iy);
+    k = (uint32_t)ix + (uint32_t)n1 * (uint32_t)iy;
+  }
+  if (i < nslots) key[i] = k;
+  count_keys(valid, k, cnt);
+}
+
+__global__ void k_keys_mixed(const int32_t* __restrict__ cell, const int32_t* __restrict__ seg, int64_t nslots,
+                             int n1, int n2, uint32_t* __restrict__ key, uint32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < nslots && (int)(i & (kChunk - 1)) < seg[i / kChunk];
+  uint32_t k = kEmpty;
   if (valid) {
     int c = cell[i];
     int ix = (c & 0xFFFF) - 1, iy = (c >> 16) - 1;
     ix = ix < 0 ? 0 : (ix > n1 - 1 ? n1 - 1 : ix);
     iy = iy < 0 ? 0 : (iy > n2 - 1 ? n2 - 1 : iy);
     k = (uint32_t)ix + (uint32_t)n1 * (uint32_t)iy;
-    key[i] = k;
   }
-  const int lane = threadIdx.x & 63;
-  uint64_t remaining = __ballot(valid);
-  while (remaining) {
-    const int leader = __builtin_ctzll(remaining);
-    const uint32_t kl = __shfl(k, leader, 64);
-    const uint64_t mask = __ballot(valid && k == kl) & remaining;
-    if (lane == leader) atomicAdd(&cnt[kl], (uint32_t)__popcll(mask));
-    remaining &= ~mask;
-  }
+  if (i < nslots) key[i] = k;
+  count_keys(valid, k, cnt);
 }
 
-__global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, const int64_t* nptr, int shift,
+template <int R>   // buckets per pass
+__global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, int64_t nslots, int shift,
                                               uint32_t* __restrict__ hist, int nb) {
-  __shared__ uint32_t cnt[256];
-  const int64_t N = *nptr;
-  cnt[threadIdx.x] = 0;
+  __shared__ uint32_t cnt[R];
+  for (int q = threadIdx.x; q < R; q += 256) cnt[q] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile;
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t i = base + (int64_t)j * 256 + threadIdx.x;
-    const bool valid = i < N;
-    const uint32_t d = valid ? (key[i] >> shift) & 255u : 0u;
+    const bool valid = i < nslots;
+    const uint32_t d = valid ? (key[i] >> shift) & (uint32_t)(R - 1) : 0u;
     // waves add to the shared table one at a time per round (no LDS atomics)
     for (int w = 0; w < 4; ++w) {
       if ((threadIdx.x >> 6) == w) wave_match(valid, d, cnt);
       __syncthreads();
     }
   }
-  hist[(int64_t)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+  for (int q = threadIdx.x; q < R; q += 256) hist[(int64_t)q * nb + blockIdx.x] = cnt[q];
 }
 
+template <int R>
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                 const uint32_t* __restrict__ hist, const int64_t* nptr,
-                                                 int shift, int nb) {
-  __shared__ uint32_t run[256];
-  __shared__ uint32_t wc[4][256];
-  const int64_t N = *nptr;
+                                                 const uint32_t* __restrict__ hist, int64_t nslots, int shift,
+                                                 int nb) {
+  __shared__ uint32_t run[R];
+  __shared__ uint32_t wc[4][R];
   const int t = threadIdx.x, w = t >> 6;
-  run[t] = hist[(int64_t)t * nb + blockIdx.x];
-  for (int v = 0; v < 4; ++v) wc[v][t] = 0;
+  for (int q = t; q < R; q += 256) {
+    run[q] = hist[(int64_t)q * nb + blockIdx.x];
+    wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
+  }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile;
   for (int j = 0; j < kSortItems; ++j) {
     const int64_t i = base + (int64_t)j * 256 + t;
-    const bool valid = i < N;
+    const bool valid = i < nslots;
     const uint32_t k = valid ? kin[i] : 0u;
     const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
-    const uint32_t d = (k >> shift) & 255u;
+    const uint32_t d = (k >> shift) & (uint32_t)(R - 1);
     const uint32_t r = wave_match(valid, d, wc[w]);
     __syncthreads();
     if (valid) {
@@ -139,8 +143,10 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ki
       vout[pos] = v;
     }
     __syncthreads();
-    run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
-    wc[0][t] = wc[1][t] = wc[2][t] = wc[3][t] = 0;
+    for (int q = t; q < R; q += 256) {
+      run[q] += wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
+      wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
+    }
     __syncthreads();
   }
 }
@@ -181,10 +187,10 @@ int ensure_sort_scratch(lepic_ctx* ctx, int64_t n, int64_t nb, int64_t ncnt) {
     }
     ctx->s_cap = n;
   }
-  if (256 * nb > ctx->s_hist_cap) {
+  if (kMaxRadix * nb > ctx->s_hist_cap) {
     if (ctx->s_hist) LEPIC_HIP(hipFree(ctx->s_hist));
-    LEPIC_HIP(hipMalloc(&ctx->s_hist, 256 * nb * 4));
-    ctx->s_hist_cap = 256 * nb;
+    LEPIC_HIP(hipMalloc(&ctx->s_hist, kMaxRadix * nb * 4));
+    ctx->s_hist_cap = kMaxRadix * nb;
   }
   if (!ctx->d_plist || ncnt > ctx->scan_tmp_cap) {
     if (ctx->d_plist) LEPIC_HIP(hipFree(ctx->d_plist));
@@ -198,43 +204,59 @@ int ensure_sort_scratch(lepic_ctx* ctx, int64_t n, int64_t nb, int64_t ncnt) {
 
 namespace lepic {
 
+template <int R>
+int radix_pass(lepic_ctx* ctx, int cur, int pass, int shift, int64_t nslots, int64_t nb) {
+  hipLaunchKernelGGL(k_hist<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur], nslots, shift,
+                     ctx->s_hist, (int)nb);
+  LEPIC_HIP(hipGetLastError());
+  if (int rc = scan_exclusive_u32(ctx, ctx->s_hist, (int64_t)R * nb)) return rc;
+  hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur],
+                     pass == 0 ? (const uint32_t*)nullptr : ctx->s_val[cur], ctx->s_key[1 - cur],
+                     ctx->s_val[1 - cur], ctx->s_hist, nslots, shift, (int)nb);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
 int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
   Species& s = ctx->sp[p];
-  if (int rc = ensure_packed(ctx, p)) return rc;
-  int64_t N = 0;
-  if (int rc = lepic_get_count(ctx, p + 1, &N)) return rc;
   const int n1 = ctx->g.n1, n2 = ctx->g.n2;
   const int64_t ncell = (int64_t)n1 * n2;
+  if (ncell >= (1ll << 22)) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "more than 2^22 cells");
   const int64_t pl_max = (int64_t)n1 + ncell + 1;
-  const int64_t nb = std::max<int64_t>((N + kTile - 1) / kTile, 1);
-  if (int rc = ensure_sort_scratch(ctx, std::max<int64_t>(N, 1), nb, pl_max + 1)) return rc;
+  // every slot of every segment; empty slots sort behind the particles
+  const int64_t nslots = nseg_of(s) * kChunk;
+  const int64_t nb = std::max<int64_t>((nslots + kTile - 1) / kTile, 1);
+  if (int rc = ensure_sort_scratch(ctx, std::max<int64_t>(nslots, 1), nb, pl_max + 1)) return rc;
   uint32_t* cnt = (uint32_t*)ctx->d_plist;
   if (int rc = zero_async(ctx, cnt, (pl_max + 1) * 4)) return rc;
   const int b = s.cur, o = 1 - s.cur;
   const int64_t* nptr = ctx->d_counts + p;
-  const dim3 g1((unsigned)std::max<int64_t>((N + 255) / 256, 1));
+  const dim3 gs((unsigned)std::max<int64_t>((nslots + 255) / 256, 1));
   if (ctx->g.precision == LEPIC_MIXED)
-    hipLaunchKernelGGL(k_keys_mixed, g1, dim3(256), 0, ctx->stream, s.cell[b], nptr, n1, n2, ctx->s_key[0], cnt);
+    hipLaunchKernelGGL(k_keys_mixed, gs, dim3(256), 0, ctx->stream, s.cell[b], (const int32_t*)s.seg[b], nslots,
+                       n1, n2, ctx->s_key[0], cnt);
   else
-    hipLaunchKernelGGL(k_keys64, g1, dim3(256), 0, ctx->stream, s.d[b][0], s.d[b][1], nptr, ctx->g.h1,
-                       ctx->g.h2, n1, n2, ctx->s_key[0], cnt);
+    hipLaunchKernelGGL(k_keys64, gs, dim3(256), 0, ctx->stream, s.d[b][0], s.d[b][1], (const int32_t*)s.seg[b],
+                       nslots, ctx->g.h1, ctx->g.h2, n1, n2, ctx->s_key[0], cnt);
   LEPIC_HIP(hipGetLastError());
+  // bits of the largest cell index plus the empty-slot key's place above it
   int bits = 1;
-  while ((1ll << bits) < ncell) ++bits;
-  const int passes = (bits + 7) / 8;
+  while ((1ll << bits) <= ncell) ++bits;
+  const int lo = (bits + 1) / 2;
   int cur = 0;
-  for (int pass = 0; pass < passes; ++pass) {
-    const int shift = 8 * pass;
-    hipLaunchKernelGGL(k_hist, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur], nptr, shift,
-                       ctx->s_hist, (int)nb);
-    LEPIC_HIP(hipGetLastError());
-    if (int rc = scan_exclusive_u32(ctx, ctx->s_hist, 256 * nb)) return rc;
-    hipLaunchKernelGGL(k_scatter, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur],
-                       pass == 0 ? (const uint32_t*)nullptr : ctx->s_val[cur], ctx->s_key[1 - cur],
-                       ctx->s_val[1 - cur], ctx->s_hist, nptr, shift, (int)nb);
-    LEPIC_HIP(hipGetLastError());
+  for (int pass = 0; pass < 2; ++pass) {
+    const int shift = pass * lo, width = pass == 0 ? lo : bits - lo;
+    int rc;
+    if (width <= 8) rc = radix_pass<256>(ctx, cur, pass, shift, nslots, nb);
+    else if (width <= 9) rc = radix_pass<512>(ctx, cur, pass, shift, nslots, nb);
+    else if (width <= 10) rc = radix_pass<1024>(ctx, cur, pass, shift, nslots, nb);
+    else rc = radix_pass<2048>(ctx, cur, pass, shift, nslots, nb);
+    if (rc) return rc;
     cur = 1 - cur;
   }
+  int64_t N = 0;
+  if (int rc = lepic_get_count(ctx, p + 1, &N)) return rc;
+  const dim3 g1((unsigned)std::max<int64_t>((N + 255) / 256, 1));
   const uint32_t* perm = ctx->s_val[cur];
   if (ctx->g.precision == LEPIC_MIXED)
     hipLaunchKernelGGL(k_gather_mixed, g1, dim3(256), 0, ctx->stream, perm, nptr, s.cell[b], s.f[b][0],

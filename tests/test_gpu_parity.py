@@ -193,6 +193,32 @@ def test_sort_matches_oracle(precision):
     assert np.array_equal(got, ref) and np.array_equal(pl, pref)
 
 
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+def test_sort_of_unpacked_segments(precision):
+    """The mover leaves partly filled segments; the sort reads them as they
+    are and must give what sorting the packed species gives (and, in FP64,
+    what the oracle's mover + sort give)."""
+    dom = S.box_domain(64, 40, Bmode=None)
+    parts = S.particles(dom, 120000, 2e6, seed=35, edge_frac=0.05)
+    Ei = S.random_E(dom, 1e4, seed=36)
+    sp = [L.Species(E_CHG, E_MASS, 1.0, 120000)]
+    with make_ctx(dom, sp, precision=precision) as a, make_ctx(dom, sp, precision=precision) as b:
+        for c in (a, b):
+            c.upload(1, parts)
+            c.set_grid(L.GRID_E, Ei)
+            c.part_mover(1, deposit=False)
+        moved = b.download(1)                 # packs b
+        assert moved.shape[0] < parts.shape[0]    # some particles were lost at the walls
+        pa = a.part_sorting(1, want_plist=True)   # a is still segmented
+        pb = b.part_sorting(1, want_plist=True)
+        got_a, got_b = a.download(1), b.download(1)
+    assert np.array_equal(got_a, got_b) and np.array_equal(pa, pb)
+    if precision == L.FP64:
+        ref = O.part_mover(dom, E_CHG, E_MASS, 1.0, Ei, parts)
+        sref, pref = O.part_sorting(64, 40, (dom.h1, dom.h2), ref["vxp"])
+        assert np.array_equal(got_a, sref) and np.array_equal(pa, pref)
+
+
 def test_mixed_push_close_to_fp64():
     dom = S.box_domain(64, 48, Bmode="map", Bpeak=0.02)
     parts = S.particles(dom, 50000, 1e6, seed=33, edge_frac=0.0)
