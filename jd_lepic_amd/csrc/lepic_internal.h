@@ -46,6 +46,7 @@ struct Level {
   // multi-block levels: rotating states (index 0 is u or uf) and the
   // residual partial sums of the sweep kernels (kRing slots x blocks)
   bool mb = false;
+  bool small_tiles = false;          // multi-block sweeps on 32-wide tiles
   void* rot[kRot] = {};
   int nbx = 0, nby = 0;
   double* partials = nullptr;

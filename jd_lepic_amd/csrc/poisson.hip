@@ -42,7 +42,10 @@ constexpr size_t kTailLds = 160 * 1024 - 2048;  // dynamic LDS of k_tail (static
 constexpr int kTailThreads = 1024;
 constexpr int kS = kSweepsPerLaunch;
 constexpr int kHalo = 2 * kS + 1;             // validity shrinks by one node per colour pass
-constexpr int kTX = 64;
+constexpr int kTX = 64;                       // wide tiles: 64 x TileY nodes, 1024 threads
+constexpr int kSmallTX = 32;                  // small tiles: 32 x SmallTileY nodes, 512 threads
+constexpr int kSmallThreads = 512;
+constexpr int kSmallTileBelow = 64;           // a level with fewer wide tiles than this takes small ones
 constexpr int kSweepThreads = 1024;
 constexpr int kDefaultTailNodes = 5000;
 constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
@@ -53,6 +56,9 @@ constexpr int kRowBlocks = 128;               // row-loop blocks of the grid-tra
 template <typename T> struct TileY;
 template <> struct TileY<float> { static constexpr int v = 32; };
 template <> struct TileY<double> { static constexpr int v = 16; };
+template <typename T> struct SmallTileY;
+template <> struct SmallTileY<float> { static constexpr int v = 16; };
+template <> struct SmallTileY<double> { static constexpr int v = 8; };
 
 // ---- SOR update of one node (sors.f90:108-110 / 238-240) ----
 // FP64 divides like the reference; the FP32 smoother of the mixed form
@@ -217,10 +223,10 @@ struct SweepArgs {
   double eps;
 };
 
-// Sweeps k0+1 .. k0+kexec of a visit on one tile (see the file comment).
-template <typename T>
+// Sweeps k0+1 .. k0+kexec of a visit on one TX x TY tile with NT threads
+// (see the file comment).
+template <typename T, int kTX, int TY, int kSweepThreads>
 __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec) {
-  constexpr int TY = TileY<T>::v;
   constexpr int RW = kTX + 2 * kHalo, RH = TY + 2 * kHalo, RN = RW * RH;
   constexpr int HW = (RW - 2) / 2;  // nodes of one colour per inner region row
   static_assert((RW & 1) == 0, "region width must be even");
@@ -1040,7 +1046,12 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
   for (int k = k0; k < k_end; k += kS) {
     const int kexec = std::min(kS, k_end - k);
-    hipLaunchKernelGGL(k_sweeps<T>, dim3(L.nbx, L.nby), dim3(kSweepThreads), 0, ctx->stream, A, k, kexec);
+    if (L.small_tiles)
+      hipLaunchKernelGGL((k_sweeps<T, kSmallTX, SmallTileY<T>::v, kSmallThreads>), dim3(L.nbx, L.nby),
+                         dim3(kSmallThreads), 0, ctx->stream, A, k, kexec);
+    else
+      hipLaunchKernelGGL((k_sweeps<T, kTX, TileY<T>::v, kSweepThreads>), dim3(L.nbx, L.nby), dim3(kSweepThreads),
+                         0, ctx->stream, A, k, kexec);
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
@@ -1299,6 +1310,14 @@ int poisson_setup(lepic_ctx* ctx) {
     }
     L.nbx = (L.n1 + 1 + kTX - 1) / kTX;
     L.nby = (L.n2 + 1 + ty - 1) / ty;
+    // few wide tiles leave most CUs idle: the level takes small tiles, whose
+    // launch latency is lower (each block has less than half the nodes)
+    L.small_tiles = L.nbx * L.nby < kSmallTileBelow;
+    if (L.small_tiles) {
+      const int sty = mixed ? SmallTileY<float>::v : SmallTileY<double>::v;
+      L.nbx = (L.n1 + 1 + kSmallTX - 1) / kSmallTX;
+      L.nby = (L.n2 + 1 + sty - 1) / sty;
+    }
     const size_t np = (size_t)kRing * L.nbx * L.nby;
     LEPIC_HIP(hipMalloc(&L.partials, np * 8));
     LEPIC_HIP(hipMemsetAsync(L.partials, 0, np * 8, ctx->stream));
