@@ -51,7 +51,7 @@ constexpr int kDefaultTailNodes = 5000;
 constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
-constexpr int kRowBlocks = 128;               // row-loop blocks of the grid-transfer kernels
+constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
 
 template <typename T> struct TileY;
 template <> struct TileY<float> { static constexpr int v = 32; };
@@ -87,22 +87,35 @@ __device__ __forceinline__ T getres(PU u, PB b, int ir, int jr, int n1, T aw, T 
   return urhs - (aw * uw + aw * ue + an * un + an * us + ac * uij);
 }
 
-// full-weighted residual of coarse node (i,j), mg.f90:386-414
+// full-weighted residual of coarse node (i,j), mg.f90:386-414.  The nine
+// fine residuals are evaluated unconditionally (at a position clamped into
+// the interior when the reference skips one), so all loads issue at once;
+// the reference's conditions select which of them count.
 template <typename T, typename PU, typename PB, typename PC>
 __device__ __forceinline__ T restrict_node(PU u, PB b, PC bc, int i, int j, int n1, int n2, T aw, T an, T ac) {
   const int ir = 2 * i - 1, jr = 2 * j - 1;
   const int64_t W = n1 + 3;
   auto B = [&](int a, int c) { return bc[(int64_t)c * W + a]; };
-  T r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0, r8 = 0;
-  T r9 = getres(u, b, ir, jr, n1, aw, an, ac);
-  if (ir != n1 + 1 && jr != n2 + 1 && B(ir + 1, jr + 1) <= 0) r1 = getres(u, b, ir + 1, jr + 1, n1, aw, an, ac);
-  if (ir != n1 + 1 && B(ir + 1, jr) <= 0) r2 = getres(u, b, ir + 1, jr, n1, aw, an, ac);
-  if (B(ir + 1, jr - 1) <= 0 && ir != n1 + 1 && jr != 1) r3 = getres(u, b, ir + 1, jr - 1, n1, aw, an, ac);
-  if (B(ir, jr - 1) <= 0 && jr != 1) r4 = getres(u, b, ir, jr - 1, n1, aw, an, ac);
-  if (B(ir - 1, jr - 1) <= 0 && ir != 1 && jr != 1) r5 = getres(u, b, ir - 1, jr - 1, n1, aw, an, ac);
-  if (B(ir - 1, jr) <= 0 && ir != 1) r6 = getres(u, b, ir - 1, jr, n1, aw, an, ac);
-  if (B(ir - 1, jr + 1) <= 0 && ir != 1 && jr != n2 + 1) r7 = getres(u, b, ir - 1, jr + 1, n1, aw, an, ac);
-  if (B(ir, jr + 1) <= 0 && jr != n2 + 1) r8 = getres(u, b, ir, jr + 1, n1, aw, an, ac);
+  auto G = [&](int a, int c) {
+    a = a < 1 ? 1 : (a > n1 + 1 ? n1 + 1 : a);
+    c = c < 1 ? 1 : (c > n2 + 1 ? n2 + 1 : c);
+    return getres(u, b, a, c, n1, aw, an, ac);
+  };
+  const bool xe = ir != n1 + 1, xw = ir != 1, yn = jr != n2 + 1, ys = jr != 1;
+  const bool f1 = xe && yn && B(ir + 1, jr + 1) <= 0;
+  const bool f2 = xe && B(ir + 1, jr) <= 0;
+  const bool f3 = B(ir + 1, jr - 1) <= 0 && xe && ys;
+  const bool f4 = B(ir, jr - 1) <= 0 && ys;
+  const bool f5 = B(ir - 1, jr - 1) <= 0 && xw && ys;
+  const bool f6 = B(ir - 1, jr) <= 0 && xw;
+  const bool f7 = B(ir - 1, jr + 1) <= 0 && xw && yn;
+  const bool f8 = B(ir, jr + 1) <= 0 && yn;
+  const T g9 = G(ir, jr), g1 = G(ir + 1, jr + 1), g2 = G(ir + 1, jr), g3 = G(ir + 1, jr - 1);
+  const T g4 = G(ir, jr - 1), g5 = G(ir - 1, jr - 1), g6 = G(ir - 1, jr), g7 = G(ir - 1, jr + 1);
+  const T g8 = G(ir, jr + 1);
+  const T r9 = g9;
+  const T r1 = f1 ? g1 : (T)0, r2 = f2 ? g2 : (T)0, r3 = f3 ? g3 : (T)0, r4 = f4 ? g4 : (T)0;
+  const T r5 = f5 ? g5 : (T)0, r6 = f6 ? g6 : (T)0, r7 = f7 ? g7 : (T)0, r8 = f8 ? g8 : (T)0;
   return ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
 }
 
@@ -268,13 +281,27 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
   const int gi0 = 1 + (int)blockIdx.x * kTX - kHalo;  // global i of region column 0
   const int gj0 = 1 + (int)blockIdx.y * TY - kHalo;
   const T* uin = A.rot[(s_c0 + k0) % kRot];
-  for (int q = threadIdx.x; q < RN; q += kSweepThreads) {
-    const int r = q / RW, cc = q - r * RW;
-    const int i = gi0 + cc, j = gj0 + r;
-    const bool in = i >= 1 && i <= n1 + 1 && j >= 1 && j <= n2 + 1;
-    su[q] = in ? uin[(int64_t)j * W + i] : (T)0;
-    sbc[q] = in ? A.bc[(int64_t)j * W + i] : (int8_t)1;
-    sb[q] = in ? A.b[(int64_t)(j - 1) * (n1 + 1) + (i - 1)] : (T)0;
+  {
+    // every load of the region is issued before any LDS store: one memory
+    // latency for the whole region instead of one per loop trip
+    constexpr int kIt = (RN + kSweepThreads - 1) / kSweepThreads;
+    T lu[kIt], lb[kIt];
+    int8_t lc[kIt];
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int q = threadIdx.x + it * kSweepThreads;
+      const int r = q / RW, cc = q - r * RW;
+      const int i = gi0 + cc, j = gj0 + r;
+      const bool in = q < RN && i >= 1 && i <= n1 + 1 && j >= 1 && j <= n2 + 1;
+      lu[it] = in ? uin[(int64_t)j * W + i] : (T)0;
+      lc[it] = in ? A.bc[(int64_t)j * W + i] : (int8_t)1;
+      lb[it] = in ? A.b[(int64_t)(j - 1) * (n1 + 1) + (i - 1)] : (T)0;
+    }
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int q = threadIdx.x + it * kSweepThreads;
+      if (q < RN) { su[q] = lu[it]; sbc[q] = lc[it]; sb[q] = lb[it]; }
+    }
   }
   __syncthreads();
   const T aw = A.aw, an = A.an, ac = A.ac, om = A.omega;
