@@ -731,17 +731,28 @@ __device__ __forceinline__ void t_smooth_fused(PU u, PB b, PC bc, int n1, int n2
     mq = bc[x];
     xk[r] = x; qk[r] = q; uk[r] = ok && mq <= 0; rk[r] = ok && mq == -1;
   }
+  // in block mode the last item of a level is often held by a few waves
+  // only: waves that hold none skip it (a wave-uniform branch after the
+  // other items' loads, so those still issue together)
+  bool wr[R], wk[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    wr[r] = WAVE || r + 1 < R || __ballot(2 * (tid + P * r) + 1 < nint) != 0;
+    wk[r] = WAVE || r + 1 < R || __ballot(2 * (tid + P * r) < nint) != 0;
+  }
   T cand[R];
   auto red_pass = [&](bool want_res) {
     double rs = 0.0;
     T c[R], w[R], e[R], nn[R], so[R], bb[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+      if (!wr[r]) continue;
       c[r] = u[xr[r]]; w[r] = u[xr[r] - 1]; e[r] = u[xr[r] + 1];
       nn[r] = u[xr[r] + W]; so[r] = u[xr[r] - W]; bb[r] = b[qr[r]];
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+      if (!wr[r]) continue;
       if (want_res && rr[r]) rs += fabs(sor_resid(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac));
       cand[r] = sor_update(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac, omega);
     }
@@ -757,11 +768,13 @@ __device__ __forceinline__ void t_smooth_fused(PU u, PB b, PC bc, int n1, int n2
     T c[R], w[R], e[R], nn[R], so[R], bb[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+      if (!wk[r]) continue;
       c[r] = u[xk[r]]; w[r] = u[xk[r] - 1]; e[r] = u[xk[r] + 1];
       nn[r] = u[xk[r] + W]; so[r] = u[xk[r] - W]; bb[r] = b[qk[r]];
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
+      if (!wk[r]) continue;
       T v = c[r];
       if (uk[r]) {
         v = sor_update(c[r], bb[r], w[r], nn[r], so[r], e[r], aw, an, ac, omega);
