@@ -222,6 +222,9 @@ def cpu_baseline(W, budget_s=10.0):
                  Nm={1: W["Nm"], 2: W["Nm"], 3: W["Nm"]},
                  vt0={1: W["vte"], 2: W["vti"], 3: W["vtg"]},
                  np_mx={1: 0.0, 2: 0.0, 3: W["ngas"]}, nu_uplim={1: 6e7, 2: 5e6, 3: 5e6})
+    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if os.path.exists(harness):
+        return cpu_baseline_reference(harness, W, dom, Ei, pos, ve, vi, N, O, setup, tab, budget_s)
     t_push = t_mcc = 0.0
     reps = 0
     start = time.perf_counter()
@@ -244,6 +247,59 @@ def cpu_baseline(W, budget_s=10.0):
     return dict(value=2 * N / per_step, unit="particles/s", cores=1, kind="port",
                 sample=f"oracle C restatement, 1 core: {n}x{n} cells x 100 ppc x 2 species "
                        f"({2 * N} particles), push+deposit each step + MCC/10, {reps} reps")
+
+
+def _ref_push_deposit(harness, W, dom, Ei, vxp, charge, mass, reps, O):
+    """Seconds of reps x (part_mover + charge_deposition) of the reference's
+    own Fortran (part_expmover.f90, compiled by oracle/Makefile), one core."""
+    import subprocess
+    import tempfile
+    n, h = dom.n1, dom.h1
+    N = vxp.shape[0]
+    ints = [n, n, dom.nB[0], dom.nB[1], N, dom.ngrid, 1, dom.flag_gridB, dom.flag_B_pos, 0, reps]
+    dbls = [float(np.float32(1.60217646e-19)), 1.66053886e-27, h, h, dom.hB[0], dom.hB[1], dom.dt,
+            charge, mass, W["Nm"], dom.zmax, n * h, n * h]
+    kq = O.kq(dom)
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.int64(len(ints)).tobytes() + np.asarray(ints, np.int64).tobytes())
+            f.write(np.int64(len(dbls)).tobytes() + np.asarray(dbls, np.float64).tobytes())
+            for a in (np.asarray(Ei, np.float64), np.asarray(dom.Bi, np.float64),
+                      np.asarray(dom.bcnd, np.int32), np.ascontiguousarray(vxp, np.float64),
+                      np.zeros(N, np.int8), np.zeros((n + 3, n + 3)), np.asarray(kq, np.float64)):
+                f.write(np.ascontiguousarray(a).tobytes())
+        subprocess.run([harness, "bench", fin, fout], check=True)
+        sec = float(np.frombuffer(open(fout, "rb").read(8), np.float64)[0])
+    return sec
+
+
+def cpu_baseline_reference(harness, W, dom, Ei, pos, ve, vi, N, O, setup, tab, budget_s):
+    """The reference's own push + deposition (Fortran, one core) on the
+    sample; MCC every 10 steps from the C restatement (collisions.f90 needs
+    MPI, so it cannot be compiled here)."""
+    e = np.column_stack([pos, ve])
+    i = np.column_stack([pos, vi])
+    probe = _ref_push_deposit(harness, W, dom, Ei, e, -L.QE, L.ME, 4, O) + \
+        _ref_push_deposit(harness, W, dom, Ei, i, L.QE, 39.95 * L.AMU, 4, O)
+    reps = max(4, int(budget_s / max(probe / 4, 1e-6)))
+    t_push = _ref_push_deposit(harness, W, dom, Ei, e, -L.QE, L.ME, reps, O) + \
+        _ref_push_deposit(harness, W, dom, Ei, i, L.QE, 39.95 * L.AMU, reps, O)
+    n = dom.n1
+    cap = N + N // 10
+    pe = np.zeros((cap, 6)); pe[:N] = e
+    pi_ = np.zeros((cap, 6)); pi_[:N] = i
+    parts = {1: dict(vxp=pe, np_tot=N, flag_dead=np.zeros(cap, np.int8)),
+             2: dict(vxp=pi_, np_tot=N, flag_dead=np.zeros(cap, np.int8))}
+    np_red = [np.full((n + 3, n + 3), 1e16), np.full((n + 3, n + 3), 1e16)]
+    t = time.perf_counter()
+    O.collisions(dom, setup, tab, parts, np_red, 123456)
+    t_mcc = time.perf_counter() - t
+    per_step = t_push / reps + t_mcc / 10.0
+    return dict(value=2 * N / per_step, unit="particles/s", cores=1, kind="reference",
+                sample=f"reference part_mover + charge_deposition (part_expmover.f90 built by amdflang -O2, "
+                       f"1 core) on {n}x{n} cells x 100 ppc x 2 species ({2 * N} particles), {reps} steps; "
+                       f"plus MCC/10 from the C restatement")
 
 
 def main():

@@ -8,7 +8,8 @@
  * Usage: ref_harness <cmd> <in.bin> <out.bin>
  *   in.bin  = int64 ni, int64[ni], int64 nd, double[nd], then raw arrays
  *   out.bin = raw arrays (order per command, see tests/refharness.py)
- * Commands: ran2, gauss, mover, deposit, efield, sort.
+ * Commands: ran2, gauss, mover, deposit, efield, sort, bench (the timed
+ * push + deposition loop of bench.py's CPU baseline).
  *
  * The routines take every argument by reference (implicit interfaces) and
  * read their scalars from the COMMON blocks declared in particle_info.h:56-73
@@ -21,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
+#include <time.h>
 
 #define NPART 32
 
@@ -168,6 +170,58 @@ int main(int argc, char **argv) {
     wr(&cnt_dead, 8);
     wr(ss2D, 16 * nodes);
     wr(fd, (size_t)nmax);
+  } else if (!strcmp(cmd, "bench")) {
+    /* I: n1 n2 nB1 nB2 N ngrid flag_B flag_gridB flag_B_pos tag_neg reps
+     * D: as "mover"; arrays: Ei, Bi, bcnd, vxp, flag_dead, phi, kq.
+     * reps x (part_mover + charge_deposition), main.f90:1041-1086 with one
+     * thread; writes the seconds spent and the final particle count. */
+    set_constants();
+    int32_t n[3] = {(int32_t)I[0], (int32_t)I[1], 1}, na[3] = {n[0], n[1], 1};
+    int32_t nB[2] = {(int32_t)I[2], (int32_t)I[3]};
+    int32_t N = (int32_t)I[4], nmax = N > 0 ? N : 1, ngrid = (int32_t)I[5];
+    const int64_t reps = I[10];
+    part_info_int_.flag_B = (int32_t)I[6];
+    part_info_int_.flag_gridB = (int32_t)I[7];
+    part_info_int_.flag_B_pos = (int32_t)I[8];
+    part_info_int_.tag_neg = I[9] ? 1 : 0;
+    part_info_int_.size_na = 1;
+    double h[3] = {D[2], D[3], 1.0}, hB[2] = {D[4], D[5]};
+    part_info_dp_2_.dt = D[6];
+    part_info_dp_1_.charge[0] = D[7];
+    part_info_dp_1_.mass[0] = D[8];
+    part_info_dp_2_.Nm[0] = D[9];
+    part_info_dp_2_.zmax = D[10];
+    part_info_dp_2_.xmax = D[11];
+    part_info_dp_2_.ymax = D[12];
+    part_info_dp_2_.hz = -1.0;
+    size_t nodes = (size_t)(n[0] + 3) * (n[1] + 3), bnodes = (size_t)(nB[0] + 3) * (nB[1] + 3);
+    double *Ei = rd(16 * nodes), *Bi = rd(32 * bnodes);
+    int32_t *bcnd = rd(4 * nodes);
+    double *vxp = rd(48 * (size_t)nmax);
+    int8_t *fd = rd((size_t)nmax);
+    double *phi = rd(8 * nodes), *kq = rd(8 * nodes);
+    double *p_mac = calloc(2 * (size_t)(ngrid + 1), 8), P_loss[4] = {0, 0, 0, 0};
+    double *uB = calloc((size_t)(na[0] + 1) * (na[1] + 1), 8), cnt_dead = 0, sum_q = 0;
+    double *sum_q_y = calloc((size_t)(ngrid > 0 ? ngrid : 1) * (n[1] + 3), 8);
+    double ps[4], sum_dEk = 0;
+    double *ss2D = calloc(2 * nodes, 8), *np = calloc(nodes, 8);
+    int32_t istep = 1, ntype = 1, nproc = 1, iproc = 1, ptype = 1, iseed = 123456, ip_ps = 0,
+            nmax_ps = 1, np_tot = N, Nh = 0, n_icp = 1;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int64_t k = 0; k < reps; ++k) {
+      part_mover_(&istep, n, na, h, Ei, Bi, p_mac, P_loss, vxp, bcnd, &nmax, &ntype, &ngrid, fd,
+                  &nproc, &np_tot, phi, uB, &iproc, &ptype, &iseed, &cnt_dead, &sum_q, sum_q_y,
+                  ps, &ip_ps, &nmax_ps, nB, hB, ss2D);
+      memset(np, 0, 8 * nodes);
+      charge_deposition_(n, h, vxp, &nmax, &ntype, kq, np, &nproc, &np_tot, &sum_dEk, &Nh, &iproc,
+                         &ptype, &n_icp);
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    int64_t npt = np_tot;
+    wr(&sec, 8);
+    wr(&npt, 8);
   } else if (!strcmp(cmd, "deposit")) {
     /* I: n1 n2 N; D: qe amu h1 h2 Nm zmax; arrays: vxp, kq */
     set_constants();
