@@ -192,6 +192,26 @@ int lepic_get_cnt_col(lepic_ctx *ctx, double *cnt_col);
 /* P_loss(3,ptype) energy change of collisions since the last reset */
 int lepic_get_mcc_ploss(lepic_ctx *ctx, double *P_loss3);
 
+/* ---- electron heating (eheating, e_heating.f90:1-112; called at main.f90:1033-1037
+ *      every ns_heat steps before the push) ----
+ * Electrons (species 1) inside the slit (flag_c = 0: |x-xa| <= dr,
+ * yl_pow <= y <= yr_pow) or the disk (flag_c = 1: radius dr about
+ * (xa, ymax/2)) are heated with probability nu*dt, nu = nudt/dt or, with
+ * flag_gpp = 1, (nudt/dt)*9*Pabs_cor*exp(-9 r^2/dr^2): a Maxwellian kick of
+ * thermal speed vt(i_icp), i_icp = int(n_icp*y/ymax)+1, added to
+ * (eheat_type = 0) or replacing (1) the velocity.  *dP receives the energy
+ * change 0.5*Nm*m*sum(v^2 - v_old^2), the reference's P_loss(2,1).
+ * LEPIC_MCC_CHECK replays the reference's ran2 stream from *iseed in
+ * particle order (FP64 contexts); LEPIC_MCC_PROD draws per particle slot
+ * from a counter-based generator keyed by (seed, it). */
+typedef struct {
+  int32_t flag_c, flag_gpp, eheat_type, n_icp;   /* n_icp <= 64 */
+  double xa, dr, yl_pow, yr_pow, nudt, Pabs_cor;
+  const double *vt;       /* vt(n_icp), host memory */
+} lepic_heating;
+int lepic_eheating(lepic_ctx *ctx, const lepic_heating *h, int32_t mode, uint64_t seed, int32_t it,
+                   int32_t *iseed, double *dP);
+
 /* ---- synthetic input (load_part.f90:59-224 restated for a GPU) ----
  * Fills [xl,xr) x [0,ymax) of a species with ppc particles per cell
  * uniformly, velocities Maxwellian with thermal speed vt (vt0 of
@@ -215,6 +235,11 @@ typedef struct {
   /* outputs */
   lepic_solve_info solve;
   double ms_rhs_poisson_e, ms_sort, ms_mcc, ms_push;   /* phase times (HIP events) */
+  /* electron heating before the push when heat != NULL and it%ns_heat == 0
+   * (main.f90:1033-1037), in mode heat_mode with the seeds above */
+  const lepic_heating *heat;
+  int32_t ns_heat, heat_mode;
+  double P_heat;          /* output: the heating's energy change this step */
 } lepic_step_args;
 /* ---- learned warm start (no counterpart in the reference) ----
  * A U-Net (4 levels, 16-32-64-128 channels, BF16 MFMA kernels with FP32

@@ -353,14 +353,42 @@ class Context:
         self._check(self.lib.lepic_get_mcc_ploss(self._ctx, _dp(out)))
         return out
 
+    # ---- electron heating (e_heating.f90:1-112) ----
+    @staticmethod
+    def _heating(heat):
+        """heat: dict with flag_c, flag_gpp, eheat_type, xa, dr, yl_pow, yr_pow,
+        nudt, Pabs_cor and vt (n_icp thermal speeds).  Returns the struct and
+        the array it points at (kept alive by the caller)."""
+        vt = np.ascontiguousarray(heat["vt"], dtype=np.float64).ravel()
+        h = capi.Heating()
+        h.flag_c, h.flag_gpp, h.eheat_type = heat["flag_c"], heat["flag_gpp"], heat["eheat_type"]
+        h.n_icp = vt.size
+        h.xa, h.dr, h.yl_pow, h.yr_pow = heat["xa"], heat["dr"], heat.get("yl_pow", 0.0), heat.get("yr_pow", 0.0)
+        h.nudt, h.Pabs_cor, h.vt = heat["nudt"], heat.get("Pabs_cor", 1.0), _dp(vt)
+        return h, vt
+
+    def eheating(self, heat, it=0, mode=MCC_PROD, seed=0, iseed=123456):
+        """One call of eheating on species 1; returns (dP, iseed): the energy
+        change P_loss(2,1) gets and, in check mode, the ran2 state after it."""
+        h, keep = self._heating(heat)
+        s, dP = C.c_int32(iseed), C.c_double(0.0)
+        self._check(self.lib.lepic_eheating(self._ctx, C.byref(h), mode, seed, it, C.byref(s), C.byref(dP)))
+        del keep
+        return dP.value, s.value
+
     # ---- whole step ----
     def step(self, it, eps=1e-10, ncycl=20000, nsort=10, ns_coll=10, mcc_mode=MCC_PROD, seed=0,
-             iseed=123456, start=START_PREV):
+             iseed=123456, start=START_PREV, heat=None, ns_heat=1, heat_mode=MCC_PROD):
         a = capi.StepArgs()
         a.eps, a.ncycl, a.nsort, a.ns_coll, a.mcc_mode = eps, ncycl, nsort, ns_coll, mcc_mode
         a.seed, a.iseed, a.start = seed, iseed, start
+        keep = None
+        if heat is not None:
+            h, keep = self._heating(heat)
+            a.heat, a.ns_heat, a.heat_mode = C.pointer(h), ns_heat, heat_mode
         self._check(self.lib.lepic_step(self._ctx, it, C.byref(a)))
-        return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res,
+        del keep
+        return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res, P_heat=a.P_heat,
                     ms=dict(field=a.ms_rhs_poisson_e, sort=a.ms_sort, mcc=a.ms_mcc, push=a.ms_push))
 
 

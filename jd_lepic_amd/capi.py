@@ -59,12 +59,19 @@ class Reactions(C.Structure):
                 ("nu_uplim", _dp)]
 
 
+class Heating(C.Structure):
+    _fields_ = [("flag_c", C.c_int32), ("flag_gpp", C.c_int32), ("eheat_type", C.c_int32),
+                ("n_icp", C.c_int32), ("xa", C.c_double), ("dr", C.c_double), ("yl_pow", C.c_double),
+                ("yr_pow", C.c_double), ("nudt", C.c_double), ("Pabs_cor", C.c_double), ("vt", _dp)]
+
+
 class StepArgs(C.Structure):
     _fields_ = [("eps", C.c_double), ("ncycl", C.c_int32), ("nsort", C.c_int32),
                 ("ns_coll", C.c_int32), ("mcc_mode", C.c_int32), ("seed", C.c_uint64),
                 ("iseed", C.c_int32), ("start", C.c_int32), ("solve", SolveInfo),
                 ("ms_rhs_poisson_e", C.c_double), ("ms_sort", C.c_double), ("ms_mcc", C.c_double),
-                ("ms_push", C.c_double)]
+                ("ms_push", C.c_double), ("heat", C.POINTER(Heating)), ("ns_heat", C.c_int32),
+                ("heat_mode", C.c_int32), ("P_heat", C.c_double)]
 
 
 _ctxp = C.c_void_p
@@ -125,6 +132,7 @@ SIGNATURES = {
                                    C.POINTER(MccInfo)]),
     "lepic_get_cnt_col": (C.c_int, [_ctxp, _dp]),
     "lepic_get_mcc_ploss": (C.c_int, [_ctxp, _dp]),
+    "lepic_eheating": (C.c_int, [_ctxp, C.POINTER(Heating), C.c_int32, C.c_uint64, C.c_int32, _ip, _dp]),
     "lepic_load_uniform": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_uint64]),
     "lepic_step": (C.c_int, [_ctxp, C.c_int32, C.POINTER(StepArgs)]),

@@ -198,6 +198,7 @@ struct lepic_ctx {
   // learned warm start (unet.hip)
   lepic::Net* net = nullptr;
   Moments mom;
+  double* d_heat = nullptr;          // electron heating: vt table, power, seed (heating.hip)
 
   int fail(int code, const std::string& msg) {
     err = msg;
@@ -293,6 +294,24 @@ __host__ __device__ inline U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
 // 53-bit uniform in [0,1) from two words
 __host__ __device__ inline double u01(uint32_t hi, uint32_t lo) {
   return (double)((((uint64_t)hi << 32) | lo) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// The reference's generator (ran2.f90:1-15: Park-Miller minimal standard,
+// multiplier 16807 modulo 2^31-1, Schrage's factorisation) and its
+// Maxwellian pair (load_gauss.f90:14-21), used by the check-mode replays.
+__device__ inline double ran2(int32_t& s) {
+  const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+  int32_t k = s / iq;
+  s = ia * (s - k * iq) - ir * k;
+  if (s < 0) s += im;
+  return (1.0 / (double)im) * (double)s;
+}
+
+__device__ inline void load_gauss(double& vx, double& vy, double vt, double r1, double r2, double pi) {
+  double vp = vt * sqrt(-log(1.0 - r1));
+  double th = 2.0 * pi * r2;
+  vx = vp * cos(th);
+  vy = vp * sin(th);
 }
 
 }  // namespace lepic

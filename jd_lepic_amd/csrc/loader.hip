@@ -157,6 +157,9 @@ extern "C" int lepic_step(lepic_ctx* ctx, int32_t it, lepic_step_args* a) {
     lepic_mcc_info info;
     if (!rc) rc = lepic_collisions(ctx, it, a->mcc_mode, a->seed, &a->iseed, &info);
   }
+  a->P_heat = 0.0;
+  if (!rc && a->heat && a->ns_heat > 0 && it % a->ns_heat == 0)  // :1033-1037
+    rc = lepic_eheating(ctx, a->heat, a->heat_mode, a->seed, it, &a->iseed, &a->P_heat);
   LEPIC_HIP(hipEventRecord(ev[3], ctx->stream));
   for (int p = 1; p <= ctx->nsp && !rc; ++p) rc = lepic_part_mover(ctx, p, 1);  // :1041-1086
   LEPIC_HIP(hipEventRecord(ev[4], ctx->stream));

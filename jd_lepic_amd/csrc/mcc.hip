@@ -68,22 +68,6 @@ PartPtrs part_ptrs(lepic_ctx* ctx) {
   return P;
 }
 
-// ---- the reference's generator and Maxwellian draw -------------------------
-__device__ inline double ran2(int32_t& s) {  // ran2.f90:1-15
-  const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
-  int32_t k = s / iq;
-  s = ia * (s - k * iq) - ir * k;
-  if (s < 0) s += im;
-  return (1.0 / (double)im) * (double)s;
-}
-
-__device__ inline void load_gauss(double& vx, double& vy, double vt, double r1, double r2, double pi) {
-  double vp = vt * sqrt(-log(1.0 - r1));  // load_gauss.f90:14-21
-  double th = 2.0 * pi * r2;
-  vx = vp * cos(th);
-  vy = vp * sin(th);
-}
-
 // ---------------------------------------------------------------------------
 // check mode: one lane, FP64 particle state
 // ---------------------------------------------------------------------------

@@ -46,6 +46,12 @@ module lepic_capi
      type(c_ptr) :: charge, mass, Nm, vt0, np_mx, nu_uplim
   end type lepic_reactions
 
+  type, bind(C) :: lepic_heating
+     integer(c_int32_t) :: flag_c, flag_gpp, eheat_type, n_icp
+     real(c_double) :: xa, dr, yl_pow, yr_pow, nudt, Pabs_cor
+     type(c_ptr) :: vt
+  end type lepic_heating
+
   interface
      integer(c_int) function lepic_create(g, sp, ctx) bind(C, name='lepic_create')
        import :: c_int, c_ptr, lepic_grid_desc, lepic_species_desc
@@ -174,6 +180,16 @@ module lepic_capi
        integer(c_int32_t), intent(inout) :: iseed
        type(lepic_mcc_info), intent(out) :: info
      end function lepic_collisions
+     ! electron heating (e_heating.f90:1-112), called where main.f90:1033-1037 calls eheating
+     integer(c_int) function lepic_eheating(ctx, h, mode, seed, it, iseed, dP) bind(C, name='lepic_eheating')
+       import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double, lepic_heating
+       type(c_ptr), value :: ctx
+       type(lepic_heating), intent(in) :: h
+       integer(c_int32_t), value :: mode, it
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), intent(inout) :: iseed
+       real(c_double), intent(out) :: dP
+     end function lepic_eheating
      ! averaged moments (utils.f90:1-265) and backup files (main.f90:1102-1132, restart.f90)
      integer(c_int) function lepic_moments_accumulate(ctx, size_na) bind(C, name='lepic_moments_accumulate')
        import :: c_int, c_ptr, c_int32_t

@@ -9,7 +9,10 @@
 ! coarsened by size_na (main.f90:903-944), particle backups every nbak steps
 ! into DATA.BAK/particles0.bak (main.f90:1102-1132; the directory must
 ! exist, as for the reference), and restart = 1 starts
-! from that file (restart.f90:56-64) instead of a uniform load.  Prints the
+! from that file (restart.f90:56-64) instead of a uniform load.  With
+! flag_heat = 1 the electrons in the slit or disk of e_heating.f90 are heated
+! every ns_heat steps before the push (main.f90:1033-1037); xa and dr default
+! to the centre and a quarter of the box, the slit to the box height.  Prints the
 ! phase timers in the style of the reference's "<t> (ms)" lines
 ! (main.f90:1318-1322), the particle counts and the averaged Te at the centre.
 program lepic_driver
@@ -19,10 +22,13 @@ program lepic_driver
   ! ---- namelist ----
   integer :: n = 128, ppc = 20, nsteps = 20, ng = 7, precision = 0, nsort = 10, ns_coll = 10
   integer :: mcc_mode = 1, navg = 0, size_na = 1, nbak = 0, restart = 0
+  integer :: flag_heat = 0, ns_heat = 1, flag_c = 1, flag_gpp = 0, eheat_type = 0
+  real(8) :: xa = -1.d0, dr = -1.d0, yl_pow = 0.d0, yr_pow = -1.d0, nudt = 1.d-3
   real(8) :: dx = 5.d-5, dt = 5.d-12, Te = 3.d0, Ti = 0.03d0, Tgas = 0.02585d0, n0 = 1.d16
   real(8) :: B0 = 0.01d0, pgas = 0.3d0, Tgas_K = 300.d0, eps = 1.d-10
   namelist /lepic/ n, ppc, nsteps, ng, precision, nsort, ns_coll, mcc_mode, dx, dt, Te, Ti, Tgas, &
-       n0, B0, pgas, Tgas_K, eps, navg, size_na, nbak, restart
+       n0, B0, pgas, Tgas_K, eps, navg, size_na, nbak, restart, flag_heat, ns_heat, flag_c, flag_gpp, &
+       eheat_type, xa, dr, yl_pow, yr_pow, nudt
   ! ---- constants (main.f90:120-124) ----
   real(8) :: qe, amu, eps0, pi, me, mAr, kB
   ! ---- state ----
@@ -32,6 +38,9 @@ program lepic_driver
   type(lepic_solve_info) :: info
   type(lepic_mcc_info) :: minfo
   type(lepic_reactions) :: rx
+  type(lepic_heating) :: heat
+  real(8), target :: vt_heat(1)
+  real(8) :: dP, P_heat
   integer(c_int32_t), allocatable, target :: bcnd(:,:)
   real(8), allocatable, target :: phi(:,:), Bi(:,:,:)
   real(8), allocatable, target :: Er(:), sig(:,:), Eth(:), sigv(:,:), charge(:), mass(:), Nm(:)
@@ -149,6 +158,15 @@ program lepic_driver
      call lepic_check(ctx, lepic_charge_deposition(ctx, i), 'deposit')
   end do
 
+  ! electron heating parameters (particle_info.h members read by e_heating.f90)
+  if (xa < 0.d0) xa = L/2
+  if (dr < 0.d0) dr = L/4
+  if (yr_pow < 0.d0) yr_pow = L
+  vt_heat(1) = vte
+  heat = lepic_heating(flag_c, flag_gpp, eheat_type, 1, xa, dr, yl_pow, yr_pow, nudt, pabs_cor, &
+       c_loc(vt_heat(1)))
+  P_heat = 0.d0
+
   ! ---- time loop (main.f90:689-1090) ----
   t_field = 0; t_sort = 0; t_mcc = 0; t_move = 0; vcyc = 0
   iseed = 123456
@@ -181,6 +199,10 @@ program lepic_driver
         call lepic_check(ctx, lepic_collisions(ctx, it, mcc_mode, 7_8, iseed, minfo), 'collisions')
      end if
      call system_clock(c1); t_mcc = t_mcc + real(c1 - c0, 8)/crate; c0 = c1
+     if (flag_heat == 1 .and. mod(it, ns_heat) == 0) then
+        call lepic_check(ctx, lepic_eheating(ctx, heat, mcc_mode, 11_8, it, iseed, dP), 'eheating')
+        P_heat = P_heat + dP
+     end if
      do i = 1, 2
         call lepic_check(ctx, lepic_part_mover(ctx, i, 1), 'part_mover')
      end do
@@ -198,6 +220,7 @@ program lepic_driver
   call lepic_check(ctx, lepic_get_count(ctx, 2, ni), 'count')
   print '(a,i12,a,i12)', ' particles: e- ', ne, '  Ar+ ', ni
   print '(a,f10.3)', ' V-cycles per step: ', real(vcyc, 8)/nsteps
+  if (flag_heat == 1) print '(a,es14.6)', ' heating P_loss(2,1) (J): ', P_heat
   print '(a)', ' <t> (ms): E/rho+Poisson   sorting   MCC   mover'
   print '(a,4f12.4)', ' <t> (ms): ', 1.d3*t_field/nsteps, 1.d3*t_sort/nsteps, 1.d3*t_mcc/nsteps, &
        1.d3*t_move/nsteps

@@ -291,6 +291,58 @@ void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const doub
   }
 }
 
+/* eheating, e_heating.f90:1-112, one partition: electrons (species 1) in
+ * the heating region (slit flag_c=0 or disk flag_c=1) are heated with
+ * probability nu*dt per call, taking (eheat_type=1) or adding a Maxwellian
+ * of thermal speed vt(i_icp); returns the change of P_loss(2). */
+double lo_eheating(const lo_eheat *e, double *vxp, long long np_tot, int32_t *iseed) {
+  const double dt = e->dt, dr = e->dr;
+  double vz_sav = 0.0;
+  double nu_tmp = e->nudt / dt;
+  double dP = 0.0;
+  for (long long i = 0; i < np_tot; ++i) {
+    double *v = vxp + 6 * i;
+    const double xp = v[0], yp = v[1];
+    int flag_eh = 0;
+    double rp2;
+    if (e->flag_c == 0) {
+      rp2 = (xp - e->xa) * (xp - e->xa);
+      if (rp2 <= dr * dr && yp >= e->yl_pow && yp <= e->yr_pow) flag_eh = 1;
+    } else {
+      rp2 = (xp - e->xa) * (xp - e->xa) + (yp - e->ymax / 2.0) * (yp - e->ymax / 2.0);
+      if (rp2 <= dr * dr) flag_eh = 1;
+    }
+    if (!flag_eh) continue;
+    if (e->flag_gpp == 1) nu_tmp = (e->nudt / dt) * 9.0 * e->Pabs_cor * exp(-9.0 * rp2 / (dr * dr));
+    double rnd[2];
+    rnd[0] = lo_ran2(iseed);
+    if (rnd[0] <= nu_tmp * dt) {
+      const double v2old = v[3] * v[3] + v[4] * v[4] + v[5] * v[5];
+      const int i_icp = (int)(e->n_icp * yp / e->ymax) + 1;
+      const double vt = e->vt[i_icp - 1];
+      double dvx, dvy, dvz;
+      rnd[0] = lo_ran2(iseed);
+      rnd[1] = lo_ran2(iseed);
+      lo_load_gauss(&dvx, &dvy, vt, rnd, e->pi);
+      if (vz_sav == 0.0) {
+        rnd[0] = lo_ran2(iseed);
+        rnd[1] = lo_ran2(iseed);
+        lo_load_gauss(&dvz, &vz_sav, vt, rnd, e->pi);
+      } else {
+        dvz = vz_sav;
+        vz_sav = 0.0;
+      }
+      if (e->eheat_type == 1) {
+        v[3] = dvx; v[4] = dvy; v[5] = dvz;
+      } else {
+        v[3] = v[3] + dvx; v[4] = v[4] + dvy; v[5] = v[5] + dvz;
+      }
+      dP = dP + 0.5 * e->Nm * e->mass * (v[3] * v[3] + v[4] * v[4] + v[5] * v[5] - v2old);
+    }
+  }
+  return dP;
+}
+
 /* calc_rho.f90:37-77, one partition (nproc=1, nproc_mpi=1). */
 void lo_calc_rhs(int n1, int n2, int ntype, const double *charge,
                  const double *const *np, double *rhs) {

@@ -20,6 +20,8 @@
  *   lo_calc_efield     oracle/_ref calc_Efield (bit-exact); quadratic phi.
  *   lo_part_sorting    oracle/_ref part_sorting (bit-exact); stable sort.
  *   lo_calc_rhs, lo_dens_red   closed forms (sums / maxima).
+ *   lo_eheating        oracle/_ref eheating (bit-exact, e_heating.f90 has no
+ *                      MPI); heated fraction and Maxwellian energy.
  *   lo_part_moments, lo_calc_avg   closed forms: single-particle weights,
  *                      cold beams (Tp = 0, j/n = beam velocity), Maxwellian
  *                      temperature within its sampling error, drift
@@ -103,6 +105,13 @@ void lo_charge_deposition(const lo_domain *d, const lo_species *sp,
 void lo_kq(const lo_domain *d, double *kq);
 void lo_calc_rhs(int n1, int n2, int ntype, const double *charge,
                  const double *const *np, double *rhs);
+/* electron heating parameters (particle_info.h COMMON members of e_heating.f90) */
+typedef struct {
+  int flag_c, flag_gpp, eheat_type, n_icp;
+  double xa, dr, yl_pow, yr_pow, ymax, nudt, dt, Pabs_cor, Nm, mass, pi;
+  const double *vt;      /* n_icp thermal speeds */
+} lo_eheat;
+double lo_eheating(const lo_eheat *e, double *vxp, long long np_tot, int32_t *iseed);
 void lo_part_moments(const lo_domain *d, const lo_species *sp, const double *vxp,
                      long long np_tot, const double *kq, int size_na, double *p_mts);
 void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const double *const *np,

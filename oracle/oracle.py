@@ -67,6 +67,13 @@ class LoMoverTally(C.Structure):
     _fields_ = [("P_loss", C.c_double * 4), ("cnt_dead", C.c_double), ("np_lost", C.c_longlong)]
 
 
+class LoEheat(C.Structure):
+    _fields_ = [("flag_c", C.c_int), ("flag_gpp", C.c_int), ("eheat_type", C.c_int), ("n_icp", C.c_int),
+                ("xa", C.c_double), ("dr", C.c_double), ("yl_pow", C.c_double), ("yr_pow", C.c_double),
+                ("ymax", C.c_double), ("nudt", C.c_double), ("dt", C.c_double), ("Pabs_cor", C.c_double),
+                ("Nm", C.c_double), ("mass", C.c_double), ("pi", C.c_double), ("vt", _dp)]
+
+
 class LoPoisson(C.Structure):
     _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("ng", C.c_int), ("h1", C.c_double),
                 ("h2", C.c_double), ("eps0", C.c_double), ("bcnd", _ip)]
@@ -116,6 +123,8 @@ def _declare(L):
     L.lo_ran2.restype = C.c_double
     L.lo_ran2.argtypes = [_ip]
     L.lo_load_gauss.argtypes = [_dp, _dp, C.c_double, _dp, C.c_double]
+    L.lo_eheating.restype = C.c_double
+    L.lo_eheating.argtypes = [C.POINTER(LoEheat), _dp, C.c_longlong, _ip]
     L.lo_part_moments.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, C.c_longlong, _dp, C.c_int, _dp]
     L.lo_calc_avg.argtypes = [C.POINTER(LoDomain), C.c_int, C.POINTER(LoSpecies), C.POINTER(_dp), C.POINTER(_dp),
                               C.POINTER(_dp), C.c_int, _dp]
@@ -245,6 +254,20 @@ def charge_deposition(dom: Domain, Nm, vxp, kq_arr=None):
     sp = LoSpecies(0.0, 1.0, Nm, 0)
     lib().lo_charge_deposition(C.byref(d), C.byref(sp), _p(vxp), vxp.shape[0], _p(k), _p(out))
     return out
+
+
+def eheating(vxp, heat, iseed):
+    """e_heating.f90:1-112 (lo_eheating).  heat: dict of flag_c, flag_gpp,
+    eheat_type, xa, dr, yl_pow, yr_pow, ymax, nudt, dt, Pabs_cor, Nm, mass,
+    vt (array of n_icp).  Returns (vxp, dP_loss2, iseed)."""
+    v = np.array(vxp, dtype=np.float64).reshape(-1, 6)
+    vt = np.ascontiguousarray(heat["vt"], dtype=np.float64)
+    e = LoEheat(heat["flag_c"], heat["flag_gpp"], heat["eheat_type"], vt.size, heat["xa"], heat["dr"],
+                heat["yl_pow"], heat["yr_pow"], heat["ymax"], heat["nudt"], heat["dt"], heat["Pabs_cor"],
+                heat["Nm"], heat["mass"], PI, _p(vt))
+    s = C.c_int32(iseed)
+    dP = lib().lo_eheating(C.byref(e), _p(v), v.shape[0], C.byref(s))
+    return v, dP, s.value
 
 
 def part_moments(dom: Domain, Nm, vxp, size_na=1, kq_arr=None):

@@ -8,7 +8,7 @@
  * Usage: ref_harness <cmd> <in.bin> <out.bin>
  *   in.bin  = int64 ni, int64[ni], int64 nd, double[nd], then raw arrays
  *   out.bin = raw arrays (order per command, see tests/refharness.py)
- * Commands: ran2, gauss, mover, deposit, efield, sort, bench (the timed
+ * Commands: ran2, gauss, mover, deposit, efield, sort, heat, bench (the timed
  * push + deposition loop of bench.py's CPU baseline).
  *
  * The routines take every argument by reference (implicit interfaces) and
@@ -73,6 +73,9 @@ void charge_deposition_(int32_t *n, double *h, double *vxp, int32_t *nmax, int32
                         double *sum_dEk, int32_t *Nh, int32_t *iproc, int32_t *ptype,
                         int32_t *n_icp);
 void calc_efield_(int32_t *n, double *h, double *phi, double *Ei, int32_t *bcnd);
+void eheating_(double *vxp, int32_t *nmax, int32_t *ntype, int32_t *nproc, int32_t *iseed,
+               int32_t *np_tot, double *vt, int32_t *n_icp, int32_t *iproc, double *P_loss,
+               double *Pabs_cor);
 void part_sorting_(double *vxp, double *sorting, int32_t *nmax, int32_t *ntype, int32_t *n,
                    double *h, int32_t *Plist_thd, int32_t *pl_max, int32_t *nproc,
                    int32_t *np_tot, int32_t *ptype);
@@ -170,6 +173,32 @@ int main(int argc, char **argv) {
     wr(&cnt_dead, 8);
     wr(ss2D, 16 * nodes);
     wr(fd, (size_t)nmax);
+  } else if (!strcmp(cmd, "heat")) {
+    /* I: N n_icp flag_c flag_gpp eheat_type seed
+     * D: qe amu xa dr yl_pow yr_pow ymax nudt dt Pabs_cor Nm mass; arrays: vxp, vt */
+    set_constants();
+    int32_t N = (int32_t)I[0], nmax = N > 0 ? N : 1, n_icp = (int32_t)I[1];
+    part_info_int_.flag_c = (int32_t)I[2];
+    part_info_int_.flag_gpp = (int32_t)I[3];
+    part_info_int_.eheat_type = (int32_t)I[4];
+    int32_t iseed = (int32_t)I[5];
+    part_info_dp_2_.xa = D[2];
+    part_info_dp_2_.dr = D[3];
+    part_info_dp_1_.yl_pow = D[4];
+    part_info_dp_1_.yr_pow = D[5];
+    part_info_dp_2_.ymax = D[6];
+    part_info_dp_2_.nudt = D[7];
+    part_info_dp_2_.dt = D[8];
+    double Pabs_cor = D[9];
+    part_info_dp_2_.Nm[0] = D[10];
+    part_info_dp_1_.mass[0] = D[11];
+    double *vxp = rd(48 * (size_t)nmax), *vt = rd(8 * (size_t)n_icp);
+    double P_loss[4] = {0, 0, 0, 0};
+    int32_t ntype = 1, nproc = 1, iproc = 1, np_tot = N;
+    eheating_(vxp, &nmax, &ntype, &nproc, &iseed, &np_tot, vt, &n_icp, &iproc, P_loss, &Pabs_cor);
+    wr(vxp, 48 * (size_t)N);
+    wr(&P_loss[1], 8);
+    wr(&iseed, 4);
   } else if (!strcmp(cmd, "bench")) {
     /* I: n1 n2 nB1 nB2 N ngrid flag_B flag_gridB flag_B_pos tag_neg reps
      * D: as "mover"; arrays: Ei, Bi, bcnd, vxp, flag_dead, phi, kq.

@@ -98,3 +98,16 @@ def part_sorting(n1, n2, h, vxp):
     npt = int(r.take(np.int64, 1)[0])
     v = r.take(np.float64, 6 * npt).reshape(-1, 6)
     return v, r.take(np.int32, n1 + n1 * n2 + 2)
+
+
+def eheating(vxp, heat, iseed):
+    """The reference's eheating (e_heating.f90) on one partition."""
+    vxp = np.ascontiguousarray(vxp, np.float64).reshape(-1, 6)
+    vt = np.ascontiguousarray(heat["vt"], np.float64)
+    ints = [vxp.shape[0], vt.size, heat["flag_c"], heat["flag_gpp"], heat["eheat_type"], iseed]
+    dbls = [QE, AMU, heat["xa"], heat["dr"], heat["yl_pow"], heat["yr_pow"], heat["ymax"], heat["nudt"],
+            heat["dt"], heat["Pabs_cor"], heat["Nm"], heat["mass"]]
+    r = _Reader(_run("heat", ints, dbls, [vxp, vt]))
+    v = r.take(np.float64, 6 * vxp.shape[0]).reshape(-1, 6)
+    dP = float(r.take(np.float64, 1)[0])
+    return v, dP, int(r.take(np.int32, 1)[0])
