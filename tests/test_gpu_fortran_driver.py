@@ -47,3 +47,22 @@ def test_fortran_driver_backup_restart_and_moments(tmp_path):
     assert out2.returncode == 0, out2.stdout + out2.stderr
     counts2 = [l for l in out2.stdout.splitlines() if "particles:" in l][0].split()
     assert abs(int(counts2[2]) - counts[0, 0]) < 0.01 * counts[0, 0]
+
+
+@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
+@pytest.mark.parametrize("precision,mode", [(0, 0), (1, 1)])
+def test_fortran_driver_heating(tmp_path, precision, mode):
+    """flag_heat = 1: the driver calls eheating every ns_heat steps before
+    the push (check mode on the FP64 state, production mode on the mixed
+    one); eheat_type = 1 replaces heated velocities by 3 eV Maxwellian draws
+    from 3 eV electrons, so the tally is small next to eheat_type = 0, where
+    every kick adds energy on average."""
+    tallies = []
+    for etype in (0, 1):
+        (tmp_path / "lepic.nml").write_text(
+            f"&lepic n=64, ppc=10, nsteps=6, ng=6, precision={precision}, mcc_mode={mode}, flag_heat=1, "
+            f"ns_heat=2, flag_c=1, eheat_type={etype}, nudt=0.2 /\n")
+        out = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stdout + out.stderr
+        tallies.append(float([l for l in out.stdout.splitlines() if "heating P_loss" in l][0].split()[-1]))
+    assert tallies[0] > 0.0 and abs(tallies[1]) < 0.3 * tallies[0]
