@@ -52,7 +52,8 @@ enum {
   LEPIC_GRID_E = 2,     /* Ei(2,0:n1+2,0:n2+2) */
   LEPIC_GRID_NP = 3,    /* np(0:n1+2,0:n2+2,ptype): deposited density */
   LEPIC_GRID_NPRED = 4, /* np_red(0:n1+2,0:n2+2,ptype) */
-  LEPIC_GRID_SINK = 5   /* ss2D(2,:,:,ptype): wall-loss sink counts */
+  LEPIC_GRID_SINK = 5,  /* ss2D(2,:,:,ptype): sink counts (wall losses, killed reactants) */
+  LEPIC_GRID_SRC = 6    /* ss2D(1,:,:,ptype): source counts (created, injected, emitted) */
 };
 
 typedef struct {
@@ -278,10 +279,11 @@ int lepic_net_loss_grad(lepic_ctx *ctx, const float *x, const float *target, con
  * species' density, temperature, current densities and sink map to
  * data_pavg(7,na1+1,na2+1,ntype) = [n, Tp, jx, jy, jz, source, sink], phi
  * and E to phi_avg(3,0:n1+2,0:n2+2), counts the call (cnt_avg(1)) and
- * clears the sink maps, as the reference does every navg steps.  Moments
- * use the particles' half-step-back positions and the density np of the
- * last deposition.  The source map stays zero (nothing on this path
- * fills it).  size_na must divide n1 and n2 and stay the same until a reset. */
+ * clears the source and sink maps ss2D (LEPIC_GRID_SRC, LEPIC_GRID_SINK:
+ * counts of created/injected/emitted and of lost/killed particles), as the
+ * reference does every navg steps.  Moments use the particles' half-step-
+ * back positions and the density np of the last deposition.  size_na must
+ * divide n1 and n2 and stay the same until a reset. */
 int lepic_moments_accumulate(lepic_ctx *ctx, int32_t size_na);
 int lepic_moments_get(lepic_ctx *ctx, double *data_pavg, double *phi_avg, int32_t *count);
 int lepic_moments_reset(lepic_ctx *ctx);

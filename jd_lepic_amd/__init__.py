@@ -14,7 +14,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import capi
-from .capi import (FP64, GRID_E, GRID_NP, GRID_NPRED, GRID_PHI, GRID_RHS, GRID_SINK, MCC_CHECK,
+from .capi import (FP64, GRID_E, GRID_NP, GRID_NPRED, GRID_PHI, GRID_RHS, GRID_SINK, GRID_SRC, MCC_CHECK,
                    MCC_PROD, MIXED, START_COLD, START_GIVEN, START_NET, START_PREV)
 from .reactions import AMU, ME, QE
 
@@ -22,7 +22,7 @@ EPS0 = 8.854187817e-12   # main.f90:122
 
 __all__ = ["Context", "LepicError", "Species", "FP64", "MIXED", "MCC_CHECK", "MCC_PROD",
            "START_PREV", "START_COLD", "START_GIVEN", "START_NET", "GRID_PHI", "GRID_RHS", "GRID_E", "GRID_NP",
-           "GRID_NPRED", "GRID_SINK", "QE", "AMU", "ME", "EPS0", "backup_write", "backup_header",
+           "GRID_NPRED", "GRID_SINK", "GRID_SRC", "QE", "AMU", "ME", "EPS0", "backup_write", "backup_header",
            "backup_read_species"]
 
 

@@ -16,7 +16,7 @@ LEPIC_ERR_CAPACITY, LEPIC_ERR_DIVERGED, LEPIC_ERR_MCC, LEPIC_ERR_STATE = -4, -5,
 FP64, MIXED = 0, 1
 MCC_CHECK, MCC_PROD = 0, 1
 START_PREV, START_COLD, START_GIVEN, START_NET = 0, 1, 2, 3
-GRID_PHI, GRID_RHS, GRID_E, GRID_NP, GRID_NPRED, GRID_SINK = range(6)
+GRID_PHI, GRID_RHS, GRID_E, GRID_NP, GRID_NPRED, GRID_SINK, GRID_SRC = range(7)
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -295,6 +295,8 @@ int alloc_species(lepic_ctx* ctx, int p) {
   LEPIC_HIP(hipMemsetAsync(s.np_red, 0, ctx->nodes * 8, ctx->stream));
   LEPIC_HIP(hipMalloc(&s.sink, ctx->nodes * 8));
   LEPIC_HIP(hipMemsetAsync(s.sink, 0, ctx->nodes * 8, ctx->stream));
+  LEPIC_HIP(hipMalloc(&s.src, ctx->nodes * 8));
+  LEPIC_HIP(hipMemsetAsync(s.src, 0, ctx->nodes * 8, ctx->stream));
   return 0;
 }
 
@@ -394,7 +396,7 @@ int lepic_destroy(lepic_ctx* ctx) {
   for (int p = 0; p < ctx->nsp; ++p) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);
-    (void)hipFree(s.flag_dead); (void)hipFree(s.np); (void)hipFree(s.np_red); (void)hipFree(s.sink);
+    (void)hipFree(s.flag_dead); (void)hipFree(s.np); (void)hipFree(s.np_red); (void)hipFree(s.sink); (void)hipFree(s.src);
     (void)hipFree(s.seg[0]); (void)hipFree(s.seg[1]);
   }
   void* ptrs[] = {ctx->bc, ctx->phi, ctx->phi_given, ctx->rhs, ctx->E, ctx->Ef, ctx->Bi, ctx->Bf,
@@ -657,9 +659,12 @@ static int grid_ptr(lepic_ctx* ctx, int32_t which, int32_t ptype, double** p, in
     case LEPIC_GRID_NP:
     case LEPIC_GRID_NPRED:
     case LEPIC_GRID_SINK:
+    case LEPIC_GRID_SRC:
       if (!valid_ptype(ctx, ptype)) return LEPIC_ERR_ARG;
-      *p = which == LEPIC_GRID_NP ? ctx->sp[ptype - 1].np
-           : which == LEPIC_GRID_NPRED ? ctx->sp[ptype - 1].np_red : ctx->sp[ptype - 1].sink;
+      *p = which == LEPIC_GRID_NP      ? ctx->sp[ptype - 1].np
+           : which == LEPIC_GRID_NPRED ? ctx->sp[ptype - 1].np_red
+           : which == LEPIC_GRID_SINK  ? ctx->sp[ptype - 1].sink
+                                       : ctx->sp[ptype - 1].src;
       *n = ctx->nodes;
       return 0;
   }
@@ -726,8 +731,10 @@ int lepic_reset_tallies(lepic_ctx* ctx) {
   if (ctx->rx.d_cnt_col)
     if (int rc = zero_async(ctx, ctx->rx.d_cnt_col, 3 * (size_t)ctx->rx.ncol * std::max(ctx->rx.n_rg, 1) * 8)) return rc;
   if (int rc = zero_async(ctx, ctx->d_mcc_f64, 64 * 8)) return rc;
-  for (int p = 0; p < ctx->nsp; ++p)
+  for (int p = 0; p < ctx->nsp; ++p) {
     if (int rc = zero_async(ctx, ctx->sp[p].sink, ctx->nodes * 8)) return rc;
+    if (int rc = zero_async(ctx, ctx->sp[p].src, ctx->nodes * 8)) return rc;
+  }
   return 0;
 }
 

@@ -86,7 +86,8 @@ struct Species {
   int8_t* flag_dead = nullptr;
   double* np = nullptr;              // deposited density (nodes)
   double* np_red = nullptr;
-  double* sink = nullptr;            // wall-loss sink counts (nodes)
+  double* sink = nullptr;            // ss2D(2): sink counts (nodes)
+  double* src = nullptr;             // ss2D(1): source counts (nodes)
 };
 
 // Tally block per species on the device.

@@ -54,6 +54,8 @@ struct PartPtrs {
   float* f[kMaxSpecies][6];
   int8_t* fd[kMaxSpecies];
   int64_t cap[kMaxSpecies];
+  double* src[kMaxSpecies];      // ss2D(1), ss2D(2)
+  double* sink[kMaxSpecies];
 };
 
 PartPtrs part_ptrs(lepic_ctx* ctx) {
@@ -63,6 +65,8 @@ PartPtrs part_ptrs(lepic_ctx* ctx) {
     for (int k = 0; k < 6; ++k) { P.d[p][k] = s.d[s.cur][k]; P.f[p][k] = s.f[s.cur][k]; }
     P.cell[p] = s.cell[s.cur];
     P.fd[p] = s.flag_dead;
+    P.src[p] = s.src;
+    P.sink[p] = s.sink;
     P.cap[p] = s.cap;
   }
   return P;
@@ -257,6 +261,7 @@ __global__ void k_mcc_check(CheckArgs A) {
             if (ib >= A.P.cap[bq]) { *A.status = -17; return; }
             A.P.d[bq][0][ib] = X[i0]; A.P.d[bq][1][ib] = Y[i0]; A.P.d[bq][2][ib] = A.P.d[q][2][i0];
             A.P.fd[bq][ib] = 0;
+            A.P.src[bq][node_index(ix, iy, A.n1)] += 1.0;   // collisions.f90:809
             v2old = 0.0;
           } else {
             ib = i0;
@@ -279,6 +284,7 @@ __global__ void k_mcc_check(CheckArgs A) {
           if (fa == -1) {
             if (ct == tt || ct != pt) { *A.status = -14; return; }
             A.P.fd[q][i0] = 1;
+            A.P.sink[q][node_index(ix, iy, A.n1)] += 1.0;   // collisions.f90:931
           }
         }
       } else {  // charge exchange, lines 942-1000
@@ -504,10 +510,14 @@ __global__ __launch_bounds__(256) void k_mcc_prod(ProdArgs A) {
             A.st_v[3 * so] = bx; A.st_v[3 * so + 1] = by; A.st_v[3 * so + 2] = bz;
             A.ncreate[(int64_t)(bt - 1) * A.Nc + c] += 1;
             ++slot;
+            atomic_add_f64(&A.P.src[bt - 1][node_index(ix, iy, A.n1)], 1.0);   // collisions.f90:809
             atomicAdd(&s_pl[bt], 0.5 * T.Nm[bt] * T.mass[bt] * (bx * bx + by * by + bz * bz));
           }
         }
-        if (!incident_kept && T.mass[pt] > 0) A.P.fd[q][ip] = 1;  // reactant killed
+        if (!incident_kept && T.mass[pt] > 0) {  // reactant killed
+          A.P.fd[q][ip] = 1;
+          atomic_add_f64(&A.P.sink[q][node_index(ix, iy, A.n1)], 1.0);   // collisions.f90:931
+        }
       } else {
         double o0, o1, o2;
         read_v<kMixed>(A, q, ip, o0, o1, o2);

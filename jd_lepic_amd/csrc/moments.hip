@@ -105,6 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_part_moments(MomArgs a) {
 struct AvgArgs {
   const double* np;              // fine density of the species
   const double* sink;            // ss2D(2) of the species
+  const double* src;             // ss2D(1) of the species
   const double* pm;
   const int8_t* bc;
   int n1, size_na, na1, na2;
@@ -127,7 +128,7 @@ __global__ void k_calc_avg(AvgArgs a) {
   double Tp = 0.0;
   if (n > 0) Tp = a.mass_over_qe * (1.0 / 3.0) * (m[0] / n - (jx * jx + jy * jy + jz * jz) / (n * n));
   if (Tp > 0) o[1] = o[1] + Tp;
-  // o[5], the source map: nothing on this path fills ss2D(1)
+  o[5] = o[5] + a.src[f] * a.k;
   o[6] = o[6] + a.sink[f] * a.k;
 }
 
@@ -202,7 +203,7 @@ int lepic_moments_accumulate(lepic_ctx* ctx, int32_t size_na) {
       LEPIC_HIP(hipGetLastError());
     }
     AvgArgs g{};
-    g.np = s.np; g.sink = s.sink; g.pm = M.pm; g.bc = ctx->bc;
+    g.np = s.np; g.sink = s.sink; g.src = s.src; g.pm = M.pm; g.bc = ctx->bc;
     g.n1 = ctx->g.n1; g.size_na = size_na; g.na1 = na1; g.na2 = na2;
     g.mass_over_qe = s.mass / ctx->g.qe;
     g.k = s.Nm / (ctx->g.h1 * ctx->g.h2 * ctx->g.zmax);
@@ -211,6 +212,7 @@ int lepic_moments_accumulate(lepic_ctx* ctx, int32_t size_na) {
     LEPIC_HIP(hipGetLastError());
     // ss2D = 0 after the average (main.f90:931, plt_src = 0)
     if (int rc = zero_async(ctx, s.sink, ctx->nodes * 8)) return rc;
+    if (int rc = zero_async(ctx, s.src, ctx->nodes * 8)) return rc;
   }
   hipLaunchKernelGGL(k_phi_avg, dim3(nblk(ctx->nodes)), dim3(256), 0, ctx->stream, (const double*)ctx->phi,
                      (const double*)ctx->E, ctx->nodes, M.phi);

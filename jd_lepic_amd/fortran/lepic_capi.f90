@@ -10,7 +10,7 @@ module lepic_capi
   integer(c_int32_t), parameter :: LEPIC_START_PREV = 0, LEPIC_START_COLD = 1, LEPIC_START_GIVEN = 2, &
        LEPIC_START_NET = 3
   integer(c_int32_t), parameter :: LEPIC_GRID_PHI = 0, LEPIC_GRID_RHS = 1, LEPIC_GRID_E = 2, &
-       LEPIC_GRID_NP = 3, LEPIC_GRID_NPRED = 4, LEPIC_GRID_SINK = 5
+       LEPIC_GRID_NP = 3, LEPIC_GRID_NPRED = 4, LEPIC_GRID_SINK = 5, LEPIC_GRID_SRC = 6
 
   type, bind(C) :: lepic_grid_desc
      integer(c_int32_t) :: n1, n2

@@ -259,12 +259,13 @@ void lo_part_moments(const lo_domain *d, const lo_species *sp, const double *vxp
 }
 
 /* calc_avg, utils.f90:138-265, one partition and rank, plt_src = 0.
- * np: node arrays per species; sink: ss2D(2,...) node arrays per species
- * (ss2D(1,...), the source map, is zero: nothing on this path fills it);
+ * np: node arrays per species; src, sink: ss2D(1,...), ss2D(2,...) node
+ * arrays per species (either may be NULL: zero);
  * data_pavg(7,na1+1,na2+1,ntype) accumulates:
  * [(p*(na2+1) + iy-1)*(na1+1) + ix-1]*7 + {0 n, 1 Tp, 2..4 j, 5 source, 6 sink}. */
 void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const double *const *np,
-                 const double *const *p_mts, const double *const *sink, int size_na, double *data_pavg) {
+                 const double *const *p_mts, const double *const *src, const double *const *sink,
+                 int size_na, double *data_pavg) {
   const int n1 = d->n1;
   const int na1 = d->n1 / size_na, na2 = d->n2 / size_na;
   const double Vc = d->h1 * d->h2 * d->zmax;
@@ -285,7 +286,7 @@ void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const doub
         if (np_tmp > 0)
           Tp = sp[p].mass / d->qe * (1.0 / 3.0) * (m[0] / np_tmp - (jx * jx + jy * jy + jz * jz) / (np_tmp * np_tmp));
         if (Tp > 0) o[1] = o[1] + Tp;
-        /* o[5], the source map, gets nothing here */
+        o[5] = o[5] + (src ? src[p][NODE(ix_f, iy_f, n1)] * k : 0.0);
         o[6] = o[6] + (sink ? sink[p][NODE(ix_f, iy_f, n1)] * k : 0.0);
       }
   }
@@ -907,6 +908,7 @@ int lo_collisions(const lo_domain *d, const lo_mcc_setup *s,
             const double *src = vxp + 6 * (ip - 1);
             dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
             flag_add[n_re + iby] = 1;
+            if (s->plt_src == 0 && p->src[bt]) p->src[bt][NODE(ix, iy, n1)] += 1.0;   /* line 809 */
             vb = dst;
             v2old = 0.0;
           } else {
@@ -930,6 +932,7 @@ int lo_collisions(const lo_domain *d, const lo_mcc_setup *s,
           if (flag_add[ire] == -1) {
             if (ct == tt) return -14;
             p->flag_dead[ct][ip - 1] = 1;
+            if (s->plt_src == 0 && p->sink[ct]) p->sink[ct][NODE(ix, iy, n1)] += 1.0;   /* line 931 */
           }
         }
       } else { /* charge exchange with a neutral target, lines 942-1000 */

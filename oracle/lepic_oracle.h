@@ -115,7 +115,8 @@ double lo_eheating(const lo_eheat *e, double *vxp, long long np_tot, int32_t *is
 void lo_part_moments(const lo_domain *d, const lo_species *sp, const double *vxp,
                      long long np_tot, const double *kq, int size_na, double *p_mts);
 void lo_calc_avg(const lo_domain *d, int ntype, const lo_species *sp, const double *const *np,
-                 const double *const *p_mts, const double *const *sink, int size_na, double *data_pavg);
+                 const double *const *p_mts, const double *const *src, const double *const *sink,
+                 int size_na, double *data_pavg);
 void lo_dens_red(int n1, int n2, int ntype, const int32_t *bcnd,
                  const double *const *np, double *const *np_red, double *np_mx);
 void lo_calc_efield(int n1, int n2, const double h[2], const double *phi,
@@ -196,6 +197,8 @@ typedef struct {
   int8_t *flag_dead[LO_MAX_TYPES + 1];
   long long np_tot[LO_MAX_TYPES + 1];
   long long cap[LO_MAX_TYPES + 1];
+  double *src[LO_MAX_TYPES + 1];     /* ss2D(1,...) / ss2D(2,...) node arrays, or NULL */
+  double *sink[LO_MAX_TYPES + 1];
 } lo_particles;
 
 typedef struct {

@@ -109,7 +109,8 @@ class LoMccSetup(C.Structure):
 
 class LoParticles(C.Structure):
     _fields_ = [("vxp", _dp * (MAX_TYPES + 1)), ("flag_dead", _bp * (MAX_TYPES + 1)),
-                ("np_tot", C.c_longlong * (MAX_TYPES + 1)), ("cap", C.c_longlong * (MAX_TYPES + 1))]
+                ("np_tot", C.c_longlong * (MAX_TYPES + 1)), ("cap", C.c_longlong * (MAX_TYPES + 1)),
+                ("src", _dp * (MAX_TYPES + 1)), ("sink", _dp * (MAX_TYPES + 1))]
 
 
 class LoMccOut(C.Structure):
@@ -127,7 +128,7 @@ def _declare(L):
     L.lo_eheating.argtypes = [C.POINTER(LoEheat), _dp, C.c_longlong, _ip]
     L.lo_part_moments.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, C.c_longlong, _dp, C.c_int, _dp]
     L.lo_calc_avg.argtypes = [C.POINTER(LoDomain), C.c_int, C.POINTER(LoSpecies), C.POINTER(_dp), C.POINTER(_dp),
-                              C.POINTER(_dp), C.c_int, _dp]
+                              C.POINTER(_dp), C.POINTER(_dp), C.c_int, _dp]
     L.lo_part_mover.restype = C.c_int
     L.lo_part_mover.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, _dp, _bp,
                                 C.POINTER(C.c_longlong), C.POINTER(LoMoverTally), _dp, _dp]
@@ -283,18 +284,21 @@ def part_moments(dom: Domain, Nm, vxp, size_na=1, kq_arr=None):
     return out
 
 
-def calc_avg(dom: Domain, masses, Nms, nps, p_mts, sinks, size_na=1, data_pavg=None):
+def calc_avg(dom: Domain, masses, Nms, nps, p_mts, sinks, size_na=1, data_pavg=None, srcs=None):
     """utils.f90:138-265 (lo_calc_avg): adds into data_pavg, shaped
-    (ntype, na2+1, na1+1, 7) = [n, Tp, jx, jy, jz, source, sink]."""
+    (ntype, na2+1, na1+1, 7) = [n, Tp, jx, jy, jz, source, sink]; srcs
+    (the ss2D(1) maps) default to zero."""
     ntype = len(nps)
     na1, na2 = dom.n1 // size_na, dom.n2 // size_na
     out = np.zeros((ntype, na2 + 1, na1 + 1, 7)) if data_pavg is None else data_pavg
     sps = (LoSpecies * ntype)(*[LoSpecies(0.0, m, w, 0) for m, w in zip(masses, Nms)])
     arr = lambda xs: [np.ascontiguousarray(a, dtype=np.float64) for a in xs]
     a_np, a_m, a_s = arr(nps), arr(p_mts), arr(sinks)
+    a_src = arr(srcs) if srcs is not None else None
     ptr = lambda xs: (_dp * ntype)(*[_p(a) for a in xs])
     d = dom.struct()
-    lib().lo_calc_avg(C.byref(d), ntype, sps, ptr(a_np), ptr(a_m), ptr(a_s), size_na, _p(out))
+    lib().lo_calc_avg(C.byref(d), ntype, sps, ptr(a_np), ptr(a_m), ptr(a_src) if a_src is not None else None,
+                      ptr(a_s), size_na, _p(out))
     return out
 
 
@@ -445,6 +449,10 @@ def collisions(dom: Domain, setup: dict, table: dict, parts: dict, np_red, iseed
         P.flag_dead[p] = _p(d["flag_dead"], _bp)
         P.np_tot[p] = d["np_tot"]
         P.cap[p] = d["vxp"].shape[0]
+        if "src" in d:
+            P.src[p] = _p(d["src"])
+        if "sink" in d:
+            P.sink[p] = _p(d["sink"])
     ncol = table["struct"].ncol
     cnt = np.zeros(3 * ncol * max(setup["n_rg"], 1))
     log = np.zeros(2 * max(log_cap, 1), np.int32)

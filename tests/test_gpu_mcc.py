@@ -39,7 +39,16 @@ def test_check_mode_replays_oracle(Tgas):
         ge, gi = ctx.download(1), ctx.download(2)
         cc = ctx.cnt_col()
         pl = ctx.mcc_ploss()
+        gsrc = [ctx.get_grid(L.GRID_SRC, p) for p in (1, 2)]
+        gsink = [ctx.get_grid(L.GRID_SINK, p) for p in (1, 2)]
+    for p in (1, 2):
+        parts[p]["src"] = np.zeros((dom.n2 + 3, dom.n1 + 3))
+        parts[p]["sink"] = np.zeros((dom.n2 + 3, dom.n1 + 3))
     ref = O.collisions(dom, setup, tab, parts, np_red, 123456)
+    # ss2D(1)/(2): a count per created / killed particle at the incident cell
+    for p in (1, 2):
+        assert np.array_equal(gsrc[p - 1], parts[p]["src"]) and np.array_equal(gsink[p - 1], parts[p]["sink"])
+    assert parts[2]["src"].sum() == parts[2]["np_tot"] - 20000 > 0   # one count per created Ar+
     assert out["seed"] == ref["seed"]
     assert out["Nc"][0] == ref["Nc"][1]
     assert np.array_equal(out["events"], ref["events"]) and ref["events"].sum() > 0

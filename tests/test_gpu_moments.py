@@ -37,6 +37,7 @@ def test_moments_match_oracle(prec, size_na):
         ctx.iterative_solver(eps=1e-10, start=L.START_COLD)
         ctx.calc_efield()
         phi, E = ctx.get_grid(L.GRID_PHI).copy(), ctx.get_grid(L.GRID_E).copy()
+        rng = np.random.default_rng(size_na)
         for rep in range(2):
             for p in (1, 2):
                 ctx.part_mover(p, deposit=True)
@@ -45,10 +46,15 @@ def test_moments_match_oracle(prec, size_na):
             sinks = [ctx.get_grid(L.GRID_SINK, p).copy() for p in (1, 2)]
             if rep == 0:
                 assert sinks[0].sum() > 0         # the case exercises the sink map
+            # source maps as injection/emission/ionization would leave them
+            srcs = [np.floor(rng.uniform(0, 3, nps[0].shape)) for _ in (1, 2)]
+            for p in (1, 2):
+                ctx.set_grid(L.GRID_SRC, srcs[p - 1], p)
             pm = [O.part_moments(dom, Nms[p], parts[p], size_na) for p in range(2)]
-            want = O.calc_avg(dom, masses, Nms, nps, pm, sinks, size_na, want)
+            want = O.calc_avg(dom, masses, Nms, nps, pm, sinks, size_na, want, srcs=srcs)
             ctx.moments_accumulate(size_na)
             assert ctx.get_grid(L.GRID_SINK, 1).sum() == 0.0   # cleared after the average
+            assert ctx.get_grid(L.GRID_SRC, 2).sum() == 0.0
         got, phi_avg, count = ctx.moments()
     assert count == 2
     np.testing.assert_array_equal(phi_avg[..., 0], 2 * phi)

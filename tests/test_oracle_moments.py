@@ -90,9 +90,13 @@ def test_particles_outside_are_skipped_and_sink_scaled():
     assert np.count_nonzero(m[..., 1]) == 4      # only the second particle
     sink = np.zeros((N + 3, N + 3))
     sink[4, 4] = 3.0
-    avg = O.calc_avg(d, [O.ME], [2.0], [np.zeros_like(sink)], [m], [sink])
+    src = np.zeros((N + 3, N + 3))
+    src[6, 2] = 5.0
+    avg = O.calc_avg(d, [O.ME], [2.0], [np.zeros_like(sink)], [m], [sink], srcs=[src])
     assert avg[0, 3, 3, 6] == 3.0 * 2.0 / (H * H)   # ss2D * Nm/Vc at node (4,4)
     assert np.count_nonzero(avg[0, ..., 6]) == 1
+    assert avg[0, 5, 1, 5] == 5.0 * 2.0 / (H * H)   # the source column, node (2,6)
+    assert np.count_nonzero(avg[0, ..., 5]) == 1
 
 
 def test_coarse_grid_sums_fine_moments():
