@@ -779,4 +779,28 @@ int deposit_launch(lepic_ctx* ctx, int p) {
   return 0;
 }
 
+// Adds the charge of segments [seg0, seg0+nseg) of the current buffer to
+// np(:,:,ptype) without clearing it: particles appended after the mover
+// (volume injection, part_injection.f90, runs between the push and the
+// deposition, main.f90:1063-1071) join the step's density this way.
+int deposit_range_launch(lepic_ctx* ctx, int p, int64_t seg0, int64_t nseg) {
+  if (nseg <= 0) return 0;
+  Species& s = ctx->sp[p];
+  MoverCommon a = common_args(ctx, p);
+  a.deposit = 1;
+  a.seg_in = s.seg[s.cur] + seg0;
+  const int b = s.cur;
+  const int64_t o = seg0 * kChunk;
+  if (ctx->g.precision == LEPIC_MIXED) {
+    SoaMx in{s.cell[b] + o, s.f[b][0] + o, s.f[b][1] + o, s.f[b][2] ? s.f[b][2] + o : nullptr,
+             s.f[b][3] + o, s.f[b][4] + o, s.f[b][5] + o};
+    hipLaunchKernelGGL(k_deposit_mixed, dim3((unsigned)nseg), dim3(kBlock), 0, ctx->stream, a, in);
+  } else {
+    Soa64 in{s.d[b][0] + o, s.d[b][1] + o, s.d[b][2] + o, s.d[b][3] + o, s.d[b][4] + o, s.d[b][5] + o};
+    hipLaunchKernelGGL(k_deposit64, dim3((unsigned)nseg), dim3(kBlock), 0, ctx->stream, a, in);
+  }
+  LEPIC_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace lepic

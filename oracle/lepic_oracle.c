@@ -962,3 +962,108 @@ int lo_collisions(const lo_domain *d, const lo_mcc_setup *s,
   }
   return 0;
 }
+
+/* part_flux_emitter, part_flux_emitter.f90:1-217 (part_flux_emitter with
+ * nproc = nproc_mpi = 1, then part_flux_OMP on the one partition): Nh
+ * particles of ptype_fx (tag_b on the chamfer) leave the electrode at xg1
+ * with a flux-Maxwellian normal speed and a Maxwellian tangential pair of
+ * temperature THm, spread over the ns_Hm*dt they travel.  A candidate
+ * whose cell is outside the node range counts as off the electrode (the
+ * reference would read bcnd out of bounds there); more than max_tries
+ * candidates for one particle returns -1 where the reference loops forever.
+ * Returns -2 when the chamfer is longer than ymax (line 39), -17 when a
+ * species overflows its capacity. */
+int lo_flux_emitter(const lo_domain *d, const lo_emitter *e, lo_particles *p, int32_t *iseed,
+                    long long max_tries, lo_emitter_out *o) {
+  const int n1 = d->n1, n2 = d->n2;
+  const double h1 = d->h1, h2 = d->h2, ymax = d->ymax, zmax = d->zmax, dt = d->dt, qe = d->qe;
+  const double pi = e->pi;
+  const int chamfered = e->ptype_fx == e->tag_neg && e->flag_chmfrd == 1;
+  const double js = e->jne * 10.0;
+  double Lgrd = (double)e->n_icp * (e->Lg - e->Lh * (double)e->n_holes);
+  double th_chrd = 0.0, y1 = 0.0;
+  if (chamfered) {
+    const double x1 = e->chmfrd_array[0], x2 = e->chmfrd_array[1];
+    y1 = e->chmfrd_array[2];
+    const double y2 = e->chmfrd_array[3];
+    const double d_chrd = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+    th_chrd = acos((x2 - x1) / d_chrd);
+    Lgrd = 2.0 * (y1 + d_chrd);
+    if (Lgrd > ymax) return -2;
+  }
+  o->Lgrd = Lgrd;
+  const double I_sec = js * zmax * Lgrd;
+  const double dNh = I_sec * ((double)(float)e->ns_Hm * dt) / (e->Nm[e->ptype_fx] * qe) / 1.0 / 1.0;
+  long long Nh = (long long)dNh;
+  if (lo_ran2(iseed) <= dNh - (double)Nh) Nh = Nh + 1;
+  o->dNh = dNh;
+  o->Nh = Nh;
+  const double x_injtmp = e->xg1;
+  long long cnt_lhs = 0, cnt_rhs = 0;
+  const int32_t *bc = d->bcnd;
+  for (long long i = 1; i <= Nh; ++i) {
+    double x_inj, y_inj, z_inj, dir = 0.0;
+    int ptype, ix, iy;
+    long long tries = 0;
+    for (;;) { /* label 90 */
+      if (++tries > max_tries) return -1;
+      x_inj = x_injtmp;
+      const double r = lo_ran2(iseed);
+      if (e->n_icp == 1) y_inj = (ymax - e->Lg) / 2.0 + e->Lg * r;
+      else y_inj = ymax * (1.0 - r);
+      ptype = e->ptype_fx;
+      if (chamfered && ((y_inj <= ymax / 2.0 && y_inj > y1) || (y_inj > ymax / 2.0 && fabs(ymax - y_inj) > y1))) {
+        if (e->tag_b > 0) ptype = e->tag_b;
+        double th = th_chrd, d0 = y_inj - y1, d1 = y1;
+        if (y_inj > ymax / 2.0) {
+          th = -th_chrd;
+          d0 = fabs(y_inj - ymax) - y1;
+          d1 = ymax - y1;
+        }
+        x_inj = x_injtmp + d0 * cos(th);
+        y_inj = d1 + d0 * sin(th);
+      }
+      ix = (int)(x_inj / h1) + 1;
+      iy = (int)(y_inj / h2) + 1;
+      z_inj = zmax * (1.0 - lo_ran2(iseed));
+      if (ix < 1 || ix > n1 + 1 || iy < 1 || iy > n2 + 1) continue;
+      if ((bc[NODE(ix, iy, n1)] >= 1 && bc[NODE(ix, iy + 1, n1)] >= 1) ||
+          (bc[NODE(ix + 1, iy, n1)] >= 1 && bc[NODE(ix + 1, iy + 1, n1)] >= 1)) {
+        if (bc[NODE(ix - 1, iy, n1)] == -1) cnt_lhs = cnt_lhs + 1;
+        if (bc[NODE(ix + 1, iy, n1)] == -1) cnt_rhs = cnt_rhs + 1;
+        if (cnt_lhs != cnt_rhs) {
+          dir = cnt_lhs > cnt_rhs ? 1.0 : -1.0;
+          break;
+        }
+      }
+    }
+    if (ptype == e->tag_b) o->cnt_b[1] += 1;
+    else o->cnt_b[0] += 1;
+    const long long k = p->np_tot[ptype] + 1;
+    if (k > p->cap[ptype]) return -17;
+    p->np_tot[ptype] = k;
+    if (p->src[ptype]) p->src[ptype][NODE(ix, iy, n1)] += 1.0;
+    const double vt = sqrt(2.0 * qe * e->THm / fabs(e->mass[ptype]));
+    const double vx = dir * vt * sqrt(-log(1.0 - lo_ran2(iseed)));
+    double rnd[2], vy, vz;
+    rnd[0] = lo_ran2(iseed);
+    rnd[1] = lo_ran2(iseed);
+    lo_load_gauss(&vy, &vz, vt, rnd, pi);
+    double th = pi / 2.0;
+    if (chamfered && ((y_inj <= ymax / 2.0 && y_inj > y1) || (y_inj > ymax / 2.0 && fabs(ymax - y_inj) > y1))) {
+      th = th_chrd;
+      if (y_inj > ymax / 2.0) th = pi / 2.0 + th_chrd;
+    }
+    double *v = p->vxp[ptype] + 6 * (k - 1);
+    v[3] = -(vx * sin(th)) + vy * cos(th);
+    v[4] = vx * cos(th) + vy * sin(th);
+    v[5] = vz;
+    const double dt_tmp = lo_ran2(iseed) * ((double)(float)e->ns_Hm * dt);
+    v[0] = x_inj + v[3] * dt_tmp;
+    v[1] = y_inj + v[4] * dt_tmp;
+    v[2] = z_inj + v[5] * dt_tmp;
+    o->P_loss4[ptype] = o->P_loss4[ptype] + 0.5 * e->Nm[ptype] * e->mass[ptype] *
+                                                (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+  }
+  return 0;
+}

@@ -22,6 +22,8 @@
  *   lo_calc_rhs, lo_dens_red   closed forms (sums / maxima).
  *   lo_eheating        oracle/_ref eheating (bit-exact, e_heating.f90 has no
  *                      MPI); heated fraction and Maxwellian energy.
+ *   lo_flux_emitter    oracle/_ref part_flux_emitter (bit-exact; the file
+ *                      has no MPI); emitted current, flux-Maxwellian speeds.
  *   lo_part_moments, lo_calc_avg   closed forms: single-particle weights,
  *                      cold beams (Tp = 0, j/n = beam velocity), Maxwellian
  *                      temperature within its sampling error, drift
@@ -214,6 +216,23 @@ typedef struct {
 int lo_collisions(const lo_domain *d, const lo_mcc_setup *s,
                   const lo_reaction_table *t, lo_particles *p,
                   const double *const *np_red, int32_t *iseed, lo_mcc_out *o);
+
+/* surface emitter parameters (particle_info.h COMMON members read by
+ * part_flux_emitter.f90); species arrays are indexed by ptype (1-based) */
+typedef struct {
+  double jne, Lg, Lh, xg1, THm, chmfrd_array[4];
+  int n_holes, n_icp, ptype_fx, tag_neg, tag_b, flag_chmfrd, ns_Hm;
+  double Nm[LO_MAX_TYPES + 1], mass[LO_MAX_TYPES + 1];
+  double pi;
+} lo_emitter;
+typedef struct {
+  long long cnt_b[2];
+  double P_loss4[LO_MAX_TYPES + 1];
+  double Lgrd, dNh;
+  long long Nh;
+} lo_emitter_out;
+int lo_flux_emitter(const lo_domain *d, const lo_emitter *e, lo_particles *p, int32_t *iseed,
+                    long long max_tries, lo_emitter_out *o);
 
 #ifdef __cplusplus
 }

@@ -74,6 +74,19 @@ class LoEheat(C.Structure):
                 ("Nm", C.c_double), ("mass", C.c_double), ("pi", C.c_double), ("vt", _dp)]
 
 
+class LoEmitter(C.Structure):
+    _fields_ = [("jne", C.c_double), ("Lg", C.c_double), ("Lh", C.c_double), ("xg1", C.c_double),
+                ("THm", C.c_double), ("chmfrd_array", C.c_double * 4), ("n_holes", C.c_int), ("n_icp", C.c_int),
+                ("ptype_fx", C.c_int), ("tag_neg", C.c_int), ("tag_b", C.c_int), ("flag_chmfrd", C.c_int),
+                ("ns_Hm", C.c_int), ("Nm", C.c_double * (MAX_TYPES + 1)), ("mass", C.c_double * (MAX_TYPES + 1)),
+                ("pi", C.c_double)]
+
+
+class LoEmitterOut(C.Structure):
+    _fields_ = [("cnt_b", C.c_longlong * 2), ("P_loss4", C.c_double * (MAX_TYPES + 1)), ("Lgrd", C.c_double),
+                ("dNh", C.c_double), ("Nh", C.c_longlong)]
+
+
 class LoPoisson(C.Structure):
     _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("ng", C.c_int), ("h1", C.c_double),
                 ("h2", C.c_double), ("eps0", C.c_double), ("bcnd", _ip)]
@@ -126,6 +139,9 @@ def _declare(L):
     L.lo_load_gauss.argtypes = [_dp, _dp, C.c_double, _dp, C.c_double]
     L.lo_eheating.restype = C.c_double
     L.lo_eheating.argtypes = [C.POINTER(LoEheat), _dp, C.c_longlong, _ip]
+    L.lo_flux_emitter.restype = C.c_int
+    L.lo_flux_emitter.argtypes = [C.POINTER(LoDomain), C.POINTER(LoEmitter), C.POINTER(LoParticles), _ip,
+                                  C.c_longlong, C.POINTER(LoEmitterOut)]
     L.lo_part_moments.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, C.c_longlong, _dp, C.c_int, _dp]
     L.lo_calc_avg.argtypes = [C.POINTER(LoDomain), C.c_int, C.POINTER(LoSpecies), C.POINTER(_dp), C.POINTER(_dp),
                               C.POINTER(_dp), C.POINTER(_dp), C.c_int, _dp]
@@ -269,6 +285,42 @@ def eheating(vxp, heat, iseed):
     s = C.c_int32(iseed)
     dP = lib().lo_eheating(C.byref(e), _p(v), v.shape[0], C.byref(s))
     return v, dP, s.value
+
+
+EMITTER_KEYS = ("jne", "Lg", "Lh", "xg1", "THm", "n_holes", "n_icp", "ptype_fx", "tag_neg", "tag_b",
+                "flag_chmfrd", "ns_Hm")
+
+
+def flux_emitter(dom: Domain, em: dict, parts: dict, iseed: int, max_tries: int = 1 << 20):
+    """part_flux_emitter.f90:1-217 (lo_flux_emitter).  em: the keys of
+    EMITTER_KEYS, chmfrd_array (4), and Nm / mass as {ptype: value}.  parts:
+    {ptype: dict(vxp=(cap,6), np_tot, src=(n2+3,n1+3) optional)}, appended
+    to in place.  Returns dict(rc, seed, cnt_b, P_loss4, Lgrd, dNh, Nh)."""
+    e = LoEmitter()
+    for k in EMITTER_KEYS:
+        setattr(e, k, em[k])
+    for j, v in enumerate(em.get("chmfrd_array", (0.0, 0.0, 0.0, 0.0))):
+        e.chmfrd_array[j] = v
+    for p, v in em["Nm"].items():
+        e.Nm[p] = v
+    for p, v in em["mass"].items():
+        e.mass[p] = v
+    e.pi = PI
+    P = LoParticles()
+    for p, d in parts.items():
+        P.vxp[p] = _p(d["vxp"])
+        P.np_tot[p] = d["np_tot"]
+        P.cap[p] = d["vxp"].shape[0]
+        if "src" in d:
+            P.src[p] = _p(d["src"])
+    o = LoEmitterOut()
+    s = C.c_int32(iseed)
+    d = dom.struct()
+    rc = lib().lo_flux_emitter(C.byref(d), C.byref(e), C.byref(P), C.byref(s), max_tries, C.byref(o))
+    for p, dd in parts.items():
+        dd["np_tot"] = P.np_tot[p]
+    return dict(rc=rc, seed=s.value, cnt_b=np.array(o.cnt_b[:]), P_loss4=np.array(o.P_loss4[:]), Lgrd=o.Lgrd,
+                dNh=o.dNh, Nh=o.Nh)
 
 
 def part_moments(dom: Domain, Nm, vxp, size_na=1, kq_arr=None):

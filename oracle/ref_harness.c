@@ -8,7 +8,7 @@
  * Usage: ref_harness <cmd> <in.bin> <out.bin>
  *   in.bin  = int64 ni, int64[ni], int64 nd, double[nd], then raw arrays
  *   out.bin = raw arrays (order per command, see tests/refharness.py)
- * Commands: ran2, gauss, mover, deposit, efield, sort, heat, bench (the timed
+ * Commands: ran2, gauss, mover, deposit, efield, sort, heat, emit, bench (the timed
  * push + deposition loop of bench.py's CPU baseline).
  *
  * The routines take every argument by reference (implicit interfaces) and
@@ -76,6 +76,11 @@ void calc_efield_(int32_t *n, double *h, double *phi, double *Ei, int32_t *bcnd)
 void eheating_(double *vxp, int32_t *nmax, int32_t *ntype, int32_t *nproc, int32_t *iseed,
                int32_t *np_tot, double *vt, int32_t *n_icp, int32_t *iproc, double *P_loss,
                double *Pabs_cor);
+void part_flux_emitter_(int32_t *istep, double *vxp, int32_t *n, double *h, int32_t *ntype,
+                        int32_t *nmax, int32_t *bcnd, double *ss2D, int32_t *nproc, int32_t *iseed,
+                        int32_t *np_tot, int32_t *nproc_mpi, int32_t *mpi_rank, int32_t *cnt_b,
+                        int32_t *n_icp, double *P_loss);
+void omp_set_num_threads(int);
 void part_sorting_(double *vxp, double *sorting, int32_t *nmax, int32_t *ntype, int32_t *n,
                    double *h, int32_t *Plist_thd, int32_t *pl_max, int32_t *nproc,
                    int32_t *np_tot, int32_t *ptype);
@@ -199,6 +204,51 @@ int main(int argc, char **argv) {
     wr(vxp, 48 * (size_t)N);
     wr(&P_loss[1], 8);
     wr(&iseed, 4);
+  } else if (!strcmp(cmd, "emit")) {
+    /* I: n1 n2 ntype nmax ptype_fx tag_neg tag_b flag_chmfrd n_holes n_icp ns_Hm seed
+     * D: qe amu h1 h2 jne Lg Lh xg1 THm ymax zmax dt chmfrd_array(4) Nm(ntype) mass(ntype)
+     * arrays: bcnd, np_tot(ntype), vxp(6,nmax,ntype).  One OpenMP thread: the
+     * partition index of part_flux_OMP is the thread number. */
+    set_constants();
+    omp_set_num_threads(1);
+    int32_t n[3] = {(int32_t)I[0], (int32_t)I[1], 1};
+    int32_t ntype = (int32_t)I[2], nmax = (int32_t)I[3];
+    part_info_int_.ptype_fx = (int32_t)I[4];
+    part_info_int_.tag_neg = (int32_t)I[5];
+    part_info_int_.tag_b = (int32_t)I[6];
+    part_info_int_.flag_chmfrd = (int32_t)I[7];
+    part_info_int_.n_holes = (int32_t)I[8];
+    int32_t n_icp = (int32_t)I[9];
+    part_info_int_.ns_Hm = (int32_t)I[10];
+    int32_t iseed = (int32_t)I[11];
+    double h[3] = {D[2], D[3], 0.0};
+    part_info_dp_1_.jne = D[4];
+    part_info_dp_2_.Lg = D[5];
+    part_info_dp_2_.Lh = D[6];
+    part_info_dp_2_.xg1 = D[7];
+    part_info_dp_1_.THm = D[8];
+    part_info_dp_2_.ymax = D[9];
+    part_info_dp_2_.zmax = D[10];
+    part_info_dp_2_.dt = D[11];
+    for (int k = 0; k < 4; ++k) part_info_dp_1_.chmfrd_array[k] = D[12 + k];
+    for (int k = 0; k < ntype; ++k) {
+      part_info_dp_2_.Nm[k] = D[16 + k];
+      part_info_dp_1_.mass[k] = D[16 + ntype + k];
+    }
+    const size_t nodes = (size_t)(n[0] + 3) * (n[1] + 3);
+    int32_t *bcnd = rd(4 * nodes), *np_tot = rd(4 * (size_t)ntype);
+    double *vxp = rd(48 * (size_t)nmax * ntype);
+    double *ss2D = calloc(2 * nodes * ntype, 8), *P_loss = calloc(4 * (size_t)ntype, 8);
+    int32_t cnt_b[2] = {0, 0}, nproc = 1, nproc_mpi = 1, mpi_rank = 0, istep = 2;
+    part_flux_emitter_(&istep, vxp, n, h, &ntype, &nmax, bcnd, ss2D, &nproc, &iseed, np_tot, &nproc_mpi,
+                       &mpi_rank, cnt_b, &n_icp, P_loss);
+    wr(vxp, 48 * (size_t)nmax * ntype);
+    wr(np_tot, 4 * (size_t)ntype);
+    wr(ss2D, 16 * nodes * ntype);
+    wr(cnt_b, 8);
+    wr(P_loss, 32 * (size_t)ntype);
+    wr(&iseed, 4);
+    wr(&part_info_dp_2_.Lgrd, 8);
   } else if (!strcmp(cmd, "bench")) {
     /* I: n1 n2 nB1 nB2 N ngrid flag_B flag_gridB flag_B_pos tag_neg reps
      * D: as "mover"; arrays: Ei, Bi, bcnd, vxp, flag_dead, phi, kq.

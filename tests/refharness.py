@@ -100,6 +100,37 @@ def part_sorting(n1, n2, h, vxp):
     return v, r.take(np.int32, n1 + n1 * n2 + 2)
 
 
+def flux_emitter(dom, em, parts, iseed):
+    """The reference's part_flux_emitter (part_flux_emitter.f90) on one
+    partition; parts as for oracle.flux_emitter with species 1..ntype.
+    Returns (parts copy, src maps, cnt_b, P_loss4, iseed, Lgrd)."""
+    ntype = max(parts)
+    nmax = max(d["vxp"].shape[0] for d in parts.values())
+    n1, n2 = dom.n1, dom.n2
+    vxp = np.zeros((ntype, nmax, 6))
+    np_tot = np.zeros(ntype, np.int32)
+    for p, d in parts.items():
+        vxp[p - 1, :d["vxp"].shape[0]] = d["vxp"]
+        np_tot[p - 1] = d["np_tot"]
+    ints = [n1, n2, ntype, nmax, em["ptype_fx"], em["tag_neg"], em["tag_b"], em["flag_chmfrd"], em["n_holes"],
+            em["n_icp"], em["ns_Hm"], iseed]
+    dbls = [QE, AMU, dom.h1, dom.h2, em["jne"], em["Lg"], em["Lh"], em["xg1"], em["THm"], dom.n2 * dom.h2, dom.zmax,
+            dom.dt, *em.get("chmfrd_array", (0.0, 0.0, 0.0, 0.0)),
+            *[em["Nm"].get(p, 0.0) for p in range(1, ntype + 1)],
+            *[em["mass"].get(p, 0.0) for p in range(1, ntype + 1)]]
+    bc = np.ascontiguousarray(dom.bcnd, np.int32)
+    r = _Reader(_run("emit", ints, dbls, [bc, np_tot, vxp]))
+    v = r.take(np.float64, 6 * nmax * ntype).reshape(ntype, nmax, 6)
+    counts = r.take(np.int32, ntype)
+    ss = r.take(np.float64, 2 * (n1 + 3) * (n2 + 3) * ntype).reshape(ntype, n2 + 3, n1 + 3, 2)
+    cnt_b = r.take(np.int32, 2)
+    ploss = r.take(np.float64, 4 * ntype).reshape(ntype, 4)
+    seed = int(r.take(np.int32, 1)[0])
+    Lgrd = float(r.take(np.float64, 1)[0])
+    out = {p: dict(vxp=v[p - 1], np_tot=int(counts[p - 1])) for p in range(1, ntype + 1)}
+    return out, ss[..., 0], cnt_b, ploss[:, 3], seed, Lgrd
+
+
 def eheating(vxp, heat, iseed):
     """The reference's eheating (e_heating.f90) on one partition."""
     vxp = np.ascontiguousarray(vxp, np.float64).reshape(-1, 6)
