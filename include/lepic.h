@@ -213,6 +213,39 @@ typedef struct {
 int lepic_eheating(lepic_ctx *ctx, const lepic_heating *h, int32_t mode, uint64_t seed, int32_t it,
                    int32_t *iseed, double *dP);
 
+/* ---- surface emitter (part_flux_emitter.f90:1-217; called at main.f90:1092-1096
+ *      after the deposition when jne > 0 and (ns_Hm == 1 or it%ns_Hm == 1)) ----
+ * Emits round(js*zmax*Lgrd*ns_Hm*dt/(Nm*qe)) particles of ptype_fx (js =
+ * 10*jne A/m^2, Lgrd = n_icp*(Lg - Lh*n_holes), or 2*(y1 + chamfer length)
+ * on a chamfered grid) from the electrode face at x = xg1: y uniform over the
+ * grid (n_icp = 1: [(ymax-Lg)/2, (ymax+Lg)/2], else [0, ymax)), on the chamfer
+ * of a tagged negative-ion grid (ptype_fx == tag_neg, flag_chmfrd = 1) moved
+ * onto the chamfer and typed tag_b (when > 0); a candidate is kept when its
+ * cell touches the electrode, and leaves toward the plasma side with a
+ * flux-Maxwellian normal speed and a Maxwellian tangential pair of
+ * temperature THm, spread over its ns_Hm*dt flight.  Appends after each
+ * species' particles, adds one count per particle to the source map
+ * (LEPIC_GRID_SRC).  LEPIC_MCC_CHECK replays the reference's ran2 stream
+ * from *iseed (FP64 contexts); LEPIC_MCC_PROD draws per particle from a
+ * counter-based generator keyed by (seed, it). */
+typedef struct {
+  double jne;             /* |js| in mA/cm^2 */
+  double Lg, Lh, xg1;     /* grid length, hole length, electrode face x (m) */
+  double THm;             /* emitted temperature (eV) */
+  double chmfrd_array[4]; /* chamfer x1, x2, y1, y2 (m) */
+  int32_t n_holes, n_icp, ptype_fx, tag_b, flag_chmfrd, ns_Hm;
+} lepic_emitter;
+typedef struct {
+  int64_t Nh;             /* particles emitted */
+  int64_t created[8];     /* per species (index ptype-1) */
+  int64_t cnt_b[2];       /* cnt_b(1:2): flat surface, chamfer */
+  double P_loss4[8];      /* P_loss(4,ptype): kinetic energy emitted */
+  double Lgrd, dNh;       /* effective grid length, expected count */
+  int32_t seed, reserved; /* check mode: ran2 state after the call */
+} lepic_emitter_info;
+int lepic_flux_emitter(lepic_ctx *ctx, const lepic_emitter *e, int32_t it, int32_t mode, uint64_t seed,
+                       int32_t *iseed, lepic_emitter_info *info);
+
 /* ---- synthetic input (load_part.f90:59-224 restated for a GPU) ----
  * Fills [xl,xr) x [0,ymax) of a species with ppc particles per cell
  * uniformly, velocities Maxwellian with thermal speed vt (vt0 of
@@ -241,6 +274,10 @@ typedef struct {
   const lepic_heating *heat;
   int32_t ns_heat, heat_mode;
   double P_heat;          /* output: the heating's energy change this step */
+  /* surface emission after the push when emit != NULL, jne > 0 and
+   * (ns_Hm == 1 or it%ns_Hm == 1) (main.f90:1092-1096), in mode heat_mode */
+  const lepic_emitter *emit;
+  int64_t emitted;        /* output: particles emitted this step */
 } lepic_step_args;
 /* ---- learned warm start (no counterpart in the reference) ----
  * A U-Net (4 levels, 16-32-64-128 channels, BF16 MFMA kernels with FP32

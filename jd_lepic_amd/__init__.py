@@ -376,19 +376,43 @@ class Context:
         del keep
         return dP.value, s.value
 
+    # ---- surface emitter (part_flux_emitter.f90:1-217) ----
+    @staticmethod
+    def _emitter(em):
+        """em: dict with jne, Lg, Lh, xg1, THm, n_holes, n_icp, ptype_fx,
+        tag_b, flag_chmfrd, ns_Hm and optionally chmfrd_array (x1, x2, y1, y2)."""
+        e = capi.Emitter()
+        for k in ("jne", "Lg", "Lh", "xg1", "THm", "n_holes", "n_icp", "ptype_fx", "tag_b", "flag_chmfrd", "ns_Hm"):
+            setattr(e, k, em[k])
+        for j, v in enumerate(em.get("chmfrd_array", (0.0, 0.0, 0.0, 0.0))):
+            e.chmfrd_array[j] = v
+        return e
+
+    def flux_emitter(self, em, it=0, mode=MCC_PROD, seed=0, iseed=123456):
+        """One call of part_flux_emitter; returns dict(Nh, created, cnt_b,
+        P_loss4, Lgrd, dNh, seed)."""
+        e, info, s = self._emitter(em), capi.EmitterInfo(), C.c_int32(iseed)
+        self._check(self.lib.lepic_flux_emitter(self._ctx, C.byref(e), it, mode, seed, C.byref(s), C.byref(info)))
+        return dict(Nh=info.Nh, created=np.array(info.created[:self.nsp]), cnt_b=np.array(info.cnt_b[:]),
+                    P_loss4=np.array(info.P_loss4[:self.nsp]), Lgrd=info.Lgrd, dNh=info.dNh, seed=s.value)
+
     # ---- whole step ----
     def step(self, it, eps=1e-10, ncycl=20000, nsort=10, ns_coll=10, mcc_mode=MCC_PROD, seed=0,
-             iseed=123456, start=START_PREV, heat=None, ns_heat=1, heat_mode=MCC_PROD):
+             iseed=123456, start=START_PREV, heat=None, ns_heat=1, heat_mode=MCC_PROD, emit=None):
         a = capi.StepArgs()
         a.eps, a.ncycl, a.nsort, a.ns_coll, a.mcc_mode = eps, ncycl, nsort, ns_coll, mcc_mode
         a.seed, a.iseed, a.start = seed, iseed, start
         keep = None
+        a.heat_mode = heat_mode
         if heat is not None:
             h, keep = self._heating(heat)
-            a.heat, a.ns_heat, a.heat_mode = C.pointer(h), ns_heat, heat_mode
+            a.heat, a.ns_heat = C.pointer(h), ns_heat
+        if emit is not None:
+            e = self._emitter(emit)
+            a.emit = C.pointer(e)
         self._check(self.lib.lepic_step(self._ctx, it, C.byref(a)))
         del keep
-        return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res, P_heat=a.P_heat,
+        return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res, P_heat=a.P_heat, emitted=a.emitted,
                     ms=dict(field=a.ms_rhs_poisson_e, sort=a.ms_sort, mcc=a.ms_mcc, push=a.ms_push))
 
 

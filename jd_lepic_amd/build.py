@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblepic_hip.so")
 SOURCES = ["lepic_ctx.hip", "push.hip", "field.hip", "poisson.hip", "sort.hip", "mcc.hip", "loader.hip",
-           "unet.hip", "checkpoint.hip", "moments.hip", "heating.hip"]
+           "unet.hip", "checkpoint.hip", "moments.hip", "heating.hip", "sources.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",

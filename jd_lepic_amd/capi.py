@@ -65,13 +65,27 @@ class Heating(C.Structure):
                 ("yr_pow", C.c_double), ("nudt", C.c_double), ("Pabs_cor", C.c_double), ("vt", _dp)]
 
 
+class Emitter(C.Structure):
+    _fields_ = [("jne", C.c_double), ("Lg", C.c_double), ("Lh", C.c_double), ("xg1", C.c_double),
+                ("THm", C.c_double), ("chmfrd_array", C.c_double * 4), ("n_holes", C.c_int32),
+                ("n_icp", C.c_int32), ("ptype_fx", C.c_int32), ("tag_b", C.c_int32), ("flag_chmfrd", C.c_int32),
+                ("ns_Hm", C.c_int32)]
+
+
+class EmitterInfo(C.Structure):
+    _fields_ = [("Nh", C.c_int64), ("created", C.c_int64 * 8), ("cnt_b", C.c_int64 * 2),
+                ("P_loss4", C.c_double * 8), ("Lgrd", C.c_double), ("dNh", C.c_double), ("seed", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 class StepArgs(C.Structure):
     _fields_ = [("eps", C.c_double), ("ncycl", C.c_int32), ("nsort", C.c_int32),
                 ("ns_coll", C.c_int32), ("mcc_mode", C.c_int32), ("seed", C.c_uint64),
                 ("iseed", C.c_int32), ("start", C.c_int32), ("solve", SolveInfo),
                 ("ms_rhs_poisson_e", C.c_double), ("ms_sort", C.c_double), ("ms_mcc", C.c_double),
                 ("ms_push", C.c_double), ("heat", C.POINTER(Heating)), ("ns_heat", C.c_int32),
-                ("heat_mode", C.c_int32), ("P_heat", C.c_double)]
+                ("heat_mode", C.c_int32), ("P_heat", C.c_double), ("emit", C.POINTER(Emitter)),
+                ("emitted", C.c_int64)]
 
 
 _ctxp = C.c_void_p
@@ -132,6 +146,8 @@ SIGNATURES = {
                                    C.POINTER(MccInfo)]),
     "lepic_get_cnt_col": (C.c_int, [_ctxp, _dp]),
     "lepic_get_mcc_ploss": (C.c_int, [_ctxp, _dp]),
+    "lepic_flux_emitter": (C.c_int, [_ctxp, C.POINTER(Emitter), C.c_int32, C.c_int32, C.c_uint64, _ip,
+                                     C.POINTER(EmitterInfo)]),
     "lepic_eheating": (C.c_int, [_ctxp, C.POINTER(Heating), C.c_int32, C.c_uint64, C.c_int32, _ip, _dp]),
     "lepic_load_uniform": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_uint64]),

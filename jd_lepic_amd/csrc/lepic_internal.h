@@ -200,6 +200,13 @@ struct lepic_ctx {
   lepic::Net* net = nullptr;
   Moments mom;
   double* d_heat = nullptr;          // electron heating: vt table, power, seed (heating.hip)
+  // particle sources (sources.hip): staged particles (x y z vx vy vz),
+  // their 1-based species, per-species ranks, and the device tally
+  double* d_stage6 = nullptr;
+  int8_t* d_stype = nullptr;
+  uint32_t* d_srank = nullptr;
+  int64_t stage6_cap = 0;
+  void* d_srctally = nullptr;
 
   int fail(int code, const std::string& msg) {
     err = msg;
@@ -215,6 +222,7 @@ int set_packed_counts(lepic_ctx* ctx, int p);    // segments from the device cou
 inline int64_t nseg_of(const Species& s) { return (s.ub + kChunk - 1) / kChunk; }
 int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n);   // in place
 int zero_async(lepic_ctx* ctx, void* p, size_t bytes);
+int deposit_range_launch(lepic_ctx* ctx, int p, int64_t seg0, int64_t nseg);   // push.hip
 
 // ---- phase entry points implemented per translation unit ----
 int mover_launch(lepic_ctx* ctx, int ptype, int deposit);
@@ -300,7 +308,7 @@ __host__ __device__ inline double u01(uint32_t hi, uint32_t lo) {
 // The reference's generator (ran2.f90:1-15: Park-Miller minimal standard,
 // multiplier 16807 modulo 2^31-1, Schrage's factorisation) and its
 // Maxwellian pair (load_gauss.f90:14-21), used by the check-mode replays.
-__device__ inline double ran2(int32_t& s) {
+__host__ __device__ inline double ran2(int32_t& s) {
   const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
   int32_t k = s / iq;
   s = ia * (s - k * iq) - ir * k;

@@ -162,6 +162,12 @@ extern "C" int lepic_step(lepic_ctx* ctx, int32_t it, lepic_step_args* a) {
     rc = lepic_eheating(ctx, a->heat, a->heat_mode, a->seed, it, &a->iseed, &a->P_heat);
   LEPIC_HIP(hipEventRecord(ev[3], ctx->stream));
   for (int p = 1; p <= ctx->nsp && !rc; ++p) rc = lepic_part_mover(ctx, p, 1);  // :1041-1086
+  a->emitted = 0;
+  if (!rc && a->emit && a->emit->jne > 0.0 && (a->emit->ns_Hm == 1 || it % a->emit->ns_Hm == 1)) {  // :1092-1096
+    lepic_emitter_info info;
+    rc = lepic_flux_emitter(ctx, a->emit, it, a->heat_mode, a->seed, &a->iseed, &info);
+    if (!rc) a->emitted = info.Nh;
+  }
   LEPIC_HIP(hipEventRecord(ev[4], ctx->stream));
   LEPIC_HIP(hipEventSynchronize(ev[4]));
   float ms[4];

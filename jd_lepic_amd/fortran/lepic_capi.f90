@@ -52,6 +52,17 @@ module lepic_capi
      type(c_ptr) :: vt
   end type lepic_heating
 
+  type, bind(C) :: lepic_emitter
+     real(c_double) :: jne, Lg, Lh, xg1, THm, chmfrd_array(4)
+     integer(c_int32_t) :: n_holes, n_icp, ptype_fx, tag_b, flag_chmfrd, ns_Hm
+  end type lepic_emitter
+
+  type, bind(C) :: lepic_emitter_info
+     integer(c_int64_t) :: Nh, created(8), cnt_b(2)
+     real(c_double) :: P_loss4(8), Lgrd, dNh
+     integer(c_int32_t) :: seed, reserved
+  end type lepic_emitter_info
+
   interface
      integer(c_int) function lepic_create(g, sp, ctx) bind(C, name='lepic_create')
        import :: c_int, c_ptr, lepic_grid_desc, lepic_species_desc
@@ -190,6 +201,17 @@ module lepic_capi
        integer(c_int32_t), intent(inout) :: iseed
        real(c_double), intent(out) :: dP
      end function lepic_eheating
+     ! surface emitter (part_flux_emitter.f90:1-217), called where main.f90:1092-1096 calls it
+     integer(c_int) function lepic_flux_emitter(ctx, e, it, mode, seed, iseed, info) &
+          bind(C, name='lepic_flux_emitter')
+       import :: c_int, c_ptr, c_int32_t, c_int64_t, lepic_emitter, lepic_emitter_info
+       type(c_ptr), value :: ctx
+       type(lepic_emitter), intent(in) :: e
+       integer(c_int32_t), value :: it, mode
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), intent(inout) :: iseed
+       type(lepic_emitter_info), intent(out) :: info
+     end function lepic_flux_emitter
      ! averaged moments (utils.f90:1-265) and backup files (main.f90:1102-1132, restart.f90)
      integer(c_int) function lepic_moments_accumulate(ctx, size_na) bind(C, name='lepic_moments_accumulate')
        import :: c_int, c_ptr, c_int32_t
