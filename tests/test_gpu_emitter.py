@@ -54,7 +54,7 @@ def residents(n, seed):
 
 def context(prec, res, cap=12000):
     sp = [L.Species(CHARGE[p], MASS[p], 1e3, cap) for p in (1, 2, 3, 4)]
-    ctx = L.Context(N1, N2, H, H, DT, sp, zmax=ZMAX, ng=4, precision=prec, tag_neg=3)
+    ctx = L.Context(N1, N2, H, H, DT, sp, zmax=ZMAX, ng=4, ngrid=2, precision=prec, tag_neg=3)
     ctx.set_boundary(bcnd(), np.zeros((N2 + 3, N1 + 3)))
     for p in (1, 2, 3, 4):
         ctx.upload(p, res[p])
@@ -133,11 +133,11 @@ def test_production_repeats_per_seed_and_step():
     assert np.array_equal(outs[0], outs[1]) and not np.array_equal(outs[0][20:50], outs[2][20:50])
 
 
-@pytest.mark.parametrize("cap", [12000, 3700])
+@pytest.mark.parametrize("cap", [12000, 4000])
 def test_append_after_push_and_mover_continues(cap):
     """After a push the species sits in partly filled segments: the emission
     lands in fresh segments (cap 12000) or, when those would overflow the
-    arrays, after a pack (cap 3700); either way the species reads residents
+    arrays, after a pack (cap 4000); either way the species reads residents
     then emitted particles, and the next push runs over all of them."""
     print(f"emitter append cap={cap}", flush=True)
     res = {p: residents(1100, 30 + p) for p in (1, 2, 3, 4)}
