@@ -1067,3 +1067,180 @@ int lo_flux_emitter(const lo_domain *d, const lo_emitter *e, lo_particles *p, in
   }
   return 0;
 }
+
+/* shifted_maxwellian_flux, part_injection.f90:239-265: acceptance-rejection
+ * of v*exp(-(v-vb)^2/vt^2) on [max(0, vb-4vt), vb+4vt] under fmax. */
+static int lo_shifted_flux(double *v, double vb, double vt, double f_max, int32_t *iseed, long long max_tries) {
+  const double vm = (vb - 4.0 * vt) > 0.0 ? vb - 4.0 * vt : 0.0, vp = vb + 4.0 * vt;
+  for (long long t = 0; t < max_tries; ++t) {
+    const double r1 = lo_ran2(iseed), r2 = lo_ran2(iseed);
+    *v = vm + r1 * (vp - vm);
+    const double f = *v * exp(-(*v - vb) * (*v - vb) / (vt * vt));
+    if (!(r2 > f / f_max)) return 0;
+  }
+  return -1;
+}
+
+/* The main loop's injection (main.f90:1063-1071): part_injection,
+ * part_injection.f90:1-237, for every species in turn with one ran2 stream,
+ * then N_inj = 0.  Candidates whose cell lies outside the node range are
+ * drawn again (the reference would index bcnd out of bounds); more than
+ * max_tries candidates for one particle return -1 where the reference loops
+ * forever; -17 when a species overflows its capacity. */
+int lo_part_injection(const lo_domain *d, const lo_injection *in, const long long *N_inj_in,
+                      const double *phi, lo_particles *p, int32_t *iseed, long long max_tries,
+                      lo_injection_out *o) {
+  const int n1 = d->n1, n2 = d->n2, ntype = in->ntype, opt = in->opt_inj, aopt = opt < 0 ? -opt : opt;
+  const double h1 = d->h1, h2 = d->h2, xmax = d->xmax, ymax = d->ymax, zmax = d->zmax, dt = d->dt, qe = d->qe;
+  const double dr = in->dr, xa = in->xa;
+  const int32_t *bc = d->bcnd;
+  long long N_inj[LO_MAX_TYPES + 1];
+  for (int q = 0; q <= LO_MAX_TYPES; ++q) N_inj[q] = q <= ntype && N_inj_in ? N_inj_in[q] : 0;
+  for (int ptype = 1; ptype <= ntype; ++ptype) {
+    if (!(fabs(in->charge[ptype]) > 0.0)) continue;
+    double vz_sav = 0.0, x_sav = 0.0, vb = 0.0, f_max = 0.0;
+    int flag = 0;
+    if (aopt == 3 || aopt == 4) {
+      vb = sqrt(2.0 * qe * in->THm / fabs(in->mass[ptype]));
+      if (aopt == 4) {
+        if (ptype == 1 && in->flag_nmn == 1) vb = 0.0;
+        const double vt0 = in->vt0[ptype];
+        const double vmax = (vb + sqrt(vb * vb + 2.0 * vt0 * vt0)) / 2.0;   /* lines 268-286 */
+        f_max = vmax * exp(-(vmax - vb) * (vmax - vb) / (vt0 * vt0));
+      }
+    }
+    double dN_inj = 0.0;
+    if (aopt == 2) {
+      if (ptype == 1 || (ptype == in->tag_neg && in->tag_neg > 0)) {
+        flag = 1;
+        long long tmp = 0;
+        for (int q = 2; q <= ntype; ++q) tmp += N_inj[q];
+        if (in->tag_neg > 0) tmp = tmp - N_inj[in->tag_neg];
+        dN_inj = in->ni0[ptype] * (double)tmp;
+      }
+    } else {
+      flag = 1;
+      dN_inj = in->I_inj * (double)in->ns_inj * dt * in->ni0[ptype] / (qe * in->Nm[1]) / 1.0 / 1.0;
+    }
+    if (flag == 1) {
+      long long n = (long long)dN_inj;
+      if (lo_ran2(iseed) <= dN_inj - (double)n) n = n + 1;
+      N_inj[ptype] = n;
+    }
+    if (aopt == 3 || (aopt == 4 && in->flag_nmn == 0))
+      for (int q = 2; q <= ntype; ++q) N_inj[q] = 0;
+    for (long long i = 1; i <= N_inj[ptype]; ++i) {
+      double x = 0.0, y = 0.0, z = 0.0;
+      int ix = 0, iy = 0;
+      long long tries = 0;
+      for (;;) { /* label 70 */
+        if (++tries > max_tries) return -1;
+        if (opt > 0) {
+          for (;;) { /* label 60 */
+            x = lo_ran2(iseed) * (in->xl_pow - in->xr_pow) + in->xr_pow;
+            if (in->flag_c != 1) break;
+            y = lo_ran2(iseed) * (2.0 * dr) + ymax / 2.0 - dr;
+            if (!(((x - xa) * (x - xa) + (y - ymax / 2.0) * (y - ymax / 2.0)) > dr * dr)) break;
+            if (++tries > max_tries) return -1;
+          }
+        }
+        if (opt < 0) {
+          if (in->flag_c == 0) {
+            for (;;) { /* label 50 */
+              if (x_sav == 0.0) {
+                double rnd[2];
+                rnd[0] = lo_ran2(iseed);
+                rnd[1] = lo_ran2(iseed);
+                lo_load_gauss(&x, &x_sav, dr, rnd, in->pi);
+                x = x + xa;
+                x_sav = x_sav + xa;
+              } else {
+                x = x_sav;
+                x_sav = 0.0;
+              }
+              if (in->flag_nmn == 1 && xa == 0.0) x = fabs(x);
+              if (!(x < (xa - 4.0 * dr) || x > (xa + 4.0 * dr) || x < 0.0 || x > xmax)) break;
+              if (++tries > max_tries) return -1;
+            }
+          } else {
+            for (;;) { /* label 40 */
+              double rnd[2];
+              rnd[0] = lo_ran2(iseed);
+              rnd[1] = lo_ran2(iseed);
+              lo_load_gauss(&x, &y, dr, rnd, in->pi);
+              x = x + xa;
+              y = y + ymax / 2.0;
+              if (!(((x - xa) * (x - xa) + (y - ymax / 2.0) * (y - ymax / 2.0)) > 4.0 * dr * dr || x < 0.0 ||
+                    x > xmax || y < 0.0 || y > ymax))
+                break;
+              if (++tries > max_tries) return -1;
+            }
+          }
+        }
+        if (in->flag_c == 0) y = lo_ran2(iseed) * (in->yl_pow - in->yr_pow) + in->yr_pow;
+        z = lo_ran2(iseed) * zmax;
+        ix = (int)(x / h1) + 1;
+        iy = (int)(y / h2) + 1;
+        if (ix < 0 || ix > n1 + 1 || iy < 0 || iy > n2 + 1) continue;
+        if (bc[NODE(ix, iy, n1)] >= 1 && bc[NODE(ix + 1, iy, n1)] >= 1 && bc[NODE(ix + 1, iy + 1, n1)] >= 1 &&
+            bc[NODE(ix, iy + 1, n1)] >= 1)
+          continue;
+        break;
+      }
+      const long long k = p->np_tot[ptype] + 1;
+      if (k > p->cap[ptype]) return -17;
+      p->np_tot[ptype] = k;
+      double vt = in->vt0[ptype];
+      if (aopt <= 2 && in->flag_Tp == 1 && ptype == 1) vt = sqrt(2.0 * qe * in->THm / fabs(in->mass[ptype]));
+      double vx, vy, vz;
+      if (aopt == 4) {
+        const double vb_tmp = vb * cos(in->th_B);
+        long long t4 = 0;
+        for (;;) { /* label 75 */
+          if (++t4 > max_tries) return -1;
+          if (lo_shifted_flux(&vx, vb_tmp, vt, f_max, iseed, max_tries)) return -1;
+          double rnd[2];
+          rnd[0] = lo_ran2(iseed);
+          rnd[1] = lo_ran2(iseed);
+          lo_load_gauss(&vy, &vz, vt, rnd, in->pi);
+          vy = vy + vb * sin(in->th_B);
+          const double dt_tmp = lo_ran2(iseed) * ((double)(float)in->ns_inj * dt);
+          x = vx * dt_tmp;
+          if (!(x < 0.0 || x > xmax)) break;
+        }
+        ix = (int)floor(x / h1) + 1;
+      } else {
+        double rnd[2];
+        rnd[0] = lo_ran2(iseed);
+        rnd[1] = lo_ran2(iseed);
+        lo_load_gauss(&vx, &vy, vt, rnd, in->pi);
+        if (vz_sav == 0.0) {
+          rnd[0] = lo_ran2(iseed);
+          rnd[1] = lo_ran2(iseed);
+          lo_load_gauss(&vx, &vy, vt, rnd, in->pi);
+          vz = vx;
+          vz_sav = vy;
+        } else {
+          vz = vz_sav;
+          vz_sav = 0.0;
+        }
+      }
+      double *v = p->vxp[ptype] + 6 * (k - 1);
+      v[0] = x; v[1] = y; v[2] = z; v[3] = vx; v[4] = vy;
+      if (aopt == 3) vz = vz + vb;
+      v[5] = vz;
+      if (p->src[ptype]) p->src[ptype][NODE(ix, iy, n1)] += 1.0;
+      o->injected[ptype] += 1;
+      o->P_loss4[ptype] = o->P_loss4[ptype] + 0.5 * in->Nm[ptype] * in->mass[ptype] *
+                                                  (v[3] * v[3] + v[4] * v[4] + v[5] * v[5]);
+      if (in->flag_diffsrc == 1 && phi) {
+        const double px = (ix * h1 - x) / h1, py = (iy * h2 - y) / h2;
+        const double k1 = px * py, k2 = (1.0 - px) * py, k3 = (1.0 - px) * (1.0 - py), k4 = px * (1.0 - py);
+        const double phip = k1 * phi[NODE(ix, iy, n1)] + k2 * phi[NODE(ix + 1, iy, n1)] +
+                            k3 * phi[NODE(ix + 1, iy + 1, n1)] + k4 * phi[NODE(ix, iy + 1, n1)];
+        o->P_loss1[ptype] = o->P_loss1[ptype] - in->Nm[ptype] * in->charge[ptype] * phip;
+      }
+    }
+  }
+  return 0;
+}

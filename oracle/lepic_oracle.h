@@ -24,6 +24,11 @@
  *                      MPI); heated fraction and Maxwellian energy.
  *   lo_flux_emitter    oracle/_ref part_flux_emitter (bit-exact; the file
  *                      has no MPI); emitted current, flux-Maxwellian speeds.
+ *   lo_part_injection  closed forms and statistics (part_injection.f90
+ *                      includes mpif.h, so it cannot be compiled here):
+ *                      injected count, region, Maxwellian and flux-
+ *                      Maxwellian velocities, the vx = vz pairing of its
+ *                      velocity draws, energy and source tallies.
  *   lo_part_moments, lo_calc_avg   closed forms: single-particle weights,
  *                      cold beams (Tp = 0, j/n = beam velocity), Maxwellian
  *                      temperature within its sampling error, drift
@@ -233,6 +238,24 @@ typedef struct {
 } lo_emitter_out;
 int lo_flux_emitter(const lo_domain *d, const lo_emitter *e, lo_particles *p, int32_t *iseed,
                     long long max_tries, lo_emitter_out *o);
+
+/* volume injection parameters (particle_info.h COMMON members and main.f90
+ * arrays read by part_injection.f90); species arrays indexed by ptype */
+typedef struct {
+  int opt_inj, flag_c, flag_nmn, flag_Tp, flag_diffsrc, ns_inj, tag_neg, ntype;
+  double I_inj, xl_pow, xr_pow, yl_pow, yr_pow, xa, dr, THm, th_B, pi;
+  double ni0[LO_MAX_TYPES + 1], vt0[LO_MAX_TYPES + 1], Nm[LO_MAX_TYPES + 1], mass[LO_MAX_TYPES + 1],
+      charge[LO_MAX_TYPES + 1];
+} lo_injection;
+typedef struct {
+  long long injected[LO_MAX_TYPES + 1];
+  double P_loss4[LO_MAX_TYPES + 1], P_loss1[LO_MAX_TYPES + 1];
+} lo_injection_out;
+/* N_inj[ptype]: particles of each species lost in this step's push (opt_inj
+ * = +-2); phi: node array (flag_diffsrc = 1) */
+int lo_part_injection(const lo_domain *d, const lo_injection *in, const long long *N_inj,
+                      const double *phi, lo_particles *p, int32_t *iseed, long long max_tries,
+                      lo_injection_out *o);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,19 @@ class LoEmitterOut(C.Structure):
                 ("dNh", C.c_double), ("Nh", C.c_longlong)]
 
 
+class LoInjection(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("opt_inj", "flag_c", "flag_nmn", "flag_Tp", "flag_diffsrc", "ns_inj",
+                                       "tag_neg", "ntype")] + \
+               [(k, C.c_double) for k in ("I_inj", "xl_pow", "xr_pow", "yl_pow", "yr_pow", "xa", "dr", "THm",
+                                          "th_B", "pi")] + \
+               [(k, C.c_double * (MAX_TYPES + 1)) for k in ("ni0", "vt0", "Nm", "mass", "charge")]
+
+
+class LoInjectionOut(C.Structure):
+    _fields_ = [("injected", C.c_longlong * (MAX_TYPES + 1)), ("P_loss4", C.c_double * (MAX_TYPES + 1)),
+                ("P_loss1", C.c_double * (MAX_TYPES + 1))]
+
+
 class LoPoisson(C.Structure):
     _fields_ = [("n1", C.c_int), ("n2", C.c_int), ("ng", C.c_int), ("h1", C.c_double),
                 ("h2", C.c_double), ("eps0", C.c_double), ("bcnd", _ip)]
@@ -139,6 +152,9 @@ def _declare(L):
     L.lo_load_gauss.argtypes = [_dp, _dp, C.c_double, _dp, C.c_double]
     L.lo_eheating.restype = C.c_double
     L.lo_eheating.argtypes = [C.POINTER(LoEheat), _dp, C.c_longlong, _ip]
+    L.lo_part_injection.restype = C.c_int
+    L.lo_part_injection.argtypes = [C.POINTER(LoDomain), C.POINTER(LoInjection), C.POINTER(C.c_longlong), _dp,
+                                    C.POINTER(LoParticles), _ip, C.c_longlong, C.POINTER(LoInjectionOut)]
     L.lo_flux_emitter.restype = C.c_int
     L.lo_flux_emitter.argtypes = [C.POINTER(LoDomain), C.POINTER(LoEmitter), C.POINTER(LoParticles), _ip,
                                   C.c_longlong, C.POINTER(LoEmitterOut)]
@@ -321,6 +337,50 @@ def flux_emitter(dom: Domain, em: dict, parts: dict, iseed: int, max_tries: int 
         dd["np_tot"] = P.np_tot[p]
     return dict(rc=rc, seed=s.value, cnt_b=np.array(o.cnt_b[:]), P_loss4=np.array(o.P_loss4[:]), Lgrd=o.Lgrd,
                 dNh=o.dNh, Nh=o.Nh)
+
+
+INJECTION_INTS = ("opt_inj", "flag_c", "flag_nmn", "flag_Tp", "flag_diffsrc", "ns_inj", "tag_neg")
+INJECTION_DBLS = ("I_inj", "xl_pow", "xr_pow", "yl_pow", "yr_pow", "xa", "dr", "THm", "th_B")
+
+
+def part_injection(dom: Domain, inj: dict, parts: dict, iseed: int, N_inj=None, phi=None,
+                   max_tries: int = 1 << 20):
+    """main.f90:1063-1071 with part_injection.f90:1-237 (lo_part_injection).
+    inj: INJECTION_INTS / INJECTION_DBLS keys and ni0, vt0, Nm, mass, charge
+    as {ptype: value}.  parts as for flux_emitter (species 1..ntype), appended
+    to in place; N_inj {ptype: lost count} for opt_inj = +-2.  Returns
+    dict(rc, seed, injected, P_loss4, P_loss1) indexed by ptype."""
+    e = LoInjection()
+    for k in INJECTION_INTS:
+        setattr(e, k, int(inj.get(k, 0)))
+    for k in INJECTION_DBLS:
+        setattr(e, k, float(inj.get(k, 0.0)))
+    e.ntype = max(parts)
+    e.pi = PI
+    for k in ("ni0", "vt0", "Nm", "mass", "charge"):
+        arr = getattr(e, k)
+        for p, v in inj[k].items():
+            arr[p] = v
+    P = LoParticles()
+    for p, d in parts.items():
+        P.vxp[p] = _p(d["vxp"])
+        P.np_tot[p] = d["np_tot"]
+        P.cap[p] = d["vxp"].shape[0]
+        if "src" in d:
+            P.src[p] = _p(d["src"])
+    n = (C.c_longlong * (MAX_TYPES + 1))()
+    for p, v in (N_inj or {}).items():
+        n[p] = v
+    ph = np.ascontiguousarray(phi, np.float64) if phi is not None else None
+    o = LoInjectionOut()
+    s = C.c_int32(iseed)
+    d = dom.struct()
+    rc = lib().lo_part_injection(C.byref(d), C.byref(e), n, _p(ph) if ph is not None else None, C.byref(P),
+                                 C.byref(s), max_tries, C.byref(o))
+    for p, dd in parts.items():
+        dd["np_tot"] = P.np_tot[p]
+    return dict(rc=rc, seed=s.value, injected=np.array(o.injected[:]), P_loss4=np.array(o.P_loss4[:]),
+                P_loss1=np.array(o.P_loss1[:]))
 
 
 def part_moments(dom: Domain, Nm, vxp, size_na=1, kq_arr=None):
