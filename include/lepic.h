@@ -246,6 +246,41 @@ typedef struct {
 int lepic_flux_emitter(lepic_ctx *ctx, const lepic_emitter *e, int32_t it, int32_t mode, uint64_t seed,
                        int32_t *iseed, lepic_emitter_info *info);
 
+/* ---- volume injection (part_injection.f90:1-237; the species loop of
+ *      main.f90:1063-1071, every ns_inj steps between the push and the
+ *      deposition) ----
+ * For each species in turn: the count -- opt_inj = +-2: electrons
+ * ni0(1)*(ions lost this step, tagged negative ions excepted), the tagged
+ * negative ion its ni0 share of the same, every other ion its own losses
+ * (N_inj, index ptype-1); otherwise I_inj*ns_inj*dt*ni0(ptype)/(qe*Nm(1)),
+ * rounded by one draw; +-3 and +-4 (flag_nmn = 0) inject electrons only --
+ * then the particles: positions uniform in [xr_pow,xl_pow] x [yr_pow,yl_pow]
+ * (flag_c = 0) or the disk of radius dr about (xa, ymax/2) (flag_c = 1), or
+ * Gaussian of width dr for opt_inj < 0, never in an all-wall cell;
+ * velocities Maxwellian of vt0(ptype) (the electrons' from THm with
+ * flag_Tp = 1), with a z beam sqrt(2 qe THm/m) for +-3, or a shifted
+ * Maxwellian flux off x = 0 along th_B for +-4.  Appends after each species'
+ * particles, counts ss2D(1) (LEPIC_GRID_SRC), and with deposit = 1 adds the
+ * new charge to np (call it after lepic_part_mover(..., 1)).  Modes as for
+ * the MCC. */
+typedef struct {
+  int32_t opt_inj, flag_c, flag_nmn, flag_Tp, flag_diffsrc, ns_inj;
+  double I_inj;           /* A */
+  double xl_pow, xr_pow, yl_pow, yr_pow, xa, dr;   /* m */
+  double THm, th_B;       /* eV, rad */
+  const double *ni0;      /* ni0(1:ntype), host memory */
+  const double *vt0;      /* vt0(1:ntype) (m/s), host memory */
+} lepic_injection;
+typedef struct {
+  int64_t injected[8];    /* per species (index ptype-1) */
+  double P_loss4[8];      /* P_loss(4,ptype): kinetic energy injected */
+  double P_loss1[8];      /* P_loss(1,ptype): -Nm q phi of the injected (flag_diffsrc = 1) */
+  int32_t seed, reserved; /* check mode: ran2 state after the call */
+} lepic_injection_info;
+int lepic_part_injection(lepic_ctx *ctx, const lepic_injection *inj, const int64_t *N_inj, int32_t it,
+                         int32_t mode, uint64_t seed, int32_t *iseed, int32_t deposit,
+                         lepic_injection_info *info);
+
 /* ---- synthetic input (load_part.f90:59-224 restated for a GPU) ----
  * Fills [xl,xr) x [0,ymax) of a species with ppc particles per cell
  * uniformly, velocities Maxwellian with thermal speed vt (vt0 of
@@ -278,6 +313,10 @@ typedef struct {
    * (ns_Hm == 1 or it%ns_Hm == 1) (main.f90:1092-1096), in mode heat_mode */
   const lepic_emitter *emit;
   int64_t emitted;        /* output: particles emitted this step */
+  /* volume injection after the push when inj != NULL and it%ns_inj == 0
+   * (main.f90:1063-1071), opt_inj = +-2 re-injecting this push's losses */
+  const lepic_injection *inj;
+  int64_t injected;       /* output: particles injected this step */
 } lepic_step_args;
 /* ---- learned warm start (no counterpart in the reference) ----
  * A U-Net (4 levels, 16-32-64-128 channels, BF16 MFMA kernels with FP32

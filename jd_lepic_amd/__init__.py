@@ -396,9 +396,38 @@ class Context:
         return dict(Nh=info.Nh, created=np.array(info.created[:self.nsp]), cnt_b=np.array(info.cnt_b[:]),
                     P_loss4=np.array(info.P_loss4[:self.nsp]), Lgrd=info.Lgrd, dNh=info.dNh, seed=s.value)
 
+    # ---- volume injection (part_injection.f90:1-237) ----
+    def _injection(self, inj):
+        """inj: dict with opt_inj, flag_c, flag_nmn, flag_Tp, flag_diffsrc,
+        ns_inj, I_inj, xl_pow, xr_pow, yl_pow, yr_pow, xa, dr, THm, th_B and
+        ni0, vt0 (sequences over the species).  Returns the struct and the
+        arrays it points at."""
+        e = capi.Injection()
+        for k in ("opt_inj", "flag_c", "flag_nmn", "flag_Tp", "flag_diffsrc", "ns_inj"):
+            setattr(e, k, int(inj.get(k, 0 if k != "ns_inj" else 1)))
+        for k in ("I_inj", "xl_pow", "xr_pow", "yl_pow", "yr_pow", "xa", "dr", "THm", "th_B"):
+            setattr(e, k, float(inj.get(k, 0.0)))
+        keep = [np.ascontiguousarray(inj["ni0"], dtype=np.float64), np.ascontiguousarray(inj["vt0"], dtype=np.float64)]
+        assert keep[0].size >= self.nsp and keep[1].size >= self.nsp
+        e.ni0, e.vt0 = _dp(keep[0]), _dp(keep[1])
+        return e, keep
+
+    def part_injection(self, inj, N_inj=None, it=0, mode=MCC_PROD, seed=0, iseed=123456, deposit=False):
+        """One call of the injection species loop; N_inj: particles of each
+        species lost in this step's push (opt_inj = +-2).  Returns
+        dict(injected, P_loss4, P_loss1, seed)."""
+        e, keep = self._injection(inj)
+        n = np.zeros(self.nsp, np.int64) if N_inj is None else np.ascontiguousarray(N_inj, dtype=np.int64)
+        info, s = capi.InjectionInfo(), C.c_int32(iseed)
+        self._check(self.lib.lepic_part_injection(self._ctx, C.byref(e), n.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                  it, mode, seed, C.byref(s), 1 if deposit else 0, C.byref(info)))
+        del keep
+        return dict(injected=np.array(info.injected[:self.nsp]), P_loss4=np.array(info.P_loss4[:self.nsp]),
+                    P_loss1=np.array(info.P_loss1[:self.nsp]), seed=s.value)
+
     # ---- whole step ----
     def step(self, it, eps=1e-10, ncycl=20000, nsort=10, ns_coll=10, mcc_mode=MCC_PROD, seed=0,
-             iseed=123456, start=START_PREV, heat=None, ns_heat=1, heat_mode=MCC_PROD, emit=None):
+             iseed=123456, start=START_PREV, heat=None, ns_heat=1, heat_mode=MCC_PROD, emit=None, inj=None):
         a = capi.StepArgs()
         a.eps, a.ncycl, a.nsort, a.ns_coll, a.mcc_mode = eps, ncycl, nsort, ns_coll, mcc_mode
         a.seed, a.iseed, a.start = seed, iseed, start
@@ -410,9 +439,13 @@ class Context:
         if emit is not None:
             e = self._emitter(emit)
             a.emit = C.pointer(e)
+        if inj is not None:
+            ij, keep_inj = self._injection(inj)
+            a.inj = C.pointer(ij)
         self._check(self.lib.lepic_step(self._ctx, it, C.byref(a)))
         del keep
         return dict(iseed=a.iseed, vcycles=a.solve.vcycles, res=a.solve.res, P_heat=a.P_heat, emitted=a.emitted,
+                    injected=a.injected,
                     ms=dict(field=a.ms_rhs_poisson_e, sort=a.ms_sort, mcc=a.ms_mcc, push=a.ms_push))
 
 

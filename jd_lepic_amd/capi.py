@@ -78,6 +78,17 @@ class EmitterInfo(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class Injection(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("opt_inj", "flag_c", "flag_nmn", "flag_Tp", "flag_diffsrc", "ns_inj")] + \
+               [(k, C.c_double) for k in ("I_inj", "xl_pow", "xr_pow", "yl_pow", "yr_pow", "xa", "dr", "THm",
+                                          "th_B")] + [("ni0", _dp), ("vt0", _dp)]
+
+
+class InjectionInfo(C.Structure):
+    _fields_ = [("injected", C.c_int64 * 8), ("P_loss4", C.c_double * 8), ("P_loss1", C.c_double * 8),
+                ("seed", C.c_int32), ("reserved", C.c_int32)]
+
+
 class StepArgs(C.Structure):
     _fields_ = [("eps", C.c_double), ("ncycl", C.c_int32), ("nsort", C.c_int32),
                 ("ns_coll", C.c_int32), ("mcc_mode", C.c_int32), ("seed", C.c_uint64),
@@ -85,7 +96,7 @@ class StepArgs(C.Structure):
                 ("ms_rhs_poisson_e", C.c_double), ("ms_sort", C.c_double), ("ms_mcc", C.c_double),
                 ("ms_push", C.c_double), ("heat", C.POINTER(Heating)), ("ns_heat", C.c_int32),
                 ("heat_mode", C.c_int32), ("P_heat", C.c_double), ("emit", C.POINTER(Emitter)),
-                ("emitted", C.c_int64)]
+                ("emitted", C.c_int64), ("inj", C.POINTER(Injection)), ("injected", C.c_int64)]
 
 
 _ctxp = C.c_void_p
@@ -148,6 +159,8 @@ SIGNATURES = {
     "lepic_get_mcc_ploss": (C.c_int, [_ctxp, _dp]),
     "lepic_flux_emitter": (C.c_int, [_ctxp, C.POINTER(Emitter), C.c_int32, C.c_int32, C.c_uint64, _ip,
                                      C.POINTER(EmitterInfo)]),
+    "lepic_part_injection": (C.c_int, [_ctxp, C.POINTER(Injection), C.POINTER(C.c_int64), C.c_int32, C.c_int32,
+                                       C.c_uint64, _ip, C.c_int32, C.POINTER(InjectionInfo)]),
     "lepic_eheating": (C.c_int, [_ctxp, C.POINTER(Heating), C.c_int32, C.c_uint64, C.c_int32, _ip, _dp]),
     "lepic_load_uniform": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_uint64]),

@@ -138,6 +138,18 @@ int load_uniform_launch(lepic_ctx* ctx, int p, int ppc, double vt, double xl, do
   return set_packed_counts(ctx, p);
 }
 
+// SUM(p_mac(ptype, np_loss, 0:ngrid)) per species: particles lost at walls
+int lost_counts(lepic_ctx* ctx, double* out) {
+  for (int q = 0; q < ctx->nsp; ++q) {
+    double p_mac[2 * 65];
+    if (int rc = lepic_get_mover_tally(ctx, q + 1, nullptr, p_mac, nullptr)) return rc;
+    double s = 0.0;
+    for (int g = 0; g <= ctx->g.ngrid; ++g) s += p_mac[g];
+    out[q] = s;
+  }
+  return 0;
+}
+
 }  // namespace lepic
 
 extern "C" int lepic_step(lepic_ctx* ctx, int32_t it, lepic_step_args* a) {
@@ -161,7 +173,21 @@ extern "C" int lepic_step(lepic_ctx* ctx, int32_t it, lepic_step_args* a) {
   if (!rc && a->heat && a->ns_heat > 0 && it % a->ns_heat == 0)  // :1033-1037
     rc = lepic_eheating(ctx, a->heat, a->heat_mode, a->seed, it, &a->iseed, &a->P_heat);
   LEPIC_HIP(hipEventRecord(ev[3], ctx->stream));
+  const bool inject = a->inj && a->inj->ns_inj > 0 && it % a->inj->ns_inj == 0;
+  double lost0[kMaxSpecies] = {};
+  if (!rc && inject) rc = lost_counts(ctx, lost0);   // sav_np, main.f90:1044
   for (int p = 1; p <= ctx->nsp && !rc; ++p) rc = lepic_part_mover(ctx, p, 1);  // :1041-1086
+  a->injected = 0;
+  if (!rc && inject) {  // :1063-1071
+    double lost1[kMaxSpecies] = {};
+    rc = lost_counts(ctx, lost1);
+    int64_t N_inj[kMaxSpecies] = {};
+    for (int q = 0; q < ctx->nsp; ++q) N_inj[q] = (int64_t)(lost1[q] - lost0[q]);
+    lepic_injection_info info;
+    if (!rc) rc = lepic_part_injection(ctx, a->inj, N_inj, it, a->heat_mode, a->seed, &a->iseed, 1, &info);
+    if (!rc)
+      for (int q = 0; q < ctx->nsp; ++q) a->injected += info.injected[q];
+  }
   a->emitted = 0;
   if (!rc && a->emit && a->emit->jne > 0.0 && (a->emit->ns_Hm == 1 || it % a->emit->ns_Hm == 1)) {  // :1092-1096
     lepic_emitter_info info;

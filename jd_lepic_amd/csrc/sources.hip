@@ -27,9 +27,10 @@ namespace {
 
 struct SrcTally {
   double P_loss4[kMaxSpecies + 1];          // by ptype
+  double P_loss1[kMaxSpecies + 1];          // by ptype (injection with flag_diffsrc)
   unsigned long long cnt_b[2];
   unsigned long long tot[kMaxSpecies];      // staged particles per species (index ptype-1)
-  int32_t status;                           // 1: a particle found no electrode cell
+  int32_t status;                           // 1: no admissible candidate, 2: staging full
   int32_t seed;                             // check mode: ran2 state after the call
 };
 
@@ -259,6 +260,290 @@ __global__ void k_emit_prod(EmitArgs a) {
   finish(a, i, ptype, dir, x, y, z, r_n, r1, r2, r_t, true);
 }
 
+// ---- volume injection ------------------------------------------------------
+
+struct InjArgs {
+  const int8_t* bc;
+  const double* phi;
+  int n1, n2, nsp, opt, aopt, flag_c, flag_nmn, flag_Tp, flag_diffsrc, tag_neg;
+  double h1, h2, xmax, ymax, zmax, pi, span;        // span = real(ns_inj)*dt
+  double xl_pow, xr_pow, yl_pow, yr_pow, xa, dr;
+  double cos_thB, sin_thB;                          // host libm
+  double vt[kMaxSpecies + 1];                       // thermal speed used for the velocities
+  double vb[kMaxSpecies + 1], fmax[kMaxSpecies + 1];  // opt_inj 3, 4 (lines 36-45, 268-286)
+  double half_Nm_m[kMaxSpecies + 1], Nm_q[kMaxSpecies + 1];
+  // check mode: the count inputs; production: the counts and stage offsets
+  double ni0[kMaxSpecies + 1], dN_cur[kMaxSpecies + 1];
+  int64_t N_inj[kMaxSpecies + 1];
+  int64_t n[kMaxSpecies + 1], off[kMaxSpecies + 1], pair_off[kMaxSpecies + 2];
+  int64_t stage_cap, max_tries;
+  double* stage;
+  int8_t* type;
+  double* src[kMaxSpecies];
+  SrcTally* t;
+  double* ploss1;                                   // [kMaxSpecies + 1]
+  uint32_t k0, k1, it;
+};
+
+__device__ __forceinline__ bool all_wall_cell(const InjArgs& a, int ix, int iy) {
+  const int64_t n0 = node_index(ix, iy, a.n1), n1 = node_index(ix, iy + 1, a.n1);
+  return a.bc[n0] >= 1 && a.bc[n0 + 1] >= 1 && a.bc[n1 + 1] >= 1 && a.bc[n1] >= 1;
+}
+
+__device__ __forceinline__ bool cell_ok(const InjArgs& a, int ix, int iy) {
+  return ix >= 0 && ix <= a.n1 + 1 && iy >= 0 && iy <= a.n2 + 1 && !all_wall_cell(a, ix, iy);
+}
+
+// the stored particle, its source count and tallies (lines 188-228)
+__device__ __forceinline__ void inj_store(const InjArgs& a, int64_t j, int ptype, int ix, int iy, double x, double y,
+                                          double z, double vx, double vy, double vz, bool atomics) {
+  double* st = a.stage + 6 * j;
+  st[0] = x; st[1] = y; st[2] = z; st[3] = vx; st[4] = vy;
+  if (a.aopt == 3) vz = vz + a.vb[ptype];
+  st[5] = vz;
+  a.type[j] = (int8_t)ptype;
+  const double e = a.half_Nm_m[ptype] * (vx * vx + vy * vy + vz * vz);
+  double e1 = 0.0;
+  if (a.flag_diffsrc == 1) {
+    const double px = (ix * a.h1 - x) / a.h1, py = (iy * a.h2 - y) / a.h2;
+    const double k1 = px * py, k2 = (1.0 - px) * py, k3 = (1.0 - px) * (1.0 - py), k4 = px * (1.0 - py);
+    const double phip = k1 * a.phi[node_index(ix, iy, a.n1)] + k2 * a.phi[node_index(ix + 1, iy, a.n1)] +
+                        k3 * a.phi[node_index(ix + 1, iy + 1, a.n1)] + k4 * a.phi[node_index(ix, iy + 1, a.n1)];
+    e1 = a.Nm_q[ptype] * phip;
+  }
+  if (atomics) {
+    atomic_add_f64(&a.src[ptype - 1][node_index(ix, iy, a.n1)], 1.0);
+    atomic_add_f64(&a.t->P_loss4[ptype], e);
+    if (a.flag_diffsrc == 1) atomic_add_f64(&a.ploss1[ptype], -e1);
+  } else {
+    a.src[ptype - 1][node_index(ix, iy, a.n1)] += 1.0;
+    a.t->P_loss4[ptype] = a.t->P_loss4[ptype] + e;
+    if (a.flag_diffsrc == 1) a.ploss1[ptype] = a.ploss1[ptype] - e1;
+  }
+}
+
+// One lane: the main loop's species loop (main.f90:1063-1071) over
+// part_injection with the reference's ran2 order.
+__global__ void k_inject_check(InjArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int32_t s = a.t->seed;
+  int64_t N_inj[kMaxSpecies + 1];
+  for (int q = 0; q <= kMaxSpecies; ++q) N_inj[q] = a.N_inj[q];
+  int64_t j = 0;
+  for (int ptype = 1; ptype <= a.nsp; ++ptype) {
+    double vz_sav = 0.0, x_sav = 0.0;
+    int flag = 0;
+    double dN = 0.0;
+    if (a.aopt == 2) {
+      if (ptype == 1 || (ptype == a.tag_neg && a.tag_neg > 0)) {
+        flag = 1;
+        int64_t tmp = 0;
+        for (int q = 2; q <= a.nsp; ++q) tmp += N_inj[q];
+        if (a.tag_neg > 0) tmp = tmp - N_inj[a.tag_neg];
+        dN = a.ni0[ptype] * (double)tmp;
+      }
+    } else {
+      flag = 1;
+      dN = a.dN_cur[ptype];
+    }
+    if (flag == 1) {
+      int64_t n = (int64_t)dN;
+      if (ran2(s) <= dN - (double)n) n = n + 1;
+      N_inj[ptype] = n;
+    }
+    if (a.aopt == 3 || (a.aopt == 4 && a.flag_nmn == 0))
+      for (int q = 2; q <= a.nsp; ++q) N_inj[q] = 0;
+    for (int64_t i = 1; i <= N_inj[ptype]; ++i) {
+      double x = 0.0, y = 0.0, z = 0.0;
+      int ix = 0, iy = 0;
+      int64_t tries = 0;
+      bool fail = false;
+      for (;;) {  // label 70
+        if (++tries > a.max_tries) { fail = true; break; }
+        if (a.opt > 0) {
+          for (;;) {  // label 60
+            x = ran2(s) * (a.xl_pow - a.xr_pow) + a.xr_pow;
+            if (a.flag_c != 1) break;
+            y = ran2(s) * (2.0 * a.dr) + a.ymax / 2.0 - a.dr;
+            if (!(((x - a.xa) * (x - a.xa) + (y - a.ymax / 2.0) * (y - a.ymax / 2.0)) > a.dr * a.dr)) break;
+            if (++tries > a.max_tries) { fail = true; break; }
+          }
+        }
+        if (fail) break;
+        if (a.opt < 0) {
+          if (a.flag_c == 0) {
+            for (;;) {  // label 50
+              if (x_sav == 0.0) {
+                const double r1 = ran2(s), r2 = ran2(s);
+                load_gauss(x, x_sav, a.dr, r1, r2, a.pi);
+                x = x + a.xa;
+                x_sav = x_sav + a.xa;
+              } else {
+                x = x_sav;
+                x_sav = 0.0;
+              }
+              if (a.flag_nmn == 1 && a.xa == 0.0) x = fabs(x);
+              if (!(x < (a.xa - 4.0 * a.dr) || x > (a.xa + 4.0 * a.dr) || x < 0.0 || x > a.xmax)) break;
+              if (++tries > a.max_tries) { fail = true; break; }
+            }
+          } else {
+            for (;;) {  // label 40
+              const double r1 = ran2(s), r2 = ran2(s);
+              load_gauss(x, y, a.dr, r1, r2, a.pi);
+              x = x + a.xa;
+              y = y + a.ymax / 2.0;
+              if (!(((x - a.xa) * (x - a.xa) + (y - a.ymax / 2.0) * (y - a.ymax / 2.0)) > 4.0 * a.dr * a.dr ||
+                    x < 0.0 || x > a.xmax || y < 0.0 || y > a.ymax))
+                break;
+              if (++tries > a.max_tries) { fail = true; break; }
+            }
+          }
+        }
+        if (fail) break;
+        if (a.flag_c == 0) y = ran2(s) * (a.yl_pow - a.yr_pow) + a.yr_pow;
+        z = ran2(s) * a.zmax;
+        ix = (int)(x / a.h1) + 1;
+        iy = (int)(y / a.h2) + 1;
+        if (cell_ok(a, ix, iy)) break;
+      }
+      if (fail || j >= a.stage_cap) {
+        a.t->status = fail ? 1 : 2;
+        a.t->seed = s;
+        return;
+      }
+      const double vt = a.vt[ptype];
+      double vx, vy, vz;
+      if (a.aopt == 4) {
+        const double vb_tmp = a.vb[ptype] * a.cos_thB;
+        const double vm = (vb_tmp - 4.0 * vt) > 0.0 ? vb_tmp - 4.0 * vt : 0.0, vp = vb_tmp + 4.0 * vt;
+        int64_t t4 = 0;
+        for (;;) {  // label 75
+          if (++t4 > a.max_tries) { fail = true; break; }
+          for (int64_t t5 = 0;; ++t5) {  // shifted_maxwellian_flux, lines 239-265
+            if (t5 >= a.max_tries) { fail = true; break; }
+            const double r1 = ran2(s), r2 = ran2(s);
+            vx = vm + r1 * (vp - vm);
+            const double f = vx * exp(-(vx - vb_tmp) * (vx - vb_tmp) / (vt * vt));
+            if (!(r2 > f / a.fmax[ptype])) break;
+          }
+          if (fail) break;
+          const double r1 = ran2(s), r2 = ran2(s);
+          load_gauss(vy, vz, vt, r1, r2, a.pi);
+          vy = vy + a.vb[ptype] * a.sin_thB;
+          const double dt_tmp = ran2(s) * a.span;
+          x = vx * dt_tmp;
+          if (!(x < 0.0 || x > a.xmax)) break;
+        }
+        if (fail) {
+          a.t->status = 1;
+          a.t->seed = s;
+          return;
+        }
+        ix = (int)floor(x / a.h1) + 1;
+      } else {
+        double r1 = ran2(s), r2 = ran2(s);
+        load_gauss(vx, vy, vt, r1, r2, a.pi);
+        if (vz_sav == 0.0) {
+          r1 = ran2(s); r2 = ran2(s);
+          load_gauss(vx, vy, vt, r1, r2, a.pi);
+          vz = vx;
+          vz_sav = vy;
+        } else {
+          vz = vz_sav;
+          vz_sav = 0.0;
+        }
+      }
+      inj_store(a, j, ptype, ix, iy, x, y, z, vx, vy, vz, false);
+      ++j;
+    }
+  }
+  a.t->seed = s;
+}
+
+// One lane per pair of particles of a species: the pair shares the velocity
+// draws the way consecutive particles of part_injection share vz_sav
+// (lines 181-193): the first takes vx = vz, the second the first's vy as vz.
+__global__ void k_inject_prod(InjArgs a) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.pair_off[a.nsp + 1]) return;
+  int ptype = 1;
+  while (ptype < a.nsp && g >= a.pair_off[ptype + 1]) ++ptype;
+  const int64_t k = g - a.pair_off[ptype];
+  Stream r{U4{(uint32_t)g, (uint32_t)ptype, a.it, 0u}, a.k0, a.k1, U4{0, 0, 0, 0}};
+  double carry = 0.0;
+  for (int m = 0; m < 2; ++m) {
+    const int64_t i = 2 * k + m;
+    if (i >= a.n[ptype]) break;
+    double x = 0.0, y = 0.0, z = 0.0;
+    int ix = 0, iy = 0;
+    bool ok = false;
+    for (int64_t tries = 0; tries < a.max_tries && !ok; ++tries) {
+      if (a.opt > 0) {
+        x = r.next() * (a.xl_pow - a.xr_pow) + a.xr_pow;
+        if (a.flag_c == 1) {
+          y = r.next() * (2.0 * a.dr) + a.ymax / 2.0 - a.dr;
+          if (((x - a.xa) * (x - a.xa) + (y - a.ymax / 2.0) * (y - a.ymax / 2.0)) > a.dr * a.dr) continue;
+        }
+      } else {
+        double gx, gy;
+        load_gauss(gx, gy, a.dr, r.next(), r.next(), a.pi);
+        if (a.flag_c == 0) {
+          x = gx + a.xa;
+          if (a.flag_nmn == 1 && a.xa == 0.0) x = fabs(x);
+          if (x < (a.xa - 4.0 * a.dr) || x > (a.xa + 4.0 * a.dr) || x < 0.0 || x > a.xmax) continue;
+        } else {
+          x = gx + a.xa;
+          y = gy + a.ymax / 2.0;
+          if (((x - a.xa) * (x - a.xa) + (y - a.ymax / 2.0) * (y - a.ymax / 2.0)) > 4.0 * a.dr * a.dr ||
+              x < 0.0 || x > a.xmax || y < 0.0 || y > a.ymax)
+            continue;
+        }
+      }
+      if (a.flag_c == 0) y = r.next() * (a.yl_pow - a.yr_pow) + a.yr_pow;
+      z = r.next() * a.zmax;
+      ix = (int)(x / a.h1) + 1;
+      iy = (int)(y / a.h2) + 1;
+      ok = cell_ok(a, ix, iy);
+    }
+    const int64_t j = a.off[ptype] + i;
+    if (!ok) {
+      a.type[j] = 0;
+      atomicOr(&a.t->status, 1);
+      return;
+    }
+    const double vt = a.vt[ptype];
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (a.aopt == 4) {
+      const double vb_tmp = a.vb[ptype] * a.cos_thB;
+      const double vm = (vb_tmp - 4.0 * vt) > 0.0 ? vb_tmp - 4.0 * vt : 0.0, vp = vb_tmp + 4.0 * vt;
+      ok = false;
+      for (int64_t t4 = 0; t4 < a.max_tries && !ok; ++t4) {
+        vx = vm + r.next() * (vp - vm);
+        const double f = vx * exp(-(vx - vb_tmp) * (vx - vb_tmp) / (vt * vt));
+        if (r.next() > f / a.fmax[ptype]) continue;
+        load_gauss(vy, vz, vt, r.next(), r.next(), a.pi);
+        vy = vy + a.vb[ptype] * a.sin_thB;
+        x = vx * (r.next() * a.span);
+        ok = !(x < 0.0 || x > a.xmax);
+      }
+      if (!ok) {
+        a.type[j] = 0;
+        atomicOr(&a.t->status, 1);
+        return;
+      }
+      ix = (int)floor(x / a.h1) + 1;
+    } else if (m == 0) {
+      load_gauss(vx, vy, vt, r.next(), r.next(), a.pi);
+      vz = vx;
+      carry = vy;
+    } else {
+      load_gauss(vx, vy, vt, r.next(), r.next(), a.pi);
+      vz = carry;
+    }
+    inj_store(a, j, ptype, ix, iy, x, y, z, vx, vy, vz, true);
+  }
+}
+
 }  // namespace
 
 namespace lepic {
@@ -282,8 +567,8 @@ int reserve_stage(lepic_ctx* ctx, int64_t M) {
 // receives the first segment holding species q's new particles and added[q]
 // their number.  A species goes to fresh segments after its last one; when
 // those would not fit its arrays it is packed first and continued densely.
-int append_staged(lepic_ctx* ctx, int64_t M, int64_t* seg0, int64_t* added) {
-  for (int q = 0; q < kMaxSpecies; ++q) { seg0[q] = 0; added[q] = 0; }
+int append_staged(lepic_ctx* ctx, int64_t M, int64_t* seg0, int64_t* added, bool* dense_out) {
+  for (int q = 0; q < kMaxSpecies; ++q) { seg0[q] = 0; added[q] = 0; if (dense_out) dense_out[q] = false; }
   if (M <= 0) return 0;
   SrcTally* t = (SrcTally*)ctx->d_srctally;
   unsigned long long* tot = t->tot;
@@ -341,6 +626,7 @@ int append_staged(lepic_ctx* ctx, int64_t M, int64_t* seg0, int64_t* added) {
     s.ub = base + cnt;
     seg0[q] = s0;
     added[q] = cnt;
+    if (dense_out) dense_out[q] = dense;
   }
   return 0;
 }
@@ -439,7 +725,7 @@ int lepic_flux_emitter(lepic_ctx* ctx, const lepic_emitter* e, int32_t it, int32
   std::memcpy(&h, ctx->h_pinned, sizeof(h));
   if (h.status) return ctx->fail(LEPIC_ERR_STATE, "flux emitter: no electrode cell facing the plasma at xg1");
   int64_t seg0[kMaxSpecies], added[kMaxSpecies];
-  if (int rc = append_staged(ctx, Nh, seg0, added)) return rc;
+  if (int rc = append_staged(ctx, Nh, seg0, added, nullptr)) return rc;
   if (mode == LEPIC_MCC_CHECK) *iseed = h.seed;
   if (info) {
     std::memset(info, 0, sizeof(*info));
@@ -452,6 +738,160 @@ int lepic_flux_emitter(lepic_ctx* ctx, const lepic_emitter* e, int32_t it, int32
     }
     info->Lgrd = Lgrd;
     info->dNh = dNh;
+    info->seed = h.seed;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int lepic_part_injection(lepic_ctx* ctx, const lepic_injection* in, const int64_t* N_inj_lost, int32_t it,
+                         int32_t mode, uint64_t seed, int32_t* iseed, int32_t deposit, lepic_injection_info* info) {
+  if (!ctx || !in || !in->ni0 || !in->vt0) return LEPIC_ERR_ARG;
+  if (mode != LEPIC_MCC_CHECK && mode != LEPIC_MCC_PROD) return LEPIC_ERR_ARG;
+  if (mode == LEPIC_MCC_CHECK && !iseed) return LEPIC_ERR_ARG;
+  const int aopt = in->opt_inj < 0 ? -in->opt_inj : in->opt_inj;
+  if (aopt < 1 || aopt > 4 || in->ns_inj < 1 || (aopt == 2 && !N_inj_lost)) return LEPIC_ERR_ARG;
+  if (!ctx->boundary_set) return ctx->fail(LEPIC_ERR_STATE, "injection before lepic_set_boundary");
+  if (mode == LEPIC_MCC_CHECK && ctx->g.precision != LEPIC_FP64)
+    return ctx->fail(LEPIC_ERR_UNSUPPORTED, "check-mode injection replays the reference on the FP64 state");
+  const int nsp = ctx->nsp;
+  const double qe = ctx->g.qe, dt = ctx->g.dt;
+  InjArgs a{};
+  a.bc = ctx->bc; a.phi = ctx->phi;
+  a.n1 = ctx->g.n1; a.n2 = ctx->g.n2; a.nsp = nsp; a.opt = in->opt_inj; a.aopt = aopt;
+  a.flag_c = in->flag_c; a.flag_nmn = in->flag_nmn; a.flag_Tp = in->flag_Tp; a.flag_diffsrc = in->flag_diffsrc;
+  a.tag_neg = ctx->g.tag_neg;
+  a.h1 = ctx->g.h1; a.h2 = ctx->g.h2; a.xmax = ctx->g.n1 * ctx->g.h1; a.ymax = ctx->g.n2 * ctx->g.h2;
+  a.zmax = ctx->g.zmax; a.pi = 4.0 * std::atan(1.0); a.span = (double)(float)in->ns_inj * dt;
+  a.xl_pow = in->xl_pow; a.xr_pow = in->xr_pow; a.yl_pow = in->yl_pow; a.yr_pow = in->yr_pow;
+  a.xa = in->xa; a.dr = in->dr;
+  a.cos_thB = std::cos(in->th_B); a.sin_thB = std::sin(in->th_B);
+  int64_t N[kMaxSpecies + 1] = {};
+  for (int p = 1; p <= nsp; ++p) {
+    const Species& sp = ctx->sp[p - 1];
+    const double m = std::fabs(sp.mass);
+    a.vt[p] = in->vt0[p - 1];
+    if (aopt <= 2 && in->flag_Tp == 1 && p == 1) a.vt[p] = std::sqrt(2.0 * qe * in->THm / m);  // lines 153-158
+    if (aopt == 3 || aopt == 4) {
+      a.vb[p] = std::sqrt(2.0 * qe * in->THm / m);
+      if (aopt == 4) {
+        if (p == 1 && in->flag_nmn == 1) a.vb[p] = 0.0;
+        const double vt0 = in->vt0[p - 1], vb = a.vb[p];
+        const double vmax = (vb + std::sqrt(vb * vb + 2.0 * vt0 * vt0)) / 2.0;   // lines 268-286
+        a.fmax[p] = vmax * std::exp(-(vmax - vb) * (vmax - vb) / (vt0 * vt0));
+      }
+    }
+    a.half_Nm_m[p] = 0.5 * sp.Nm * sp.mass;
+    a.Nm_q[p] = sp.Nm * sp.charge;
+    a.ni0[p] = in->ni0[p - 1];
+    a.dN_cur[p] = in->I_inj * (double)in->ns_inj * dt * in->ni0[p - 1] / (qe * ctx->sp[0].Nm) / 1.0 / 1.0;
+    N[p] = aopt == 2 ? N_inj_lost[p - 1] : 0;
+    a.N_inj[p] = N[p];
+    a.src[p - 1] = sp.src;
+  }
+  // the counts of lines 47-77: exact in production (the rounding draws come
+  // from the counter-based stream), an upper bound in check mode (its draws
+  // interleave with the particles' and happen on the device)
+  const uint32_t k0 = (uint32_t)seed ^ 0x696e6a65u, k1 = (uint32_t)(seed >> 32);
+  int64_t total = 0;
+  for (int p = 1; p <= nsp; ++p) {
+    int flag = 0;
+    double dN = 0.0;
+    if (aopt == 2) {
+      if (p == 1 || (p == a.tag_neg && a.tag_neg > 0)) {
+        flag = 1;
+        int64_t tmp = 0;
+        for (int q = 2; q <= nsp; ++q) tmp += N[q];
+        if (a.tag_neg > 0) tmp = tmp - N[a.tag_neg];
+        dN = a.ni0[p] * (double)tmp;
+      }
+    } else {
+      flag = 1;
+      dN = a.dN_cur[p];
+    }
+    if (flag == 1) {
+      int64_t n = (int64_t)dN;
+      if (mode == LEPIC_MCC_CHECK) {
+        n = n + 1;
+      } else {
+        const U4 w = philox4x32_10(U4{0xFFFFFFFDu, (uint32_t)p, (uint32_t)it, 0u}, k0, k1);
+        if (u01(w.x, w.y) <= dN - (double)n) n = n + 1;
+      }
+      N[p] = n;
+    }
+    if (mode == LEPIC_MCC_CHECK) {
+      total += N[p];
+    } else if (aopt == 3 || (aopt == 4 && in->flag_nmn == 0)) {
+      for (int q = 2; q <= nsp; ++q) N[q] = 0;
+    }
+  }
+  if (mode != LEPIC_MCC_CHECK) {
+    a.pair_off[1] = 0;
+    for (int p = 1; p <= nsp; ++p) {
+      a.n[p] = N[p];
+      a.off[p] = total;
+      total += N[p];
+      a.pair_off[p + 1] = a.pair_off[p] + (N[p] + 1) / 2;
+    }
+  }
+  if (int rc = reserve_srctally(ctx)) return rc;
+  if (int rc = reserve_stage(ctx, std::max<int64_t>(total, 1))) return rc;
+  SrcTally* t = (SrcTally*)ctx->d_srctally;
+  {
+    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    SrcTally h{};
+    h.seed = mode == LEPIC_MCC_CHECK ? *iseed : 0;
+    std::memcpy(ctx->h_pinned, &h, sizeof(h));
+    LEPIC_HIP(hipMemcpyAsync(t, ctx->h_pinned, sizeof(h), hipMemcpyHostToDevice, ctx->stream));
+  }
+  if (int rc = zero_async(ctx, ctx->d_stype, (size_t)std::max<int64_t>(total, 1))) return rc;
+  a.stage = ctx->d_stage6; a.type = ctx->d_stype; a.t = t; a.ploss1 = t->P_loss1;
+  a.stage_cap = total;
+  a.k0 = k0; a.k1 = k1; a.it = (uint32_t)it;
+  if (total > 0 || mode == LEPIC_MCC_CHECK) {
+    if (mode == LEPIC_MCC_CHECK) {
+      a.max_tries = 1 << 20;
+      hipLaunchKernelGGL(k_inject_check, dim3(1), dim3(64), 0, ctx->stream, a);
+    } else {
+      a.max_tries = 1 << 16;
+      const int64_t pairs = a.pair_off[nsp + 1];
+      if (pairs > 0)
+        hipLaunchKernelGGL(k_inject_prod, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, ctx->stream, a);
+    }
+    LEPIC_HIP(hipGetLastError());
+  }
+  LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, t, sizeof(SrcTally), hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  SrcTally h;
+  std::memcpy(&h, ctx->h_pinned, sizeof(h));
+  if (h.status == 1) return ctx->fail(LEPIC_ERR_STATE, "injection: no admissible position in the injection region");
+  if (h.status) return ctx->fail(LEPIC_ERR_CAPACITY, "injection: staging overflow");
+  int64_t seg0[kMaxSpecies], added[kMaxSpecies];
+  bool dense[kMaxSpecies];
+  if (int rc = append_staged(ctx, total, seg0, added, dense)) return rc;
+  if (deposit) {
+    // injection precedes the deposition (main.f90:1063-1085): the new
+    // particles join this step's density
+    for (int q = 0; q < nsp; ++q) {
+      if (!added[q]) continue;
+      if (dense[q]) {
+        if (int rc = deposit_launch(ctx, q)) return rc;
+      } else if (int rc = deposit_range_launch(ctx, q, seg0[q], (added[q] + kChunk - 1) / kChunk)) {
+        return rc;
+      }
+    }
+  }
+  if (mode == LEPIC_MCC_CHECK) *iseed = h.seed;
+  if (info) {
+    std::memset(info, 0, sizeof(*info));
+    for (int q = 0; q < nsp && q < 8; ++q) {
+      info->injected[q] = added[q];
+      info->P_loss4[q] = h.P_loss4[q + 1];
+      info->P_loss1[q] = h.P_loss1[q + 1];
+    }
     info->seed = h.seed;
   }
   return 0;

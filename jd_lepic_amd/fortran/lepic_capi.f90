@@ -57,6 +57,18 @@ module lepic_capi
      integer(c_int32_t) :: n_holes, n_icp, ptype_fx, tag_b, flag_chmfrd, ns_Hm
   end type lepic_emitter
 
+  type, bind(C) :: lepic_injection
+     integer(c_int32_t) :: opt_inj, flag_c, flag_nmn, flag_Tp, flag_diffsrc, ns_inj
+     real(c_double) :: I_inj, xl_pow, xr_pow, yl_pow, yr_pow, xa, dr, THm, th_B
+     type(c_ptr) :: ni0, vt0
+  end type lepic_injection
+
+  type, bind(C) :: lepic_injection_info
+     integer(c_int64_t) :: injected(8)
+     real(c_double) :: P_loss4(8), P_loss1(8)
+     integer(c_int32_t) :: seed, reserved
+  end type lepic_injection_info
+
   type, bind(C) :: lepic_emitter_info
      integer(c_int64_t) :: Nh, created(8), cnt_b(2)
      real(c_double) :: P_loss4(8), Lgrd, dNh
@@ -212,6 +224,18 @@ module lepic_capi
        integer(c_int32_t), intent(inout) :: iseed
        type(lepic_emitter_info), intent(out) :: info
      end function lepic_flux_emitter
+     ! volume injection (part_injection.f90:1-237), the species loop of main.f90:1063-1071
+     integer(c_int) function lepic_part_injection(ctx, inj, N_inj, it, mode, seed, iseed, deposit, info) &
+          bind(C, name='lepic_part_injection')
+       import :: c_int, c_ptr, c_int32_t, c_int64_t, lepic_injection, lepic_injection_info
+       type(c_ptr), value :: ctx
+       type(lepic_injection), intent(in) :: inj
+       integer(c_int64_t), intent(in) :: N_inj(*)
+       integer(c_int32_t), value :: it, mode, deposit
+       integer(c_int64_t), value :: seed
+       integer(c_int32_t), intent(inout) :: iseed
+       type(lepic_injection_info), intent(out) :: info
+     end function lepic_part_injection
      ! averaged moments (utils.f90:1-265) and backup files (main.f90:1102-1132, restart.f90)
      integer(c_int) function lepic_moments_accumulate(ctx, size_na) bind(C, name='lepic_moments_accumulate')
        import :: c_int, c_ptr, c_int32_t
