@@ -193,11 +193,12 @@ def test_production_repeats_per_seed_and_step():
     assert np.array_equal(outs[0], outs[1]) and not np.array_equal(outs[0][20:60], outs[2][20:60])
 
 
-@pytest.mark.parametrize("cap", [30000, 6600])
+@pytest.mark.parametrize("cap", [30000, 6140])
 def test_injection_after_push_deposits_once(cap):
     """lepic_step order: push + deposit, then injection with deposit = 1
     adds the new particles' charge (fresh segments, cap 30000, or the dense
-    fall-back after a pack, cap 6600): np counts every particle once."""
+    fall-back after a pack, the electrons at cap 6140): np counts every
+    particle once."""
     print(f"injection deposit cap={cap}", flush=True)
     inj = injection(1, I_inj=0.1)
     res = {p: residents(3000, 40 + p) for p in (1, 2, 3)}
