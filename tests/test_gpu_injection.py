@@ -75,6 +75,15 @@ def oracle_run(res, inj, iseed, tag_neg=0, N_inj=None, phi=None, cap=30000):
     return out, parts
 
 
+def close(got, want):
+    """Within a few ulp of the column's scale: the device's log/sin/cos may
+    differ from the host's in the last bit, and a beam offset added to a
+    drawn vz can cancel most of it."""
+    for c in range(6):
+        scale = np.abs(want[:, c]).max() if want.shape[0] else 0.0
+        np.testing.assert_allclose(got[:, c], want[:, c], rtol=1e-13, atol=1e-14 * scale + 1e-18)
+
+
 CASES = {
     "slit": dict(opt=1),
     "disk": dict(opt=1, flag_c=1),
@@ -113,7 +122,7 @@ def test_check_mode_matches_oracle(case):
         n0, n = res[p].shape[0], parts[p]["np_tot"]
         assert got["injected"][p - 1] == want["injected"][p] == n - n0
         assert down[p].shape[0] == n and np.array_equal(down[p][:n0], res[p])
-        np.testing.assert_allclose(down[p][n0:], parts[p]["vxp"][n0:n], rtol=1e-13, atol=1e-18)
+        close(down[p][n0:], parts[p]["vxp"][n0:n])
         assert np.array_equal(src[p], parts[p]["src"])
         np.testing.assert_allclose(got["P_loss4"][p - 1], want["P_loss4"][p], rtol=1e-12)
         np.testing.assert_allclose(got["P_loss1"][p - 1], want["P_loss1"][p], rtol=1e-12)
