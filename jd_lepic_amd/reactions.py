@@ -78,6 +78,19 @@ class Chemistry:
     def masses(self) -> dict:
         return {k + 1: s.mass for k, s in enumerate(self.species)}
 
+    def tags(self) -> dict:
+        """tag_neg, tag_neu, tag_b of read_reactions.f90:187-189: the last
+        particle named [H-]/[D-], [H]/[D], [Hb]/[Db] (0 when absent)."""
+        out = dict(tag_neg=0, tag_neu=0, tag_b=0)
+        for k, sp in enumerate(self.species[1:], start=2):
+            if sp.name in ("[H-]", "[D-]"):
+                out["tag_neg"] = k
+            if sp.name in ("[H]", "[D]"):
+                out["tag_neu"] = k
+            if sp.name in ("[Hb]", "[Db]"):
+                out["tag_b"] = k
+        return out
+
 
 def _rtype(word: str) -> int:
     w = word.strip()

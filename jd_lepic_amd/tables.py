@@ -18,6 +18,54 @@ import numpy as np
 from .reactions import QE, Chemistry
 
 
+def _table_on_grid(E: np.ndarray, S: np.ndarray, Er: np.ndarray) -> np.ndarray:
+    """One reaction's table on the union grid (read_reactions.f90:601-631),
+    vectorised: union point Ekp takes the first interval [E(j), E(j+1)] that
+    holds it -- the one ending at the first table point >= Ekp -- as
+    sL + (Ekp-EkL)*(sR-sL)/(EkR-EkL), the first value below the table, and
+    past the table the value of the previous union point.  Tables whose
+    first points repeat, or with a single point, take the point-by-point
+    scan of the reference (its divisions by a zero interval included)."""
+    n = len(E)
+    if n < 2 or E[0] == E[1]:
+        return _table_on_grid_scan(E, S, Er)
+    idx = np.searchsorted(E, Er, side="left")
+    out = np.empty_like(Er)
+    below = idx == 0
+    out[below] = S[0]
+    at0 = below & (Er == E[0])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        out[at0] = S[0] + (Er[at0] - E[0]) * (S[1] - S[0]) / (E[1] - E[0])
+        mid = (idx > 0) & (idx < n)
+        j = idx[mid] - 1
+        EkL, EkR, sL, sR = E[j], E[j + 1], S[j], S[j + 1]
+        out[mid] = sL + (Er[mid] - EkL) * (sR - sL) / (EkR - EkL)
+    past = idx == n
+    if past.any():
+        k_last = np.searchsorted(Er, E[-1], side="left")
+        out[past] = out[k_last]
+    return out
+
+
+def _table_on_grid_scan(E, S, Er):
+    out = np.zeros_like(Er)
+    sig_p = 0.0
+    for ip, Ekp in enumerate(Er):
+        for j in range(len(E)):
+            EkL, sL = E[j], S[j]
+            EkR = E[j + 1] if j + 1 < len(E) else 0.0
+            sR = S[j + 1] if j + 1 < len(E) else 0.0
+            if Ekp < EkL:
+                sig_p = sL
+                break
+            if EkL <= Ekp <= EkR:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    sig_p = sL + (Ekp - EkL) * (sR - sL) / (EkR - EkL)
+                break
+        out[ip] = sig_p
+    return out
+
+
 def ordering(chem: Chemistry) -> dict:
     rs = chem.reactions
     ncol = len(rs)
@@ -25,20 +73,7 @@ def ordering(chem: Chemistry) -> dict:
     npt = len(Er)
     sig = np.zeros((ncol, npt))
     for c, r in enumerate(rs):
-        E, S = r.E, r.S
-        sig_p = 0.0
-        for ip, Ekp in enumerate(Er):
-            for j in range(len(E)):
-                EkL, sL = E[j], S[j]
-                EkR = E[j + 1] if j + 1 < len(E) else 0.0
-                sR = S[j + 1] if j + 1 < len(E) else 0.0
-                if Ekp < EkL:
-                    sig_p = sL
-                    break
-                if EkL <= Ekp <= EkR:
-                    sig_p = sL + (Ekp - EkL) * (sR - sL) / (EkR - EkL)
-                    break
-            sig[c, ip] = sig_p
+        sig[c] = _table_on_grid(np.asarray(r.E, float), np.asarray(r.S, float), Er)
     nt = chem.ntype_all
     masses = chem.masses()
     sigv = np.zeros((nt + 1, max(ncol, 1)))
