@@ -21,8 +21,15 @@ using namespace lepic;
 
 namespace {
 
-constexpr int kWinRows = 4;
-constexpr int kWinCols = 128;
+// LDS deposition window of a segment: kWinRows x kWinCols nodes from
+// kWinX0 columns left and kWinY0 rows below the segment's first cell (the
+// particles of a segment sit in a few consecutive cells of one row when
+// sorted, and drift a few cells between sorts); nodes outside go straight to
+// the global density with FP64 atomics
+constexpr int kWinRows = 16;
+constexpr int kWinCols = 64;
+constexpr int kWinX0 = 8;
+constexpr int kWinY0 = 7;
 constexpr int kWin = kWinRows * kWinCols;
 // threads per mover block; a block owns one segment of kChunk slots
 constexpr int kMoverThreadsMixed = kBlock;
@@ -242,15 +249,32 @@ __device__ __forceinline__ void seg_sum4(float w[4], int s) {
   }
 }
 
+// Weights of one cell straight to the global density (nodes outside the LDS
+// window), with the kq factor of the flush.
+__device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int iy, float q0, float q1, float q2,
+                                            float q3, double scale) {
+  const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
+  const int64_t nd[4] = {n00, n00 + 1, n01 + 1, n01};
+  const float q[4] = {q0, q1, q2, q3};
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double kq = a.bc[nd[c]] != -1 ? 2.0 : 1.0;
+    atomic_add_f64(&a.np[nd[c]], kq * ((double)q[c] * scale));
+  }
+}
+
 // CIC deposition of thread-major items (consecutive particles per thread):
 // equal cells within a thread are summed first, then runs of equal cells
-// across lanes per item slot; slots that hold nothing in the whole wave are
-// skipped.  Run sums go to the LDS window (or to global atomics outside it).
+// across adjacent lanes per item slot by a segmented DPP scan (cheaper than
+// the LDS atomics it saves even when runs are short); slots empty in the
+// whole wave are skipped.  Window-local cell indices keep the node
+// arithmetic to adds; cells whose nodes leave the window take the global
+// path.
 __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win, int wx0, int wy0,
                                                const int C[kItems], const float FX[kItems],
                                                const float FY[kItems], const bool ok[kItems]) {
   const int lane = threadIdx.x & 63;
-  int key[kItems];
+  int key[kItems];          // window-local cell, -1: nothing in this slot
   float w[kItems][4];
   int prev = -1;
 #pragma unroll
@@ -260,9 +284,14 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
     if (!ok[k]) continue;
     int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
     clamp_cell(ix, iy, a.n1, a.n2);
-    const int kk = (int)node_index(ix, iy, a.n1);
     const float px = 1.0f - FX[k], py = 1.0f - FY[k];
     const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
+    const int cx = ix - wx0, cy = iy - wy0;
+    if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
+      deposit_global(a, ix, iy, q0, q1, q2, q3, a.k1dep);
+      continue;
+    }
+    const int kk = cy * kWinCols + cx;
     if (kk == prev) {
       // same cell as this thread's previous particle: add into its slot
 #pragma unroll
@@ -287,8 +316,10 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
     seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
     if (tail && key[k] >= 0) {
-      const int ix = key[k] % (a.n1 + 3), iy = key[k] / (a.n1 + 3);
-      deposit_cell(a, win, wx0, wy0, ix, iy, v, a.k1dep);
+      atomicAdd(&win[kk], v[0]);
+      atomicAdd(&win[kk + 1], v[1]);
+      atomicAdd(&win[kk + kWinCols + 1], v[2]);
+      atomicAdd(&win[kk + kWinCols], v[3]);
     }
   }
 }
@@ -314,8 +345,8 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     if (t == 0) {
       int ix0 = 1, iy0 = 1;
       if (cnt_in > 0) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
-      s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
-      s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+      s_wx0 = ix0 - kWinX0 < 0 ? 0 : ix0 - kWinX0;
+      s_wy0 = iy0 - kWinY0 < 0 ? 0 : iy0 - kWinY0;
     }
   }
 
@@ -471,8 +502,8 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     if (t == 0) {
       int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
       int ix0 = c0 & 0xFFFF, iy0 = c0 >> 16;
-      s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
-      s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+      s_wx0 = ix0 - kWinX0 < 0 ? 0 : ix0 - kWinX0;
+      s_wy0 = iy0 - kWinY0 < 0 ? 0 : iy0 - kWinY0;
     }
   }
   // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
@@ -628,8 +659,8 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
   if (t == 0) {
     int ix0 = 1, iy0 = 1;
     if (cnt_in > 0) { ix0 = (int)(in.x[base] / a.h1) + 1; iy0 = (int)(in.y[base] / a.h2) + 1; }
-    s_wx0 = ix0 - 2 < 0 ? 0 : ix0 - 2;
-    s_wy0 = iy0 - 1 < 0 ? 0 : iy0 - 1;
+    s_wx0 = ix0 - kWinX0 < 0 ? 0 : ix0 - kWinX0;
+    s_wy0 = iy0 - kWinY0 < 0 ? 0 : iy0 - kWinY0;
   }
   __syncthreads();
   const int wx0 = s_wx0, wy0 = s_wy0;
@@ -669,8 +700,8 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx i
   for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
   if (t == 0) {
     int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
-    s_wx0 = (c0 & 0xFFFF) - 2 < 0 ? 0 : (c0 & 0xFFFF) - 2;
-    s_wy0 = (c0 >> 16) - 1 < 0 ? 0 : (c0 >> 16) - 1;
+    s_wx0 = (c0 & 0xFFFF) - kWinX0 < 0 ? 0 : (c0 & 0xFFFF) - kWinX0;
+    s_wy0 = (c0 >> 16) - kWinY0 < 0 ? 0 : (c0 >> 16) - kWinY0;
   }
   const int64_t i0 = base + (int64_t)kItems * t;
   const int nv = min(max(cnt_in - kItems * t, 0), kItems);
