@@ -185,10 +185,13 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cn
 // Offsets for thread-major items (thread t holds slots kItems*t + k): the
 // same order-preserving compaction as compact_offsets.  Returns the number
 // of survivors of the segment.
+// extra: a per-wave count summed over the block through the same exchange
+// (the deposition's run count), returned in *extra_total.
 template <int BT>
 __device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t bid, int cnt_in,
-                                                  const bool surv[kItems], uint32_t off[kItems]) {
-  __shared__ uint32_t s_wave[BT / 64];
+                                                  const bool surv[kItems], uint32_t off[kItems],
+                                                  uint32_t extra, uint32_t* extra_total) {
+  __shared__ uint32_t s_wave[BT / 64], s_extra[BT / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t mine = 0;
 #pragma unroll
@@ -200,15 +203,17 @@ __device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t 
     const uint32_t o = __shfl_up(incl, d, 64);
     if (lane >= d) incl += o;
   }
-  if (lane == 63) s_wave[w] = incl;
+  if (lane == 63) { s_wave[w] = incl; s_extra[w] = extra; }
   __syncthreads();
-  uint32_t before = incl - mine, total = 0;
+  uint32_t before = incl - mine, total = 0, ex = 0;
 #pragma unroll
   for (int v = 0; v < BT / 64; ++v) {
     const uint32_t c = s_wave[v];
     if (v < w) before += c;
     total += c;
+    ex += s_extra[v];
   }
+  *extra_total = ex;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     off[k] = before;
@@ -322,6 +327,90 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
       atomicAdd(&win[kk + kWinCols], v[3]);
     }
   }
+}
+
+// Block-level counting sort of the segment's items by window-local cell
+// before the deposition (LDS: histogram, exclusive scan, stable-per-run
+// scatter), so that equal cells sit in adjacent lanes and slots whatever
+// order the particles are in; the particles themselves keep their order.
+// Pays off once pushes have scattered a sorted segment (electrons between
+// sorts); k_mover_mixed chooses it per block from the run count.
+// key[k]: window-local cell, -1 for no item; s_hist zeroed by the caller
+// and ordered by a barrier.
+template <int BT>
+__device__ __forceinline__ void deposit_sorted(const MoverCommon& a, float* win, int wx0, int wy0,
+                                               const int key[kItems], const float FX[kItems],
+                                               const float FY[kItems], uint32_t* s_hist) {
+  static_assert(kWin == 4 * BT, "one thread per four window cells");
+  __shared__ float s_fx[BT * kItems], s_fy[BT * kItems];
+  __shared__ uint16_t s_key[BT * kItems];
+  __shared__ uint32_t s_tot[BT / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  uint64_t heads[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const int kk = key[k] >= 0 ? key[k] : -2 - lane;
+    const int kp = __shfl_up(kk, 1, 64);
+    heads[k] = __ballot(lane == 0 || kp != kk);
+    const uint64_t above = heads[k] & ~((2ull << lane) - 1ull);
+    const int next = above ? __builtin_ctzll(above) : 64;
+    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) atomicAdd(&s_hist[kk], (uint32_t)(next - lane));
+  }
+  __syncthreads();
+  const uint32_t h0 = s_hist[4 * t], h1 = s_hist[4 * t + 1], h2 = s_hist[4 * t + 2], h3 = s_hist[4 * t + 3];
+  const uint32_t mine = h0 + h1 + h2 + h3;
+  uint32_t incl = mine;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += o;
+  }
+  if (lane == 63) s_tot[wv] = incl;
+  __syncthreads();
+  uint32_t base = incl - mine, M = 0;
+#pragma unroll
+  for (int v = 0; v < BT / 64; ++v) {
+    const uint32_t c = s_tot[v];
+    if (v < wv) base += c;
+    M += c;
+  }
+  s_hist[4 * t] = base;
+  s_hist[4 * t + 1] = base + h0;
+  s_hist[4 * t + 2] = base + h0 + h1;
+  s_hist[4 * t + 3] = base + h0 + h1 + h2;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint64_t above = heads[k] & ~((2ull << lane) - 1ull);
+    const int next = above ? __builtin_ctzll(above) : 64;
+    uint32_t start = 0;
+    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) start = atomicAdd(&s_hist[key[k]], (uint32_t)(next - lane));
+    const int s = 63 - __builtin_clzll(heads[k] & ((2ull << lane) - 1ull));
+    const uint32_t st = __shfl(start, s, 64);
+    if (key[k] >= 0) {
+      const uint32_t pos = st + (uint32_t)(lane - s);
+      s_fx[pos] = FX[k];
+      s_fy[pos] = FY[k];
+      s_key[pos] = (uint16_t)key[k];
+    }
+  }
+  __syncthreads();
+  int C2[kItems];
+  float FX2[kItems], FY2[kItems];
+  bool ok2[kItems];
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    const uint32_t j = (uint32_t)(kItems * t + k);
+    ok2[k] = j < M;
+    C2[k] = 0; FX2[k] = 0.f; FY2[k] = 0.f;
+    if (ok2[k]) {
+      const int kk = s_key[j];
+      C2[k] = (((kk / kWinCols) + wy0) << 16) | ((kk % kWinCols) + wx0);
+      FX2[k] = s_fx[j];
+      FY2[k] = s_fy[j];
+    }
+  }
+  deposit_tmajor(a, win, wx0, wy0, C2, FX2, FY2, ok2);
 }
 
 // ---------------------------------------------------------------------------
@@ -490,22 +579,18 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
                                                         const float* __restrict__ E,
                                                         const float* __restrict__ B) {
   __shared__ float win[kWin];
-  __shared__ int s_wx0, s_wy0;
+  __shared__ uint32_t s_hist[kWin];
   const int64_t bid = blockIdx.x;
   const int cnt_in = a.seg_in[bid];
   const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const float k1 = m.k1f;
-  if (a.deposit) {
-    for (int e = t; e < kWin; e += BT) win[e] = 0.0f;
-    if (t == 0) {
-      int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
-      int ix0 = c0 & 0xFFFF, iy0 = c0 >> 16;
-      s_wx0 = ix0 - kWinX0 < 0 ? 0 : ix0 - kWinX0;
-      s_wy0 = iy0 - kWinY0 < 0 ? 0 : iy0 - kWinY0;
-    }
-  }
+  // window anchor from the segment's first cell, computed by every thread
+  const int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
+  const int wx0 = max((c0 & 0xFFFF) - kWinX0, 0), wy0 = max((c0 >> 16) - kWinY0, 0);
+  if (a.deposit)
+    for (int e = t; e < kWin; e += BT) { win[e] = 0.0f; s_hist[e] = 0u; }
   // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
   // the arrays are padded to whole segments, so the 16-byte loads stay inside
   // them also past the segment's count
@@ -619,8 +704,35 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     FX[k] = fx; FY[k] = fy; Z[k] = zp; VX[k] = vx; VY[k] = vy; VZ[k] = vz;
     surv[k] = true;
   }
-  uint32_t off[kItems];
-  const uint32_t kept = compact_offsets_tmajor<BT>(a, bid, cnt_in, surv, off);
+  // the deposition's window-local keys (cells leaving the window go to the
+  // global density here) and the wave's number of runs of equal keys
+  int dkey[kItems];
+  uint32_t runs = 0;
+  if (a.deposit) {
+    const int lane = t & 63;
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      dkey[k] = -1;
+      if (!surv[k]) continue;
+      int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
+      clamp_cell(ix, iy, n1, n2);
+      const int cx = ix - wx0, cy = iy - wy0;
+      if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
+        const float px = 1.0f - FX[k], py = 1.0f - FY[k];
+        deposit_global(a, ix, iy, px * py, FX[k] * py, FX[k] * FY[k], px * FY[k], a.k1dep);
+        continue;
+      }
+      dkey[k] = cy * kWinCols + cx;
+    }
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      const int kk = dkey[k] >= 0 ? dkey[k] : -2 - lane;
+      const int kp = __shfl_up(kk, 1, 64);
+      runs += __popcll(__ballot(dkey[k] >= 0 && (lane == 0 || kp != kk)));
+    }
+  }
+  uint32_t off[kItems], block_runs = 0;
+  const uint32_t kept = compact_offsets_tmajor<BT>(a, bid, cnt_in, surv, off, runs, &block_runs);
   if (kept == (uint32_t)cnt_in && nv == kItems) {
     // nothing of the segment was lost: every survivor keeps its slot
     *(int4*)(out.cell + i0) = make_int4(C[0], C[1], C[2], C[3]);
@@ -641,9 +753,17 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     }
   }
   if (!a.deposit) return;
-  deposit_tmajor(a, win, s_wx0, s_wy0, C, FX, FY, surv);
+  if (4 * block_runs > kept) {
+    // fewer than four items per run on average: sort the items by cell first
+    deposit_sorted<BT>(a, win, wx0, wy0, dkey, FX, FY, s_hist);
+  } else {
+    bool ok[kItems];
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) ok[k] = dkey[k] >= 0;
+    deposit_tmajor(a, win, wx0, wy0, C, FX, FY, ok);
+  }
   __syncthreads();
-  flush_window(a, win, s_wx0, s_wy0, a.k1dep);
+  flush_window(a, win, wx0, wy0, a.k1dep);
 }
 
 // ---------------------------------------------------------------------------
