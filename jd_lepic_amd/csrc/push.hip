@@ -329,6 +329,52 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
   }
 }
 
+// Items whose nodes leave the LDS window (a segment that continues on the
+// next row after the end of one, or particles far from their segment):
+// merged like deposit_tmajor -- within a thread, then runs across lanes --
+// and added to the global density per run.  gk: packed cell (iy<<16)|ix or
+// -1.  Skipped by waves that have none.
+__device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const int gk[kItems],
+                                                    const float FX[kItems], const float FY[kItems]) {
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) any = any || gk[k] >= 0;
+  if (__ballot(any) == 0) return;                 // wave-uniform
+  const int lane = threadIdx.x & 63;
+  int key[kItems];
+  float w[kItems][4];
+  int prev = -1;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    key[k] = -1;
+    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0.f;
+    if (gk[k] < 0) continue;
+    const float px = 1.0f - FX[k], py = 1.0f - FY[k];
+    const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
+    if (gk[k] == prev) {
+#pragma unroll
+      for (int j = k - 1; j >= 0; --j)
+        if (key[j] == gk[k]) { w[j][0] += q0; w[j][1] += q1; w[j][2] += q2; w[j][3] += q3; break; }
+    } else {
+      key[k] = gk[k];
+      w[k][0] = q0; w[k][1] = q1; w[k][2] = q2; w[k][3] = q3;
+      prev = gk[k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (__ballot(key[k] >= 0) == 0) continue;     // wave-uniform
+    const int kk = key[k] >= 0 ? key[k] : -2 - lane;
+    const int kp = __shfl_up(kk, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || kp != kk);
+    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+    float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
+    seg_sum4(v, s);
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (tail && key[k] >= 0) deposit_global(a, key[k] & 0xFFFF, key[k] >> 16, v[0], v[1], v[2], v[3], a.k1dep);
+  }
+}
+
 // Block-level counting sort of the segment's items by window-local cell
 // before the deposition (LDS: histogram, exclusive scan, stable-per-run
 // scatter), so that equal cells sit in adjacent lanes and slots whatever
@@ -706,23 +752,20 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   }
   // the deposition's window-local keys (cells leaving the window go to the
   // global density here) and the wave's number of runs of equal keys
-  int dkey[kItems];
+  int dkey[kItems], gkey[kItems];
   uint32_t runs = 0;
   if (a.deposit) {
     const int lane = t & 63;
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
       dkey[k] = -1;
+      gkey[k] = -1;
       if (!surv[k]) continue;
       int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
       clamp_cell(ix, iy, n1, n2);
       const int cx = ix - wx0, cy = iy - wy0;
-      if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
-        const float px = 1.0f - FX[k], py = 1.0f - FY[k];
-        deposit_global(a, ix, iy, px * py, FX[k] * py, FX[k] * FY[k], px * FY[k], a.k1dep);
-        continue;
-      }
-      dkey[k] = cy * kWinCols + cx;
+      if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) gkey[k] = (iy << 16) | ix;
+      else dkey[k] = cy * kWinCols + cx;
     }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -753,6 +796,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     }
   }
   if (!a.deposit) return;
+  deposit_global_runs(a, gkey, FX, FY);
   if (4 * block_runs > kept) {
     // fewer than four items per run on average: sort the items by cell first
     deposit_sorted<BT>(a, win, wx0, wy0, dkey, FX, FY, s_hist);
