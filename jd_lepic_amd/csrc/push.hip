@@ -381,8 +381,7 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
 // order the particles are in; the particles themselves keep their order.
 // Pays off once pushes have scattered a sorted segment (electrons between
 // sorts); k_mover_mixed chooses it per block from the run count.
-// key[k]: window-local cell, -1 for no item; s_hist zeroed by the caller
-// and ordered by a barrier.
+// key[k]: window-local cell, -1 for no item; s_hist: kWin words of LDS.
 template <int BT>
 __device__ __forceinline__ void deposit_sorted(const MoverCommon& a, float* win, int wx0, int wy0,
                                                const int key[kItems], const float FX[kItems],
@@ -392,15 +391,20 @@ __device__ __forceinline__ void deposit_sorted(const MoverCommon& a, float* win,
   __shared__ uint16_t s_key[BT * kItems];
   __shared__ uint32_t s_tot[BT / 64];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int e = t; e < kWin; e += BT) s_hist[e] = 0u;
   uint64_t heads[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     const int kk = key[k] >= 0 ? key[k] : -2 - lane;
     const int kp = __shfl_up(kk, 1, 64);
     heads[k] = __ballot(lane == 0 || kp != kk);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
     const uint64_t above = heads[k] & ~((2ull << lane) - 1ull);
     const int next = above ? __builtin_ctzll(above) : 64;
-    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) atomicAdd(&s_hist[kk], (uint32_t)(next - lane));
+    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) atomicAdd(&s_hist[key[k]], (uint32_t)(next - lane));
   }
   __syncthreads();
   const uint32_t h0 = s_hist[4 * t], h1 = s_hist[4 * t + 1], h2 = s_hist[4 * t + 2], h3 = s_hist[4 * t + 3];
@@ -636,7 +640,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   const int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
   const int wx0 = max((c0 & 0xFFFF) - kWinX0, 0), wy0 = max((c0 >> 16) - kWinY0, 0);
   if (a.deposit)
-    for (int e = t; e < kWin; e += BT) { win[e] = 0.0f; s_hist[e] = 0u; }
+    for (int e = t; e < kWin; e += BT) win[e] = 0.0f;
   // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
   // the arrays are padded to whole segments, so the 16-byte loads stay inside
   // them also past the segment's count
