@@ -406,7 +406,7 @@ int lepic_destroy(lepic_ctx* ctx) {
                   ctx->d_plist, ctx->d_scan_tmp, ctx->rx.d_sig_Er, ctx->rx.d_sig,
                   ctx->rx.d_sig_Er_f, ctx->rx.d_sig_f, ctx->rx.d_cnt_col, ctx->d_claim,
                   ctx->d_mcc_stage, ctx->d_mcc_i64, ctx->d_mcc_f64, ctx->d_mcc_tab, ctx->d_heat,
-                  ctx->d_stage6, ctx->d_stype, ctx->d_srank, ctx->d_srctally};
+                  ctx->d_stage6, ctx->d_stype, ctx->d_srank, ctx->d_srctally, ctx->d_cellwall};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   for (int l = 2; l <= kMaxLevels; ++l) {
     if (ctx->lv[l].u) (void)hipFree(ctx->lv[l].u);
@@ -494,6 +494,15 @@ int lepic_set_boundary(lepic_ctx* ctx, const int32_t* bcnd, const double* phi) {
   std::vector<int8_t> b8(ctx->nodes);
   for (int64_t i = 0; i < ctx->nodes; ++i) b8[i] = (int8_t)bcnd[i];
   LEPIC_HIP(hipMemcpyAsync(ctx->bc, b8.data(), ctx->nodes, hipMemcpyHostToDevice, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  // the particle-loss test of part_expmover.f90:254-258 as one byte per cell
+  std::vector<int8_t> cw(ctx->nodes, 0);
+  for (int iy = 0; iy <= n2 + 1; ++iy)
+    for (int ix = 0; ix <= n1 + 1; ++ix)
+      cw[(int64_t)iy * (n1 + 3) + ix] =
+          (at(ix, iy) >= 1 && at(ix + 1, iy) >= 1 && at(ix + 1, iy + 1) >= 1 && at(ix, iy + 1) >= 1) ? 1 : 0;
+  if (!ctx->d_cellwall) LEPIC_HIP(hipMalloc(&ctx->d_cellwall, ctx->nodes));
+  LEPIC_HIP(hipMemcpyAsync(ctx->d_cellwall, cw.data(), ctx->nodes, hipMemcpyHostToDevice, ctx->stream));
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipMemcpyAsync(ctx->phi, phi, ctx->nodes * 8, hipMemcpyHostToDevice, ctx->stream));
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));

@@ -135,6 +135,7 @@ struct lepic_ctx {
   // grids
   int64_t nodes = 0, rhs_n = 0;
   int8_t* bc = nullptr;              // level-1 bcnd (int8)
+  int8_t* d_cellwall = nullptr;      // per node (ix,iy): cell (ix,iy) has four wall corners
   std::vector<int32_t> bc_host;      // int32 copy
   double* phi = nullptr;
   double* phi_given = nullptr;       // guess set through lepic_set_grid(PHI) for START_GIVEN

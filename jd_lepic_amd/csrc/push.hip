@@ -37,6 +37,7 @@ constexpr int kMoverThreads64 = kBlock;
 
 struct MoverCommon {
   const int8_t* bc;
+  const int8_t* cellwall;       // 1 where all four corners of cell (ix,iy) are walls
   int n1, n2;
   double h1, h2, dt, k1;        // k1 = dt*q/(2m)
   double Nm, mass, NmQ;         // NmQ = Nm*charge for cnt_dead
@@ -125,11 +126,6 @@ __device__ __forceinline__ void clamp_cell(int& ix, int& iy, int n1, int n2) {
   // check_outofbounds, part_expmover.f90:612-627
   ix = ix < 0 ? 0 : (ix > n1 + 1 ? n1 + 1 : ix);
   iy = iy < 0 ? 0 : (iy > n2 + 1 ? n2 + 1 : iy);
-}
-
-__device__ __forceinline__ bool all_walls(const int8_t* bc, int ix, int iy, int n1) {
-  const int64_t a = node_index(ix, iy, n1), b = node_index(ix, iy + 1, n1);
-  return bc[a] >= 1 && bc[a + 1] >= 1 && bc[b + 1] >= 1 && bc[b] >= 1;
 }
 
 // Lost-particle tallies, part_expmover.f90:314-332, 424-427.
@@ -569,7 +565,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     ix = (int)floor(xp / h1) + 1;  // FLOOR(), lines 244-245
     iy = (int)floor(yp / h2) + 1;
     clamp_cell(ix, iy, n1, n2);
-    if (all_walls(a.bc, ix, iy, n1)) {  // lines 254-258, 314-446
+    if (a.cellwall[node_index(ix, iy, n1)]) {  // lines 254-258, 314-446
       tally_lost(a, xp, yp, ix, iy, 0.5 * a.Nm * a.mass * (vx * vx + vy * vy + vz * vz));
       continue;
     }
@@ -719,7 +715,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     if (!a.magnetize) boris = false;
     if (boris) {
       float Bt2 = Bx * Bx + By * By + Bz * Bz;
-      float k2 = k1 * (2.0f / (1.0f + k1 * k1 * Bt2));
+      float k2 = (2.0f * k1) * __builtin_amdgcn_rcpf(1.0f + k1 * k1 * Bt2);
       float v1x = fmaf(k1, Ex, vx), v1y = fmaf(k1, Ey, vy), v1z = vz;
       float v3x = fmaf(k1, v1y * Bz - v1z * By, v1x);
       float v3y = fmaf(k1, v1z * Bx - v1x * Bz, v1y);
@@ -745,7 +741,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     iy += (int)sy;
     int cx = ix, cy = iy;
     clamp_cell(cx, cy, n1, n2);
-    if (all_walls(a.bc, cx, cy, n1)) {
+    if (a.cellwall[node_index(cx, cy, n1)]) {  // part_expmover.f90:254-258
       double xd = ((double)(ix - 1) + fx) * a.h1, yd = ((double)(iy - 1) + fy) * a.h2;
       tally_lost(a, xd, yd, cx, cy, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
       continue;
@@ -895,6 +891,7 @@ MoverCommon common_args(lepic_ctx* ctx, int p) {
   Species& s = ctx->sp[p];
   MoverCommon a{};
   a.bc = ctx->bc;
+  a.cellwall = ctx->d_cellwall;
   a.n1 = ctx->g.n1; a.n2 = ctx->g.n2;
   a.h1 = ctx->g.h1; a.h2 = ctx->g.h2; a.dt = ctx->g.dt;
   a.k1 = ctx->g.dt * s.charge / (2.0 * s.mass);  // part_expmover.f90:61
