@@ -314,7 +314,9 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
     const uint64_t heads = __ballot(lane == 0 || kp != kk);
     const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
     float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
-    seg_sum4(v, s);
+    // every active lane its own run (a slot holding only the cell changes of
+    // a few threads): nothing to merge
+    if (__popcll(heads & act) != __popcll(act)) seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
     if (tail && key[k] >= 0) {
       atomicAdd(&win[kk], v[0]);
