@@ -94,24 +94,25 @@ __global__ void k_keys_mixed(const int32_t* __restrict__ cell, const int32_t* __
   count_keys(valid, k, cnt);
 }
 
+// Each wave owns a contiguous quarter of the tile (kTile/4 keys, read 64 at a
+// time), so stability across waves is a per-digit prefix over the four
+// waves and no barrier is needed inside the loops.
 template <int R>   // buckets per pass
 __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, int64_t nslots, int shift,
                                               uint32_t* __restrict__ hist, int nb) {
-  __shared__ uint32_t cnt[R];
-  for (int q = threadIdx.x; q < R; q += 256) cnt[q] = 0;
+  __shared__ uint32_t wc[4][R];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  for (int q = t; q < R; q += 256) wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)w * (kTile / 4);
   for (int j = 0; j < kSortItems; ++j) {
-    const int64_t i = base + (int64_t)j * 256 + threadIdx.x;
+    const int64_t i = base + (int64_t)j * 64 + lane;
     const bool valid = i < nslots;
     const uint32_t d = valid ? (key[i] >> shift) & (uint32_t)(R - 1) : 0u;
-    // waves add to the shared table one at a time per round (no LDS atomics)
-    for (int w = 0; w < 4; ++w) {
-      if ((threadIdx.x >> 6) == w) wave_match(valid, d, cnt);
-      __syncthreads();
-    }
+    wave_match(valid, d, wc[w]);
   }
-  for (int q = threadIdx.x; q < R; q += 256) hist[(int64_t)q * nb + blockIdx.x] = cnt[q];
+  __syncthreads();
+  for (int q = t; q < R; q += 256) hist[(int64_t)q * nb + blockIdx.x] = wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
 }
 
 template <int R>
@@ -119,35 +120,41 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ki
                                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                  const uint32_t* __restrict__ hist, int64_t nslots, int shift,
                                                  int nb) {
-  __shared__ uint32_t run[R];
   __shared__ uint32_t wc[4][R];
-  const int t = threadIdx.x, w = t >> 6;
-  for (int q = t; q < R; q += 256) {
-    run[q] = hist[(int64_t)q * nb + blockIdx.x];
-    wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  for (int q = t; q < R; q += 256) wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)w * (kTile / 4);
+  // the wave's digit counts
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t i = base + (int64_t)j * 64 + lane;
+    const bool valid = i < nslots;
+    const uint32_t d = valid ? (kin[i] >> shift) & (uint32_t)(R - 1) : 0u;
+    wave_match(valid, d, wc[w]);
   }
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * kTile;
+  // first output position of each digit for each wave
+  for (int q = t; q < R; q += 256) {
+    const uint32_t g = hist[(int64_t)q * nb + blockIdx.x];
+    const uint32_t c0 = wc[0][q], c1 = wc[1][q], c2 = wc[2][q];
+    wc[0][q] = g;
+    wc[1][q] = g + c0;
+    wc[2][q] = g + c0 + c1;
+    wc[3][q] = g + c0 + c1 + c2;
+  }
+  __syncthreads();
   for (int j = 0; j < kSortItems; ++j) {
-    const int64_t i = base + (int64_t)j * 256 + t;
+    const int64_t i = base + (int64_t)j * 64 + lane;
     const bool valid = i < nslots;
     const uint32_t k = valid ? kin[i] : 0u;
     const uint32_t v = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     const uint32_t d = (k >> shift) & (uint32_t)(R - 1);
+    const uint32_t b = valid ? wc[w][d] : 0u;          // read before this round's counts are added
     const uint32_t r = wave_match(valid, d, wc[w]);
-    __syncthreads();
     if (valid) {
-      uint32_t pos = run[d] + r;
-      for (int q = 0; q < w; ++q) pos += wc[q][d];
-      kout[pos] = k;
-      vout[pos] = v;
+      kout[b + r] = k;
+      vout[b + r] = v;
     }
-    __syncthreads();
-    for (int q = t; q < R; q += 256) {
-      run[q] += wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
-      wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
-    }
-    __syncthreads();
   }
 }
 
