@@ -8,7 +8,7 @@
 //
 // With one thread the reference's counting sort is stable (particles keep
 // their order inside a cell); LSD radix sort with stable per-digit scatter
-// gives exactly that permutation.  Per pass: a digit histogram per 4096-key
+// gives exactly that permutation.  Per pass: a digit histogram per 16384-key
 // tile (wave-aggregated LDS counts), a device-wide exclusive scan of the
 // digit-major histogram, and a scatter that ranks keys inside the tile in
 // index order.
@@ -22,7 +22,7 @@ using namespace lepic;
 
 namespace {
 
-constexpr int kSortItems = 16;
+constexpr int kSortItems = 64;
 constexpr int kTile = 256 * kSortItems;
 constexpr int kMaxRadix = 2048;   // two passes cover cell indices below 2^22 (2048 x 2048 cells)
 
