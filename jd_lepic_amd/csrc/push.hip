@@ -644,8 +644,8 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   // them also past the segment's count
   const int64_t i0 = base + (int64_t)kItems * t;
   const int nv = min(max(cnt_in - kItems * t, 0), kItems);
-  int C[kItems];
-  float FX[kItems], FY[kItems], Z[kItems], VX[kItems], VY[kItems], VZ[kItems];
+  int C[kItems] = {};
+  float FX[kItems] = {}, FY[kItems] = {}, Z[kItems] = {}, VX[kItems] = {}, VY[kItems] = {}, VZ[kItems] = {};
   bool surv[kItems];
   uint32_t dead4 = 0;
   if (nv > 0) {
@@ -663,18 +663,29 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     VZ[0] = vz4.x; VZ[1] = vz4.y; VZ[2] = vz4.z; VZ[3] = vz4.w;
     Z[0] = z4.x; Z[1] = z4.y; Z[2] = z4.z; Z[3] = z4.w;
   }
+  // particles flagged dead by the MCC leave before the push
+  // (part_expmover.f90:84-98); rare, so one branch per thread
+  uint32_t deadm = 0;
+  if (dead4 != 0) {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+      if (k < nv && ((dead4 >> (8 * k)) & 0xFF) == 1) {
+        flag_dead[i0 + k] = 0;
+        tally_dead(a, 0.5 * a.Nm * a.mass *
+                          ((double)VX[k] * VX[k] + (double)VY[k] * VY[k] + (double)VZ[k] * VZ[k]));
+        deadm |= 1u << k;
+      }
+  }
+  // the push as straight-line code for every item slot; slots without a live
+  // particle are pushed from a harmless state and dropped
+  uint32_t lostm = 0;
+  int IXU[kItems], IYU[kItems];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    surv[k] = false;
-    if (k >= nv) continue;
-    const int64_t i = i0 + k;
-    float fx = FX[k], fy = FY[k], vx = VX[k], vy = VY[k], vz = VZ[k], zp = Z[k];
-    if (((dead4 >> (8 * k)) & 0xFF) == 1) {
-      flag_dead[i] = 0;
-      tally_dead(a, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
-      continue;
-    }
-    int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
+    const bool live = k < nv && !((deadm >> k) & 1u);
+    float fx = live ? FX[k] : 0.5f, fy = live ? FY[k] : 0.5f;
+    float vx = live ? VX[k] : 0.f, vy = live ? VY[k] : 0.f, vz = live ? VZ[k] : 0.f, zp = live ? Z[k] : 0.f;
+    int ix = live ? (C[k] & 0xFFFF) : 2, iy = live ? (C[k] >> 16) : 2;
     clamp_cell(ix, iy, n1, n2);
     const float px = 1.0f - fx, py = 1.0f - fy;
     const float w0 = px * py, w1 = fx * py, w2 = fx * fy, w3 = px * fy;
@@ -743,14 +754,23 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     iy += (int)sy;
     int cx = ix, cy = iy;
     clamp_cell(cx, cy, n1, n2);
-    if (a.cellwall[node_index(cx, cy, n1)]) {  // part_expmover.f90:254-258
-      double xd = ((double)(ix - 1) + fx) * a.h1, yd = ((double)(iy - 1) + fy) * a.h2;
-      tally_lost(a, xd, yd, cx, cy, 0.5 * a.Nm * a.mass * ((double)vx * vx + (double)vy * vy + (double)vz * vz));
-      continue;
-    }
+    const bool lost = live && a.cellwall[node_index(cx, cy, n1)] != 0;  // part_expmover.f90:254-258
+    lostm |= lost ? 1u << k : 0u;
+    IXU[k] = ix; IYU[k] = iy;
     C[k] = (iy << 16) | ix;
     FX[k] = fx; FY[k] = fy; Z[k] = zp; VX[k] = vx; VY[k] = vy; VZ[k] = vz;
-    surv[k] = true;
+    surv[k] = live && !lost;
+  }
+  if (lostm) {
+#pragma unroll
+    for (int k = 0; k < kItems; ++k)
+      if ((lostm >> k) & 1u) {
+        int cx = IXU[k], cy = IYU[k];
+        clamp_cell(cx, cy, n1, n2);
+        const double xd = ((double)(IXU[k] - 1) + FX[k]) * a.h1, yd = ((double)(IYU[k] - 1) + FY[k]) * a.h2;
+        tally_lost(a, xd, yd, cx, cy,
+                   0.5 * a.Nm * a.mass * ((double)VX[k] * VX[k] + (double)VY[k] * VY[k] + (double)VZ[k] * VZ[k]));
+      }
   }
   // the deposition's window-local keys (cells leaving the window go to the
   // global density here) and the wave's number of runs of equal keys
