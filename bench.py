@@ -195,6 +195,106 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
     return acc, wall, counts
 
 
+def _rel(a, b):
+    """max |a - b| over max |b| (b the FP64 result)."""
+    s = float(np.max(np.abs(b)))
+    return float(np.max(np.abs(a - b)) / s) if s > 0 else 0.0
+
+
+def accuracy_and_sor(W, steps=10, sor_solves=3):
+    """Untimed checks beside the step loop.
+
+    Mixed against FP64 from the same initial particles (the device loader
+    with the same seed): max-norm differences, relative to the FP64 field's
+    largest magnitude, of each species' density after the first deposition,
+    and of the densities, phi and E after `steps` steps, and of the density
+    and temperature maps (part_moments/calc_avg) averaged over those steps on
+    nodes above a tenth of the peak density.  The loader puts every species
+    at the same positions, so the first right-hand side is zero up to
+    rounding: its difference is given relative to one species' charge
+    density instead.
+
+    Then the Poisson problems of the last `sor_solves` steps (the FP64 run's
+    right-hand sides) solved by red-black SOR alone (ng = 1, FP64; every call
+    iterates to eps with the 10^6-sweep limit of sors.f90:63-64) beside the
+    9-level MG in FP64 and in mixed precision: ms per solve and the phi
+    difference."""
+    n = W["n"]
+    ctx = {p: build_ctx(W, p) for p in (L.FP64, L.MIXED)}
+    out, rhs_seq = {}, []
+    for c in ctx.values():
+        c.moments_reset()
+    for it in range(1, steps + 1):
+        for p, c in ctx.items():
+            c.calc_rhs()
+            if p == L.FP64:
+                rhs_seq.append(c.get_grid(L.GRID_RHS))
+            c.iterative_solver(eps=1e-10, ncycl=20000, start=L.START_COLD if it == 1 else L.START_PREV)
+            c.calc_efield()
+        if it == 1:
+            g = {p: (c.get_grid(L.GRID_RHS), c.get_grid(L.GRID_NP, 1), c.get_grid(L.GRID_NP, 2))
+                 for p, c in ctx.items()}
+            qscale = L.QE * float(np.max(np.abs(g[L.FP64][1])))
+            out["first_deposit"] = {
+                "np_e": _rel(g[L.MIXED][1], g[L.FP64][1]), "np_i": _rel(g[L.MIXED][2], g[L.FP64][2]),
+                "rhs_over_species_charge": float(np.max(np.abs(g[L.MIXED][0] - g[L.FP64][0]))) / qscale}
+        for p, c in ctx.items():
+            if it % 10 == 1:
+                for s in (1, 2):
+                    c.part_sorting(s)
+            for s in (1, 2):
+                c.part_mover(s, deposit=True)
+            c.moments_accumulate(1)
+    last = {}
+    for p, c in ctx.items():
+        last[p] = dict(np_e=c.get_grid(L.GRID_NP, 1), np_i=c.get_grid(L.GRID_NP, 2), mom=c.moments()[0])
+        c.calc_rhs()
+        c.iterative_solver(eps=1e-10, ncycl=20000, start=L.START_PREV)
+        c.calc_efield()
+        last[p].update(phi=c.get_grid(L.GRID_PHI), E=c.get_grid(L.GRID_E))
+    out["after_steps"] = {k: _rel(last[L.MIXED][k], last[L.FP64][k]) for k in ("np_e", "np_i", "phi", "E")}
+    out["after_steps"]["phi_max_abs_V"] = float(np.max(np.abs(last[L.FP64]["phi"])))
+    mom = {}
+    for s, name in ((0, "e"), (1, "i")):
+        d64, dmx = last[L.FP64]["mom"][s], last[L.MIXED]["mom"][s]
+        m = d64[..., 0] > 0.1 * d64[..., 0].max()
+        for j, q in ((0, "density"), (1, "temperature")):
+            mj = m & (d64[..., j] != 0)
+            mom[f"{q}_{name}"] = float(np.max(np.abs(dmx[..., j][mj] - d64[..., j][mj]) / np.abs(d64[..., j][mj])))
+    out["moments_max_rel_on_dense_nodes"] = mom
+    out["steps"] = steps
+    for c in ctx.values():
+        c.close()
+
+    def solve_seq(ng, prec, k):
+        c = L.Context(n, n, W["h"], W["h"], W["dt"], [L.Species(-L.QE, L.ME, 1.0, 16)], ng=ng, precision=prec)
+        c.set_boundary(W["bcnd"], np.zeros((n + 3, n + 3)))
+        ms, sw, vc = [], [], []
+        for j, b in enumerate(rhs_seq[-k:]):
+            c.set_grid(L.GRID_RHS, b)
+            c.sync()
+            t = time.perf_counter()
+            info = c.iterative_solver(eps=1e-10, ncycl=20000, start=L.START_COLD if j == 0 else L.START_PREV)
+            c.sync()
+            ms.append(1e3 * (time.perf_counter() - t))
+            sw.append(int(np.sum(info["sweeps"])))
+            vc.append(int(info["vcycles"]))
+        phi = c.get_grid(L.GRID_PHI)
+        c.close()
+        return ms, sw, vc, phi
+
+    k = min(sor_solves, len(rhs_seq))
+    ms1, sw1, _, phi1 = solve_seq(1, L.FP64, k)
+    ms9, _, vc9, phi9 = solve_seq(W["ng"], L.FP64, k)
+    msm, _, vcm, phim = solve_seq(W["ng"], L.MIXED, k)
+    sor = {"solves": k, "sor_ng1_fp64_ms": ms1, "sor_ng1_sweeps": sw1,
+           "mg_fp64_ms": ms9, "mg_fp64_vcycles": vc9, "mg_mixed_ms": msm, "mg_mixed_vcycles": vcm,
+           "phi_sor_vs_mg_fp64": _rel(phi1, phi9), "phi_mg_mixed_vs_fp64": _rel(phim, phi9),
+           "note": "first solve cold, later ones warm-started from the previous phi; "
+                   "right-hand sides of the last steps of the FP64 run"}
+    return out, sor
+
+
 def cpu_baseline(W, budget_s=10.0):
     """The oracle (scalar C restatement, one core) on a bounded sample of the
     same workload: 64 x 64 cells of it at 100 ppc for both species."""
@@ -315,6 +415,8 @@ def main():
     ap.add_argument("--train-iters", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--skip-accuracy", action="store_true",
+                    help="skip the untimed mixed-vs-FP64 comparison and the ng=1 SOR Poisson run")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="after the runs, one device-to-device copy of GIB GiB (a plain copy kernel "
                          "whose HBM counters calibrate those of the mover under rocprofv3 --pmc)")
@@ -401,6 +503,9 @@ def main():
         out["fp64"]["roofline_push_GBs"] = cnt_f[0] * 96.0 / (kf.mean() * 1e-3) / 1e9
         out["speedup_vs_fp64_step"] = s_f["ms_per_step"] / s_h["ms_per_step"]
         out["speedup_vs_fp64_pdm"] = s_h["particles_per_s"] / s_f["particles_per_s"]
+    if not args.skip_accuracy:
+        print("[bench] mixed against FP64 (untimed), ng=1 SOR beside MG", file=sys.stderr, flush=True)
+        out["accuracy_mixed_vs_fp64"], out["poisson_sor_vs_mg"] = accuracy_and_sor(W)
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = hip.copy_once(int(args.copy_calibration * (1 << 30)))
     print(json.dumps(out))
