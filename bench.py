@@ -284,6 +284,8 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
         return ms, sw, vc, phi
 
     k = min(sor_solves, len(rhs_seq))
+    if k == 0:
+        return out, None
     ms1, sw1, _, phi1 = solve_seq(1, L.FP64, k)
     ms9, _, vc9, phi9 = solve_seq(W["ng"], L.FP64, k)
     msm, _, vcm, phim = solve_seq(W["ng"], L.MIXED, k)
@@ -408,6 +410,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--ppc", type=int, default=100, help="macroparticles per cell of each species")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-plain-mixed", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
@@ -417,6 +420,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--skip-accuracy", action="store_true",
                     help="skip the untimed mixed-vs-FP64 comparison and the ng=1 SOR Poisson run")
+    ap.add_argument("--sor-solves", type=int, default=3,
+                    help="Poisson solves of the ng=1 SOR run beside MG (0: none)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="after the runs, one device-to-device copy of GIB GiB (a plain copy kernel "
                          "whose HBM counters calibrate those of the mover under rocprofv3 --pmc)")
@@ -426,7 +431,7 @@ def main():
               "split is a design only)", file=sys.stderr)
         sys.exit(2)
     hip = Hip()
-    W = workload(args.n)
+    W = workload(args.n, args.ppc)
     N = 2 * args.n * args.n * W["ppc"]
     K = args.steps
 
@@ -481,7 +486,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.n}x{args.n} cells, e-/Ar+ 100 ppc each ({N} particles), "
+        "config": {"workload": f"{args.n}x{args.n} cells, e-/Ar+ {args.ppc} ppc each ({N} particles), "
                                "B filter 10 mT, Ar 0.3 Pa analytic MCC, dt 5e-12, MG 1e-10, "
                                + ("mixed precision + learned warm start" if head == "mixed_learned"
                                   else "mixed precision"),
@@ -505,7 +510,7 @@ def main():
         out["speedup_vs_fp64_pdm"] = s_h["particles_per_s"] / s_f["particles_per_s"]
     if not args.skip_accuracy:
         print("[bench] mixed against FP64 (untimed), ng=1 SOR beside MG", file=sys.stderr, flush=True)
-        out["accuracy_mixed_vs_fp64"], out["poisson_sor_vs_mg"] = accuracy_and_sor(W)
+        out["accuracy_mixed_vs_fp64"], out["poisson_sor_vs_mg"] = accuracy_and_sor(W, sor_solves=args.sor_solves)
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = hip.copy_once(int(args.copy_calibration * (1 << 30)))
     print(json.dumps(out))

@@ -185,7 +185,7 @@ struct lepic_ctx {
   int64_t s_cap = 0, s_hist_cap = 0;
   int32_t* d_plist = nullptr;        // cell-end list (pl_max+1)
   uint32_t* d_scan_tmp = nullptr;
-  int64_t scan_tmp_cap = 0;
+  int64_t plist_cap = 0;           // entries of d_plist
 
   // collisions
   lepic::Reactions rx;

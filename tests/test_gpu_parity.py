@@ -194,6 +194,25 @@ def test_sort_matches_oracle(precision):
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+@pytest.mark.parametrize("n1,n2", [(32, 30), (2048, 1024), (2048, 2048)])
+def test_sort_digit_passes(n1, n2, precision):
+    """The radix sort splits the key (cell index, plus the empty-slot key
+    above it) into the fewest digits of at most 11 bits: one pass for 960
+    cells, two of 11 bits for 2^21 cells, three (8, 8, 7 bits) for the
+    2^22 cells of 2048 x 2048."""
+    print(f"sort passes {n1}x{n2} precision={precision}", flush=True)
+    dom = S.box_domain(n1, n2, Bmode=None)
+    parts = S.particles(dom, 60000, 1.0, seed=37, edge_frac=0.0)
+    with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 60000)], precision=precision) as ctx:
+        ctx.upload(1, parts)
+        before = ctx.download(1)
+        pl = ctx.part_sorting(1, want_plist=True)
+        got = ctx.download(1)
+    ref, pref = O.part_sorting(n1, n2, (dom.h1, dom.h2), before)
+    assert np.array_equal(got, ref) and np.array_equal(pl, pref)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
 def test_sort_of_unpacked_segments(precision):
     """The mover leaves partly filled segments; the sort reads them as they
     are and must give what sorting the packed species gives (and, in FP64,
