@@ -33,7 +33,7 @@ constexpr int kWinY0 = 7;
 constexpr int kWin = kWinRows * kWinCols;
 // threads per mover block; a block owns one segment of kChunk slots
 constexpr int kMoverThreadsMixed = kBlock;
-constexpr int kMoverThreads64 = kBlock;
+constexpr int kMoverThreads64 = 2 * kBlock;   // two particles per thread: FP64 state of 4 would cost occupancy
 
 struct MoverCommon {
   const int8_t* bc;
@@ -149,21 +149,21 @@ __device__ inline void tally_dead(const MoverCommon& a, double Eki) {
 // Order-preserving offsets of the survivors inside the block's segment;
 // records the segment's survivor count and takes the lost ones off the
 // species count.  Returns each item's offset in off[].
-template <int BT>
-__device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cnt_in, const bool surv[kItems],
-                                       uint32_t off[kItems]) {
-  __shared__ uint32_t s_cnt[kItems][BT / 64];
+template <int BT, int IT = kChunk / BT>
+__device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cnt_in, const bool surv[IT],
+                                       uint32_t off[IT]) {
+  __shared__ uint32_t s_cnt[IT][BT / 64];
   const int w = threadIdx.x >> 6;
-  uint64_t mask[kItems];
+  uint64_t mask[IT];
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < IT; ++k) {
     mask[k] = __ballot(surv[k]);
     if ((threadIdx.x & 63) == 0) s_cnt[k][w] = (uint32_t)__popcll(mask[k]);
   }
   __syncthreads();
   uint32_t run = 0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < IT; ++k) {
     uint32_t before = run;
     for (int v = 0; v < BT / 64; ++v) {
       if (v < w) before += s_cnt[k][v];
@@ -246,6 +246,33 @@ __device__ __forceinline__ void seg_sum4(float w[4], int s) {
     o = dpp_f32<0x118, 0xf>(v); if (a8) v += o;
     o = dpp_f32<0x142, 0xa>(v); if (b15) v += o;
     o = dpp_f32<0x143, 0xc>(v); if (b31) v += o;
+    w[c] = v;
+  }
+}
+
+// The same for FP64 weights, each moved as two 32-bit DPP halves.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)bits, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ __forceinline__ void seg_sum4(double w[4], int s) {
+  const int lane = threadIdx.x & 63;
+  const bool a1 = lane - 1 >= s, a2 = lane - 2 >= s, a4 = lane - 4 >= s, a8 = lane - 8 >= s;
+  const bool b15 = (lane & 16) && (lane & ~15) - 1 >= s;
+  const bool b31 = lane >= 32 && s <= 31;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    double v = w[c], o;
+    o = dpp_f64<0x111, 0xf>(v); if (a1) v += o;
+    o = dpp_f64<0x112, 0xf>(v); if (a2) v += o;
+    o = dpp_f64<0x114, 0xf>(v); if (a4) v += o;
+    o = dpp_f64<0x118, 0xf>(v); if (a8) v += o;
+    o = dpp_f64<0x142, 0xa>(v); if (b15) v += o;
+    o = dpp_f64<0x143, 0xc>(v); if (b31) v += o;
     w[c] = v;
   }
 }
@@ -468,11 +495,12 @@ template <int BT>
 __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W out,
                                                     int8_t* flag_dead, const double* __restrict__ E,
                                                     const double* __restrict__ B) {
+  constexpr int IT = kChunk / BT;   // particles per thread
   __shared__ double win[kWin];
   __shared__ int s_wx0, s_wy0;
   const int64_t bid = blockIdx.x;
   const int cnt_in = a.seg_in[bid];
-  const int64_t base = bid * (int64_t)(BT * kItems);
+  const int64_t base = bid * (int64_t)(BT * IT);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const double h1 = a.h1, h2 = a.h2, dt = a.dt, k1 = a.k1;
@@ -487,10 +515,10 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     }
   }
 
-  double X[kItems], Y[kItems], Z[kItems], VX[kItems], VY[kItems], VZ[kItems];
-  bool surv[kItems];
+  double X[IT], Y[IT], Z[IT], VX[IT], VY[IT], VZ[IT];
+  bool surv[IT];
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < IT; ++k) {
     const int64_t i = base + (int64_t)k * BT + t;
     surv[k] = false;
     if (k * BT + t >= cnt_in) continue;
@@ -575,11 +603,11 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     surv[k] = true;
   }
 
-  uint32_t off[kItems];
+  uint32_t off[IT];
   compact_offsets<BT>(a, bid, cnt_in, surv, off);  // contains __syncthreads
   const int64_t gbase = base;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < IT; ++k) {
     if (!surv[k]) continue;
     const int64_t o = gbase + off[k];
     out.x[o] = X[k]; out.y[o] = Y[k]; out.z[o] = Z[k];
@@ -588,7 +616,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
   if (!a.deposit) return;
   const int wx0 = s_wx0, wy0 = s_wy0;
 #pragma unroll
-  for (int k = 0; k < kItems; ++k) {
+  for (int k = 0; k < IT; ++k) {
     // charge_deposition, part_expmover.f90:555-584, at the new position
     int64_t key = -2 - (int64_t)t;
     double w[4] = {0, 0, 0, 0};
@@ -604,8 +632,17 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
       w[3] = a.k1dep * px * (1.0 - py);
       key = node_index(ix, iy, n1);
     }
-    bool head = run_reduce(key, w);
-    if (head && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+    // runs of equal node in adjacent lanes: segmented DPP sums, the run's
+    // last lane deposits (waves whose particles all differ skip the sums)
+    const int lane = t & 63;
+    const int kk = surv[k] ? (int)key : -2 - lane;
+    const int kp = __shfl_up(kk, 1, 64);
+    const uint64_t heads = __ballot(lane == 0 || kp != kk);
+    const uint64_t act = __ballot(surv[k]);
+    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+    if (__popcll(heads & act) != __popcll(act)) seg_sum4(w, s);
+    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+    if (tail && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
   }
   __syncthreads();
   flush_window(a, win, wx0, wy0, 1.0);
