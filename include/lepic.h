@@ -160,7 +160,8 @@ int lepic_calc_efield(lepic_ctx *ctx);
 int lepic_iterative_solver(lepic_ctx *ctx, double eps, int32_t ncycl, int32_t start,
                            lepic_solve_info *info, int32_t *sweep_log, int32_t log_cap);
 /* part_sorting (sorting.f90:92-251): stable counting sort by cell; Plist
- * (optional, n1+n1*n2+2 int32) receives the cell-end list. */
+ * (optional, n1+n1*n2+2 int32) receives the cell-end list, which is only
+ * computed when Plist is not NULL (the device MCC does not need it). */
 int lepic_part_sorting(lepic_ctx *ctx, int32_t ptype, int32_t *Plist);
 /* dens_red (calc_rho.f90:134-199): np_red and np_mx per charged species */
 int lepic_dens_red(lepic_ctx *ctx, double *np_mx);

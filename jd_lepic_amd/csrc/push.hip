@@ -1,18 +1,20 @@
 // push.hip -- particle mover (part_expmover.f90:1-517) fused with charge
 // deposition (part_expmover.f90:520-610), FP64 and mixed precision.
 //
-// One block handles one segment of kChunk = 4 x 256 slots (thread t, item k
-// -> slot base + k*256 + t, so loads are coalesced).  Survivors are compacted
-// in particle order (part_expmover.f90:495-503) by a block ballot scan into
-// the same segment of the other particle buffer; the segments' contents in
-// segment order are the species' particles, so no prefix across blocks is
-// needed and blocks never wait for each other.  Deposition reduces equal-cell runs inside each wave
-// (particles are cell-sorted, so neighbours share a cell), accumulates the
-// runs in an LDS window of 4 rows x 128 columns of nodes around the block's
-// first cell, and flushes the window to the global density with FP64
-// atomics; particles outside the window go straight to global atomics.
-// The FP64 path computes exactly the reference's expressions (no
-// contraction: the library is built with -ffp-contract=off).
+// One block handles one segment of kChunk = 1024 slots.  Mixed precision:
+// 256 threads, thread t owning the four consecutive slots 4t .. 4t+3 (16-byte
+// vector loads and stores); FP64: 512 threads, thread t owning slots t and
+// t + 512.  Survivors are compacted in particle order (part_expmover.f90:
+// 495-503) by a block ballot scan into the same segment of the other particle
+// buffer; the segments' contents in segment order are the species'
+// particles, so no prefix across blocks is needed and blocks never wait for
+// each other.  Deposition merges equal cells within a thread and in runs of
+// adjacent lanes (segmented DPP sums), accumulates in an LDS window of 16 rows
+// x 64 columns of nodes around the segment's first cell, and flushes the
+// window to the global density with FP64 atomics; nodes outside the window go
+// to the global density directly.  The FP64 path computes exactly the
+// reference's expressions (no contraction: the library is built with
+// -ffp-contract=off).
 #include <hip/hip_runtime.h>
 
 #include "lepic_internal.h"
@@ -67,24 +69,6 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // ---- deposition helpers ----------------------------------------------------
-// Wave-level segmented sum of w[0..3] over runs of equal key (adjacent lanes).
-// Returns true on the run head lane, which then holds the run sums.
-template <typename T>
-__device__ __forceinline__ bool run_reduce(int64_t key, T w[4]) {
-  const int lane = threadIdx.x & 63;
-  int64_t nxt = __shfl_down(key, 1, 64);
-  int64_t prv = __shfl_up(key, 1, 64);
-  uint64_t tails = __ballot(lane == 63 || nxt != key);
-  int seg_end = lane + __builtin_ctzll(tails >> lane);
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    T o0 = __shfl_down(w[0], off, 64), o1 = __shfl_down(w[1], off, 64);
-    T o2 = __shfl_down(w[2], off, 64), o3 = __shfl_down(w[3], off, 64);
-    if (lane + off <= seg_end) { w[0] += o0; w[1] += o1; w[2] += o2; w[3] += o3; }
-  }
-  return lane == 0 || prv != key;
-}
-
 // Deposit the four corner sums of cell (ix,iy) (corners 00,10,11,01 in the
 // order of part_expmover.f90:571-584).  kq = 2 on non-interior nodes
 // (main.f90:334-344) is applied here or at the flush (a factor 2 is exact).
@@ -275,6 +259,24 @@ __device__ __forceinline__ void seg_sum4(double w[4], int s) {
     o = dpp_f64<0x143, 0xc>(v); if (b31) v += o;
     w[c] = v;
   }
+}
+
+// FP64 deposition of one item slot: runs of equal node in adjacent lanes are
+// summed by the segmented DPP scan and the run's last lane deposits them
+// (waves whose particles all sit in different cells skip the scan).  Every
+// lane of the wave must call it; ok marks a lane with a particle, key its
+// node index.
+__device__ __forceinline__ void deposit_runs64(const MoverCommon& a, double* win, int wx0, int wy0, bool ok,
+                                               int key, int ix, int iy, double w[4]) {
+  const int lane = threadIdx.x & 63;
+  const int kk = ok ? key : -2 - lane;
+  const int kp = __shfl_up(kk, 1, 64);
+  const uint64_t heads = __ballot(lane == 0 || kp != kk);
+  const uint64_t act = __ballot(ok);
+  const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+  if (__popcll(heads & act) != __popcll(act)) seg_sum4(w, s);
+  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+  if (tail && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
 }
 
 // Weights of one cell straight to the global density (nodes outside the LDS
@@ -632,17 +634,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
       w[3] = a.k1dep * px * (1.0 - py);
       key = node_index(ix, iy, n1);
     }
-    // runs of equal node in adjacent lanes: segmented DPP sums, the run's
-    // last lane deposits (waves whose particles all differ skip the sums)
-    const int lane = t & 63;
-    const int kk = surv[k] ? (int)key : -2 - lane;
-    const int kp = __shfl_up(kk, 1, 64);
-    const uint64_t heads = __ballot(lane == 0 || kp != kk);
-    const uint64_t act = __ballot(surv[k]);
-    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
-    if (__popcll(heads & act) != __popcll(act)) seg_sum4(w, s);
-    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-    if (tail && surv[k]) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+    deposit_runs64(a, win, wx0, wy0, surv[k], (int)key, ix, iy, w);
   }
   __syncthreads();
   flush_window(a, win, wx0, wy0, 1.0);
@@ -907,8 +899,7 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
       w[3] = a.k1dep * px * (1.0 - py);
       key = node_index(ix, iy, a.n1);
     }
-    bool head = run_reduce(key, w);
-    if (head && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+    deposit_runs64(a, win, wx0, wy0, ok, (int)key, ix, iy, w);
   }
   __syncthreads();
   flush_window(a, win, wx0, wy0, 1.0);
