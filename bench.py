@@ -201,6 +201,14 @@ def _rel(a, b):
     return float(np.max(np.abs(a - b)) / s) if s > 0 else 0.0
 
 
+def _stocks(c):
+    """Kinetic energy, field energy and charge of both species now."""
+    t = [c.species_totals(q) for q in (1, 2)]
+    return dict(kinetic=t[0]["kinetic"] + t[1]["kinetic"], field=c.field_energy(),
+                charge=t[0]["charge"] + t[1]["charge"], charge_e=abs(t[0]["charge"]),
+                count=[t[0]["count"], t[1]["count"]])
+
+
 def accuracy_and_sor(W, steps=10, sor_solves=3):
     """Untimed checks beside the step loop.
 
@@ -209,7 +217,10 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
     largest magnitude, of each species' density after the first deposition,
     and of the densities, phi and E after `steps` steps, and of the density
     and temperature maps (part_moments/calc_avg) averaged over those steps on
-    nodes above a tenth of the peak density.  The loader puts every species
+    nodes above a tenth of the peak density; and per precision the energy
+    budget over those steps (kinetic + field energy at the start and end, the
+    kinetic energy the walls took from the movers' tallies, and the relative
+    drift of their sum) and the total charge.  The loader puts every species
     at the same positions, so the first right-hand side is zero up to
     rounding: its difference is given relative to one species' charge
     density instead.
@@ -222,8 +233,10 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
     n = W["n"]
     ctx = {p: build_ctx(W, p) for p in (L.FP64, L.MIXED)}
     out, rhs_seq = {}, []
+    stock0 = {}
     for c in ctx.values():
         c.moments_reset()
+        c.reset_tallies()
     for it in range(1, steps + 1):
         for p, c in ctx.items():
             c.calc_rhs()
@@ -232,6 +245,7 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
             c.iterative_solver(eps=1e-10, ncycl=20000, start=L.START_COLD if it == 1 else L.START_PREV)
             c.calc_efield()
         if it == 1:
+            stock0 = {p: _stocks(c) for p, c in ctx.items()}
             g = {p: (c.get_grid(L.GRID_RHS), c.get_grid(L.GRID_NP, 1), c.get_grid(L.GRID_NP, 2))
                  for p, c in ctx.items()}
             qscale = L.QE * float(np.max(np.abs(g[L.FP64][1])))
@@ -245,13 +259,24 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
             for s in (1, 2):
                 c.part_mover(s, deposit=True)
             c.moments_accumulate(1)
-    last = {}
+    last, budget = {}, {}
     for p, c in ctx.items():
         last[p] = dict(np_e=c.get_grid(L.GRID_NP, 1), np_i=c.get_grid(L.GRID_NP, 2), mom=c.moments()[0])
         c.calc_rhs()
         c.iterative_solver(eps=1e-10, ncycl=20000, start=L.START_PREV)
         c.calc_efield()
         last[p].update(phi=c.get_grid(L.GRID_PHI), E=c.get_grid(L.GRID_E))
+        s0, s1 = stock0[p], _stocks(c)
+        wall = sum(c.mover_tally(q)["P_loss"][0] for q in (1, 2))
+        e0 = s0["kinetic"] + s0["field"]
+        budget["fp64" if p == L.FP64 else "mixed"] = {
+            "kinetic_J": [s0["kinetic"], s1["kinetic"]], "field_J": [s0["field"], s1["field"]],
+            "wall_loss_J": wall,
+            "energy_drift_rel": (s1["kinetic"] + s1["field"] + wall - e0) / e0,
+            "charge_C": [s0["charge"], s1["charge"]],
+            "charge_change_over_species_charge": (s1["charge"] - s0["charge"]) / s0["charge_e"],
+            "lost": [a - b for a, b in zip(s0["count"], s1["count"])]}
+    out["energy_charge_budget"] = budget
     out["after_steps"] = {k: _rel(last[L.MIXED][k], last[L.FP64][k]) for k in ("np_e", "np_i", "phi", "E")}
     out["after_steps"]["phi_max_abs_V"] = float(np.max(np.abs(last[L.FP64]["phi"])))
     mom = {}

@@ -365,6 +365,16 @@ int lepic_moments_accumulate(lepic_ctx *ctx, int32_t size_na);
 int lepic_moments_get(lepic_ctx *ctx, double *data_pavg, double *phi_avg, int32_t *count);
 int lepic_moments_reset(lepic_ctx *ctx);
 
+/* ---- charge and energy totals (no counterpart routine in the reference,
+ * whose P_loss tallies account for the energy flows; these are the stocks
+ * the flows change, for drift checks) ----
+ * lepic_species_totals: out[0] = particle count, out[1] = kinetic energy
+ * 0.5*Nm*mass*sum|v|^2 (J), out[2] = charge Nm*charge*count (C).
+ * lepic_field_energy: 0.5*eps0*sum|E|^2*h1*h2*zmax over the interior nodes
+ * (bcnd == -1) of the current E grid (J).  Sums in a fixed order. */
+int lepic_species_totals(lepic_ctx *ctx, int32_t ptype, double *out);
+int lepic_field_energy(lepic_ctx *ctx, double *W);
+
 /* ---- particle backup files (main.f90:1102-1132 writer, restart.f90:56-113 reader) ----
  * The reference's particlesNNN.bak layout: Fortran unformatted sequential
  * records (4-byte length markers, subrecords with signed markers above

@@ -158,6 +158,19 @@ class Context:
     def moments_reset(self):
         self._check(self.lib.lepic_moments_reset(self._ctx))
 
+    # ---- charge and energy totals (stocks for drift checks) ----
+    def species_totals(self, ptype):
+        """dict(count, kinetic (J), charge (C)) of one species."""
+        out = np.zeros(3)
+        self._check(self.lib.lepic_species_totals(self._ctx, ptype, _dp(out)))
+        return dict(count=int(out[0]), kinetic=float(out[1]), charge=float(out[2]))
+
+    def field_energy(self):
+        """0.5 eps0 sum |E|^2 h1 h2 zmax over the interior nodes (J)."""
+        w = C.c_double()
+        self._check(self.lib.lepic_field_energy(self._ctx, C.byref(w)))
+        return w.value
+
     # ---- backup files in the reference's particlesNNN.bak format ----
     def save_particles(self, path, time=0.0, gtype=0, pabs_cor=1.0):
         self._check(self.lib.lepic_save_particles(self._ctx, str(path).encode(), time, gtype, pabs_cor))

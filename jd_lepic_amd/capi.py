@@ -121,6 +121,8 @@ SIGNATURES = {
     "lepic_moments_accumulate": (C.c_int, [_ctxp, C.c_int32]),
     "lepic_moments_get": (C.c_int, [_ctxp, _dp, _dp, _ip]),
     "lepic_moments_reset": (C.c_int, [_ctxp]),
+    "lepic_species_totals": (C.c_int, [_ctxp, C.c_int32, _dp]),
+    "lepic_field_energy": (C.c_int, [_ctxp, _dp]),
     "lepic_save_particles": (C.c_int, [_ctxp, C.c_char_p, C.c_double, C.c_int32, C.c_double]),
     "lepic_load_particles": (C.c_int, [_ctxp, C.c_char_p, _dp, _ip, _dp]),
     "lepic_net_train": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_float, C.POINTER(C.c_float),

@@ -406,7 +406,8 @@ int lepic_destroy(lepic_ctx* ctx) {
                   ctx->d_plist, ctx->d_scan_tmp, ctx->rx.d_sig_Er, ctx->rx.d_sig,
                   ctx->rx.d_sig_Er_f, ctx->rx.d_sig_f, ctx->rx.d_cnt_col, ctx->d_claim,
                   ctx->d_mcc_stage, ctx->d_mcc_i64, ctx->d_mcc_f64, ctx->d_mcc_tab, ctx->d_heat,
-                  ctx->d_stage6, ctx->d_stype, ctx->d_srank, ctx->d_srctally, ctx->d_cellwall};
+                  ctx->d_stage6, ctx->d_stype, ctx->d_srank, ctx->d_srctally, ctx->d_cellwall,
+                  ctx->d_diag};
   for (void* p : ptrs) if (p) (void)hipFree(p);
   for (int l = 2; l <= kMaxLevels; ++l) {
     if (ctx->lv[l].u) (void)hipFree(ctx->lv[l].u);

@@ -186,6 +186,7 @@ struct lepic_ctx {
   int32_t* d_plist = nullptr;        // cell-end list (pl_max+1)
   uint32_t* d_scan_tmp = nullptr;
   int64_t plist_cap = 0;           // entries of d_plist
+  double* d_diag = nullptr;          // per-block partials of the charge/energy totals
 
   // collisions
   lepic::Reactions rx;

@@ -14,6 +14,7 @@
 // kernels are HBM/atomic bound without further tuning.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "lepic_internal.h"
@@ -143,6 +144,67 @@ __global__ void k_phi_avg(const double* __restrict__ phi, const double* __restri
 
 unsigned nblk(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+
+// ---- charge and energy totals ----------------------------------------------
+constexpr int kTotBlocks = 1024;   // per-block partials, summed on the host in block order
+
+// Block sum in a fixed order: waves by xor-shuffles, then the four wave sums
+// in wave order.
+__device__ __forceinline__ double block_sum256(double v, double* s_w) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+// sum |v|^2 over the particles of one species (either layout); blocks stride
+// over the segments, threads over a segment's slots
+__global__ __launch_bounds__(256) void k_species_v2(const int32_t* __restrict__ seg, int64_t nseg,
+                                                    const double* __restrict__ vx, const double* __restrict__ vy,
+                                                    const double* __restrict__ vz, const float* __restrict__ fvx,
+                                                    const float* __restrict__ fvy, const float* __restrict__ fvz,
+                                                    double* __restrict__ partial) {
+  __shared__ double s_w[4];
+  double s = 0.0;
+  for (int64_t g = blockIdx.x; g < nseg; g += gridDim.x) {
+    const int cnt = seg[g];
+    for (int j = threadIdx.x; j < cnt; j += 256) {
+      const int64_t i = g * kChunk + j;
+      double a, b, c;
+      if (vx) { a = vx[i]; b = vy[i]; c = vz[i]; }
+      else { a = fvx[i]; b = fvy[i]; c = fvz[i]; }
+      s += a * a + b * b + c * c;
+    }
+  }
+  const double t = block_sum256(s, s_w);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// sum |E|^2 over the interior nodes (bcnd == -1)
+__global__ __launch_bounds__(256) void k_field_e2(const double* __restrict__ E, const int8_t* __restrict__ bc,
+                                                  int64_t nodes, double* __restrict__ partial) {
+  __shared__ double s_w[4];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nodes; i += (int64_t)gridDim.x * 256)
+    if (bc[i] == -1) s += E[2 * i] * E[2 * i] + E[2 * i + 1] * E[2 * i + 1];
+  const double t = block_sum256(s, s_w);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// launch one of the sums above with nb blocks and add the partials in order
+template <typename Launch>
+int sum_partials(lepic_ctx* ctx, int nb, Launch launch, double* out) {
+  static_assert(kTotBlocks <= kPinnedDoubles, "partials fit the pinned scratch");
+  if (!ctx->d_diag) LEPIC_HIP(hipMalloc(&ctx->d_diag, kTotBlocks * 8));
+  launch(ctx->d_diag);
+  LEPIC_HIP(hipGetLastError());
+  LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_diag, (size_t)nb * 8, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  double t = 0.0;
+  for (int b = 0; b < nb; ++b) t += ctx->h_pinned[b];
+  *out = t;
+  return 0;
+}
 }  // namespace
 
 extern "C" {
@@ -237,4 +299,45 @@ int lepic_moments_get(lepic_ctx* ctx, double* data_pavg, double* phi_avg, int32_
   return 0;
 }
 
+int lepic_species_totals(lepic_ctx* ctx, int32_t ptype, double* out) {
+  if (!ctx || !out || ptype < 1 || ptype > ctx->nsp) return LEPIC_ERR_ARG;
+  int64_t n = 0;
+  if (int rc = lepic_get_count(ctx, ptype, &n)) return rc;
+  Species& s = ctx->sp[ptype - 1];
+  const int64_t nseg = nseg_of(s);
+  double v2 = 0.0;
+  if (nseg > 0) {
+    const int nb = (int)std::min<int64_t>(nseg, kTotBlocks);
+    const int b = s.cur;
+    const bool mixed = ctx->g.precision == LEPIC_MIXED;
+    if (int rc = sum_partials(ctx, nb, [&](double* part) {
+          hipLaunchKernelGGL(k_species_v2, dim3(nb), dim3(256), 0, ctx->stream, (const int32_t*)s.seg[b], nseg,
+                             mixed ? nullptr : (const double*)s.d[b][3], mixed ? nullptr : (const double*)s.d[b][4],
+                             mixed ? nullptr : (const double*)s.d[b][5], mixed ? (const float*)s.f[b][3] : nullptr,
+                             mixed ? (const float*)s.f[b][4] : nullptr, mixed ? (const float*)s.f[b][5] : nullptr,
+                             part);
+        }, &v2))
+      return rc;
+  }
+  out[0] = (double)n;
+  out[1] = 0.5 * s.Nm * s.mass * v2;
+  out[2] = s.Nm * s.charge * (double)n;
+  return 0;
+}
+
+int lepic_field_energy(lepic_ctx* ctx, double* W) {
+  if (!ctx || !W) return LEPIC_ERR_ARG;
+  if (!ctx->boundary_set) return ctx->fail(LEPIC_ERR_STATE, "lepic_set_boundary not called");
+  const int nb = (int)std::min<int64_t>((ctx->nodes + 255) / 256, kTotBlocks);
+  double e2 = 0.0;
+  if (int rc = sum_partials(ctx, nb, [&](double* part) {
+        hipLaunchKernelGGL(k_field_e2, dim3(nb), dim3(256), 0, ctx->stream, (const double*)ctx->E,
+                           (const int8_t*)ctx->bc, ctx->nodes, part);
+      }, &e2))
+    return rc;
+  *W = 0.5 * ctx->g.eps0 * e2 * ctx->g.h1 * ctx->g.h2 * ctx->g.zmax;
+  return 0;
+}
+
 }  // extern "C"
+
