@@ -60,6 +60,7 @@ class Context:
         self.lib = capi.load()
         self.n1, self.n2, self.h = n1, n2, (h1, h2)
         self.nsp = len(species)
+        self.ngrid = ngrid
         self.species = list(species)
         self.precision = precision
         g = capi.GridDesc(n1, n2, h1, h2, zmax, dt, eps0, qe, ng, ngrid, len(species), precision,
@@ -220,7 +221,12 @@ class Context:
     def charge_deposition(self, ptype):
         self._check(self.lib.lepic_charge_deposition(self._ctx, ptype))
 
-    def mover_tally(self, ptype, ngrid=1):
+    def mover_tally(self, ptype, ngrid=None):
+        """P_loss(1:4), p_mac (2, ngrid+1) and cnt_dead of one species; p_mac
+        has one column per wall label of the context (ngrid as created)."""
+        if ngrid is not None and ngrid != self.ngrid:
+            raise ValueError(f"the context has ngrid={self.ngrid}, not {ngrid}")
+        ngrid = self.ngrid
         P = np.zeros(4)
         pm = np.zeros(2 * (ngrid + 1))
         cd = np.zeros(1)
