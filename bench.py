@@ -189,11 +189,13 @@ def build_ctx(W, precision, seed=20261202):
 
 
 def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_steps=40, train_iters=300):
-    """One mode of the step loop.  With learned=True the first learn_steps
-    (untimed) solves record (drhs, dphi) samples, the network is trained on
-    them (its time reported apart), and every later solve starts from the
-    network's guess when that lowers the initial residual (the library skips
-    the network for a growing back-off after a guess that gains too little)."""
+    """One mode of the step loop: learn_steps untimed steps, warmup untimed
+    steps, then `steps` timed steps, so every mode times the same steps of
+    the run.  With learned=True the first learn_steps solves record (drhs,
+    dphi) samples, the network is trained on them (its time reported apart),
+    and every later solve starts from the network's guess when that lowers
+    the initial residual (the library skips the network for a growing
+    back-off after a guess that gains too little)."""
     ctx = build_ctx(W, precision)
     ctx.kernel_timing(True)      # mover kernels bracketed by events inside the library
     stream = C.c_void_p()
@@ -203,7 +205,7 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
     ev_end = ev[4 + 2 * nsp]
     acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[] for _ in range(nsp)],
                net_used=0, net_probed=0, res0_gain=[], train_s=0.0, loss=None)
-    pre = learn_steps if learned else 0
+    pre = learn_steps
     if learned:
         ctx.net_create(capacity=64, seed=seed)
         ctx.net_recording(True)
@@ -557,12 +559,13 @@ def main():
                      train_iters=args.train_iters)
         res["mixed_learned"] = (r, summary(*r))
     if not args.skip_plain_mixed:
-        print(f"[bench] mixed precision, {N} particles, {args.warmup}+{K} steps", file=sys.stderr, flush=True)
-        r = run_mode(W, L.MIXED, K, args.warmup, hip)
+        print(f"[bench] mixed precision, {N} particles, {args.learn_steps}+{args.warmup}+{K} steps", file=sys.stderr,
+              flush=True)
+        r = run_mode(W, L.MIXED, K, args.warmup, hip, learn_steps=args.learn_steps)
         res["mixed"] = (r, summary(*r))
     if not args.skip_fp64:
         print("[bench] FP64", file=sys.stderr, flush=True)
-        r = run_mode(W, L.FP64, K, args.warmup, hip)
+        r = run_mode(W, L.FP64, K, args.warmup, hip, learn_steps=args.learn_steps)
         res["fp64"] = (r, summary(*r))
     head = "mixed_learned" if "mixed_learned" in res else "mixed"
     (acc_h, _, cnt_h), s_h = res[head]
