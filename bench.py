@@ -40,9 +40,9 @@ from jd_lepic_amd import reactions as RX, tables  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per particle of one mixed mover launch at this workload, measured
 # with rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE, each scaled by what
-# it reads for a 1 GiB device copy): 51.1 B against 48 algorithmic bytes
+# it reads for a 1 GiB device copy): 51.4 B against 48 algorithmic bytes
 # (profiles/mover_mixed_traffic_512x512_100ppc.md)
-TRAFFIC_BYTES_PER_PARTICLE = 51.1
+TRAFFIC_BYTES_PER_PARTICLE = 51.4
 
 
 class Hip:
