@@ -393,6 +393,7 @@ int lepic_destroy(lepic_ctx* ctx) {
   for (int p = 0; p < kMaxSpecies; ++p)
     for (int k = 0; k < 2; ++k)
       if (ctx->ev_mover[p][k]) (void)hipEventDestroy(ctx->ev_mover[p][k]);
+  if (ctx->ev_sync) (void)hipEventDestroy(ctx->ev_sync);
   for (int p = 0; p < ctx->nsp; ++p) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);
@@ -400,7 +401,7 @@ int lepic_destroy(lepic_ctx* ctx) {
     (void)hipFree(s.seg[0]); (void)hipFree(s.seg[1]);
   }
   void* ptrs[] = {ctx->bc, ctx->phi, ctx->phi_given, ctx->rhs, ctx->E, ctx->Ef, ctx->Bi, ctx->Bf,
-                  ctx->d_ctrl, ctx->d_partials, ctx->d_scalar, ctx->d_resid64, ctx->d_corr,
+                  ctx->d_ctrl, ctx->d_scalar, ctx->d_resid64, ctx->d_corr,
                   ctx->d_counts, ctx->d_tally, ctx->d_segoff, ctx->d_ticket, ctx->d_error,
                   ctx->s_key[0], ctx->s_key[1], ctx->s_val[0], ctx->s_val[1], ctx->s_hist,
                   ctx->d_plist, ctx->d_scan_tmp, ctx->rx.d_sig_Er, ctx->rx.d_sig,

@@ -175,6 +175,7 @@ struct lepic_ctx {
   unsigned long long ticket_base = 0;       // tickets taken by earlier launches
   bool timing = false;                      // lepic_kernel_timing
   hipEvent_t ev_mover[lepic::kMaxSpecies][2] = {};
+  hipEvent_t ev_sync = nullptr;      // polled by the solver's per-V-cycle read-back
   bool ev_recorded[lepic::kMaxSpecies] = {};
   int32_t* d_error = nullptr;        // device-side error flags
 

@@ -49,6 +49,8 @@ constexpr int kSmallTileBelow = 64;           // a level with fewer wide tiles t
 constexpr int kSweepThreads = 1024;
 constexpr int kDefaultTailNodes = 5000;
 constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
+// byte offset of the residual partials behind the scalars, visit log and status
+constexpr size_t kPartialsOff = (64 * 8 + (kVisitLog + 1) * sizeof(int) + 7) / 8 * 8;
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
@@ -1276,17 +1278,22 @@ double sum_resid_partials(const double* p, int nb) {
 
 // read back the visit log, the per-visit residuals and (mixed) the FP64
 // residual partial sums of the finest level
+// The V-cycle's visit residuals and log (and the FP64 residual's partials)
+// in one copy; the host then waits for the stream by polling an event, which
+// returns as soon as the copy lands instead of after a blocking wake-up.
 int read_back(lepic_ctx* ctx, int* visit, double* vres, double* resid_sum) {
   double* h = ctx->h_pinned;
-  LEPIC_HIP(hipMemcpyAsync(h, ctx->d_scalar + 32, 32 * 8 + (kVisitLog + 1) * sizeof(int),
-                           hipMemcpyDeviceToHost, ctx->stream));
   const int nb = resid_blocks(ctx);
-  if (resid_sum)
-    LEPIC_HIP(hipMemcpyAsync(h + 128, ctx->d_partials, nb * 8, hipMemcpyDeviceToHost, ctx->stream));
-  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  const size_t bytes = resid_sum ? kPartialsOff - 32 * 8 + (size_t)nb * 8 : 32 * 8 + (kVisitLog + 1) * sizeof(int);
+  LEPIC_HIP(hipMemcpyAsync(h, ctx->d_scalar + 32, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipEventRecord(ctx->ev_sync, ctx->stream));
+  hipError_t q;
+  while ((q = hipEventQuery(ctx->ev_sync)) == hipErrorNotReady) {
+  }
+  LEPIC_HIP(q);
   std::memcpy(vres, h, kVisitLog * 8);
   std::memcpy(visit, (char*)h + 32 * 8, kVisitLog * sizeof(int));
-  if (resid_sum) *resid_sum = sum_resid_partials(h + 128, nb);
+  if (resid_sum) *resid_sum = sum_resid_partials((const double*)((const char*)h + kPartialsOff - 32 * 8), nb);
   return 0;
 }
 
@@ -1375,11 +1382,14 @@ int poisson_setup(lepic_ctx* ctx) {
   LEPIC_HIP(hipMalloc(&ctx->d_ctrl, (kMaxLevels + 1) * sizeof(LevelCtrl)));
   LEPIC_HIP(hipMemsetAsync(ctx->d_ctrl, 0, (kMaxLevels + 1) * sizeof(LevelCtrl), ctx->stream));
   ctx->mg_epoch = 0;
-  LEPIC_HIP(hipMalloc(&ctx->d_partials, kResidBlocks * 8));
-  LEPIC_HIP(hipMemsetAsync(ctx->d_partials, 0, kResidBlocks * 8, ctx->stream));
+  // one allocation: 64 doubles of scalars (the visit residuals from 32),
+  // the visit log and status, then the FP64 residual's block partials, so a
+  // V-cycle's results come back in one copy
+  LEPIC_HIP(hipMalloc(&ctx->d_scalar, kPartialsOff + kResidBlocks * 8));
+  LEPIC_HIP(hipMemsetAsync(ctx->d_scalar, 0, kPartialsOff + kResidBlocks * 8, ctx->stream));
+  ctx->d_partials = (double*)((char*)ctx->d_scalar + kPartialsOff);
   ctx->partials_cap = kResidBlocks;
-  LEPIC_HIP(hipMalloc(&ctx->d_scalar, 64 * 8 + (kVisitLog + 1) * sizeof(int)));
-  LEPIC_HIP(hipMemsetAsync(ctx->d_scalar, 0, 64 * 8 + (kVisitLog + 1) * sizeof(int), ctx->stream));
+  if (!ctx->ev_sync) LEPIC_HIP(hipEventCreateWithFlags(&ctx->ev_sync, hipEventDisableTiming));
   return 0;
 }
 
