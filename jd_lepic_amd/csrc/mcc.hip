@@ -28,6 +28,7 @@ namespace {
 
 constexpr int kMaxPcol = 16;     // reactions per incident species
 constexpr int kMaxCreate = 3;    // particles created per event
+constexpr size_t kMccLdsTable = 48 * 1024;   // largest LDS-staged cross-section table
 
 struct MccTab {
   int ncol, npt, ntype_all, n_rg, iyl, iyr, ixl[5], ixr[5];
@@ -321,6 +322,7 @@ struct ProdArgs {
   const double* sig;
   PartPtrs P;
   const int64_t* counts;
+  int64_t N0;                    // incident species' count before this call's appends
   int mixed, nsp, n1, q;         // q: incident species (0-based)
   double h1, h2, qe, pi;
   SpeciesMcc S;
@@ -366,21 +368,38 @@ __device__ inline void write_v(const ProdArgs& A, int q, int64_t i, double vx, d
   else { A.P.d[q][3][i] = vx; A.P.d[q][4][i] = vy; A.P.d[q][5][i] = vz; }
 }
 
-template <bool kMixed>
+// kLds: the union energy grid and the incident species' cross-section rows
+// are staged in LDS (dynamic, (1 + p_ncol) * npt doubles) before the
+// candidates search them; otherwise they are read from global memory.
+template <bool kMixed, bool kLds>
 __global__ __launch_bounds__(256) void k_mcc_prod(ProdArgs A) {
   __shared__ double s_cnt[3][kMaxPcol][5];
   __shared__ unsigned long long s_ev[kMaxPcol];
   __shared__ double s_pl[kMaxSpecies + 1];
+  extern __shared__ double s_tab[];
   const MccTab& T = *A.tab;
   const SpeciesMcc& S = A.S;
   const int t = threadIdx.x;
+  const int npt = T.npt;
   for (int e = t; e < 3 * kMaxPcol * 5; e += 256) (&s_cnt[0][0][0])[e] = 0.0;
   if (t < kMaxPcol) s_ev[t] = 0;
   if (t <= kMaxSpecies) s_pl[t] = 0.0;
+  if (kLds) {
+    for (int e = t; e < npt; e += 256) s_tab[e] = A.sig_Er[e];
+    for (int icol = 0; icol < S.p_ncol; ++icol) {
+      const double* row = A.sig + (int64_t)(S.reac[icol] - 1) * npt;
+      for (int e = t; e < npt; e += 256) s_tab[(int64_t)(1 + icol) * npt + e] = row[e];
+    }
+  }
   __syncthreads();
+  const double* Er = kLds ? s_tab : A.sig_Er;
   const int64_t c = (int64_t)blockIdx.x * 256 + t;
   const int q = A.q, pt = q + 1;
-  const int64_t N = A.counts[q];
+  // candidates come from the particles present when the call began: products
+  // appended to this species by an earlier species' collisions in the same
+  // call do not collide (collisions.f90:184, 275), and the claim bitmap is
+  // sized for this count
+  const int64_t N = A.N0;
   bool active = c < A.Nc && N > 0;
   Rng rng{A.k0, A.k1, (uint32_t)c, (uint32_t)(c >> 32) ^ ((uint32_t)pt << 24), A.it, 0u, U4{0, 0, 0, 0}, 2};
   int64_t ip = -1;
@@ -430,16 +449,15 @@ __global__ __launch_bounds__(256) void k_mcc_prod(ProdArgs A) {
       }
       nu[icol] = 0.0;
       const double E = tE[k];
-      const int npt = T.npt;
-      if (E > A.sig_Er[0] && E <= A.sig_Er[npt - 1]) {
-        int lo = 0, hi = npt - 1;  // sig_Er[lo] < E <= sig_Er[hi]
+      if (E > Er[0] && E <= Er[npt - 1]) {
+        int lo = 0, hi = npt - 1;  // Er[lo] < E <= Er[hi]
         while (hi - lo > 1) {
           int mid = (lo + hi) >> 1;
-          if (A.sig_Er[mid] < E) lo = mid; else hi = mid;
+          if (Er[mid] < E) lo = mid; else hi = mid;
         }
-        const int cc = S.reac[icol] - 1;
-        double sL = A.sig[(int64_t)cc * npt + lo], sR = A.sig[(int64_t)cc * npt + hi];
-        double EkL = A.sig_Er[lo], EkR = A.sig_Er[hi];
+        const double* row = kLds ? s_tab + (int64_t)(1 + icol) * npt : A.sig + (int64_t)(S.reac[icol] - 1) * npt;
+        double sL = row[lo], sR = row[hi];
+        double EkL = Er[lo], EkR = Er[hi];
         double sig_p = sL + (E - EkL) * (sR - sL) / (EkR - EkL);
         nu[icol] = T.np_mx[tt] * sig_p * tvr[k] / S.nu_omp;
         sum_nu += nu[icol];
@@ -447,7 +465,7 @@ __global__ __launch_bounds__(256) void k_mcc_prod(ProdArgs A) {
           atomicAdd(&s_cnt[0][icol][rg_hit], nu[icol] * S.nu_omp);
           atomicAdd(&s_cnt[1][icol][rg_hit], 1.0);
         }
-      } else if (E > A.sig_Er[npt - 1]) {
+      } else if (E > Er[npt - 1]) {
         atomicOr(A.status, 2);  // energy above the table (the reference asks on stdin)
       }
     }
@@ -755,7 +773,9 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
         LEPIC_HIP(hipMalloc(&ctx->d_claim, words * 4));
         ctx->claim_cap = words;
       }
-      const size_t stage_bytes = (size_t)Nc * (kMaxCreate * (1 + 24) + 8 + 4 * kMaxSpecies) + 1024;
+      // staging: velocities, incident index, per-species create counts, the
+      // scanned copy of one species' counts, byproduct types
+      const size_t stage_bytes = (size_t)Nc * (kMaxCreate * (1 + 24) + 8 + 4 * kMaxSpecies + 4) + 1024;
       if ((int64_t)stage_bytes > ctx->stage_cap) {
         if (ctx->d_mcc_stage) LEPIC_HIP(hipFree(ctx->d_mcc_stage));
         LEPIC_HIP(hipMalloc(&ctx->d_mcc_stage, stage_bytes));
@@ -765,6 +785,7 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
       double* st_v = (double*)st;                                   st += (size_t)Nc * kMaxCreate * 24;
       int64_t* st_ip = (int64_t*)st;                                st += (size_t)Nc * 8;
       uint32_t* ncreate = (uint32_t*)st;                            st += (size_t)Nc * 4 * kMaxSpecies;
+      uint32_t* offs = (uint32_t*)st;                               st += (size_t)Nc * 4;
       int8_t* st_type = (int8_t*)st;
       if (int rc = zero_async(ctx, ctx->d_claim, words * 4)) return rc;
       if (int rc = zero_async(ctx, ncreate, (size_t)Nc * 4 * kMaxSpecies)) return rc;
@@ -773,6 +794,7 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
       for (int rg = 0; rg < R.n_rg; ++rg) std::memcpy(&Sq.rg_inv[rg], ctx->h_pinned + rg, 8);
       ProdArgs A{};
       A.tab = dtab; A.sig_Er = R.d_sig_Er; A.sig = R.d_sig; A.P = P; A.counts = ctx->d_counts;
+      A.N0 = N[q];
       A.mixed = ctx->g.precision == LEPIC_MIXED; A.nsp = ctx->nsp; A.n1 = ctx->g.n1; A.q = q;
       A.h1 = ctx->g.h1; A.h2 = ctx->g.h2; A.qe = ctx->g.qe; A.pi = pi; A.S = Sq;
       A.k0 = k0; A.k1 = k1; A.it = (uint32_t)it;
@@ -781,14 +803,17 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
       A.cnt_col = R.d_cnt_col; A.ploss3 = d_ploss3; A.events = d_events; A.missed = d_missed;
       A.status = d_status;
       const dim3 g((unsigned)((Nc + 255) / 256));
-      if (A.mixed) hipLaunchKernelGGL(k_mcc_prod<true>, g, dim3(256), 0, ctx->stream, A);
-      else hipLaunchKernelGGL(k_mcc_prod<false>, g, dim3(256), 0, ctx->stream, A);
+      // cross-section rows of this species in LDS when they fit in 48 KB
+      const size_t tab_bytes = (size_t)(1 + Sq.p_ncol) * R.npt * 8;
+      const bool lds = tab_bytes <= kMccLdsTable;
+      if (A.mixed && lds) hipLaunchKernelGGL((k_mcc_prod<true, true>), g, dim3(256), tab_bytes, ctx->stream, A);
+      else if (A.mixed) hipLaunchKernelGGL((k_mcc_prod<true, false>), g, dim3(256), 0, ctx->stream, A);
+      else if (lds) hipLaunchKernelGGL((k_mcc_prod<false, true>), g, dim3(256), tab_bytes, ctx->stream, A);
+      else hipLaunchKernelGGL((k_mcc_prod<false, false>), g, dim3(256), 0, ctx->stream, A);
       LEPIC_HIP(hipGetLastError());
       for (int b = 0; b < ctx->nsp; ++b) {
         uint32_t* cnt = ncreate + (size_t)b * Nc;
         // keep the raw counts for the total: scan a copy
-        uint32_t* offs = nullptr;
-        LEPIC_HIP(hipMallocAsync((void**)&offs, (size_t)Nc * 4, ctx->stream));
         LEPIC_HIP(hipMemcpyAsync(offs, cnt, (size_t)Nc * 4, hipMemcpyDeviceToDevice, ctx->stream));
         if (int rc = scan_exclusive_u32(ctx, offs, Nc)) return rc;
         if (A.mixed) hipLaunchKernelGGL(k_mcc_append<true>, g, dim3(256), 0, ctx->stream, A, (const uint32_t*)offs, b);
@@ -796,7 +821,6 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
         hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counts, b,
                            (const uint32_t*)offs, (const uint32_t*)cnt, Nc, d_created);
         LEPIC_HIP(hipGetLastError());
-        LEPIC_HIP(hipFreeAsync(offs, ctx->stream));
       }
     }
     int32_t st = 0;

@@ -39,3 +39,34 @@ def argon_case(n=16, N=20000, Te=3.0, Tgas=0.02585, ngas=7.2429e19, sig_el=1e-19
              2: dict(vxp=ion, np_tot=N, flag_dead=np.zeros(cap, np.int8))}
     np_red = [np.zeros((n + 3, n + 3)), np.zeros((n + 3, n + 3))]
     return dom, chem, tab, setup, parts, np_red
+
+
+def step_sigma(E, Eth, s0):
+    """sigma = s0 for E >= Eth, 0 below; the table carries a node just under
+    the threshold so the linear interpolation ramps over 1e-12 of Eth only."""
+    E = np.asarray(E, float)
+    return np.where(E >= Eth, s0, 0.0)
+
+
+def argon_step_chemistry(ngas, sig_el=1e-19, sig_exc=5e-20, sig_ion=5e-20, sig_cx=0.0, emax=500.0,
+                         Tgas=0.0):
+    """e-/Ar+ in Ar with constant cross sections above threshold (elastic from
+    0, excitation from 11.5 eV, ionization from 15.76 eV) and, when sig_cx > 0,
+    Ar+ + Ar charge exchange with a constant cross section.  With the target
+    at rest (Tgas = 0) each production rate has the closed form
+    n_gas * sigma * <|v| 1(E >= Eth)> over the incident particles."""
+    chem = RX.Chemistry([RX.Species("[e]", RX.ME, -RX.QE, 0.0, 1.0),
+                         RX.Species("[Ar+]", 39.95 * RX.AMU, RX.QE, 0.03, 1.0),
+                         RX.Species("[Ar]", -39.95 * RX.AMU, 0.0, Tgas, ngas)], [])
+    ths = (11.5, 15.76)
+    E = RX.energy_grid(ths + tuple(t * (1.0 - 1e-12) for t in ths), emax=emax, n=120)
+    chem.reactions.append(RX.Reaction("[e]+[Ar]->[e]+[Ar]", [1, 3], [1, 3], RX.COLLISION, 0.0, 0.0, E,
+                                      np.full_like(E, sig_el)))
+    chem.reactions.append(RX.Reaction("[e]+[Ar]->[e]+[Ar]", [1, 3], [1, 3], RX.EXCITATION, 11.5, 0.0, E,
+                                      step_sigma(E, 11.5, sig_exc)))
+    chem.reactions.append(RX.Reaction("[e]+[Ar]->[e]+[Ar+]+[e]", [1, 3], [1, 2, 1], RX.IONIZATION, 15.76,
+                                      0.0, E, step_sigma(E, 15.76, sig_ion)))
+    if sig_cx > 0.0:
+        chem.reactions.append(RX.Reaction("[Ar+]+[Ar]->[Ar]+[Ar+]", [2, 3], [3, 2], RX.CHARGEEXCHANGE, 0.0,
+                                          0.0, E, np.full_like(E, sig_cx)))
+    return chem
