@@ -184,6 +184,12 @@ def _declare(L):
     L.lo_ordering.restype = C.c_int
     L.lo_ordering.argtypes = [C.POINTER(LoReactionInput), _dp, C.c_double, C.c_int, _dp, _dp,
                               C.POINTER(LoReactionTable)]
+    L.lo_time_push_deposit_omp.restype = C.c_double
+    L.lo_time_push_deposit_omp.argtypes = [C.POINTER(LoDomain), C.POINTER(LoSpecies), _dp, _dp, _dp,
+                                           C.c_longlong, C.c_int, C.c_int]
+    L.lo_time_collisions_omp.restype = C.c_double
+    L.lo_time_collisions_omp.argtypes = [C.POINTER(LoDomain), C.POINTER(LoMccSetup), C.POINTER(LoReactionTable),
+                                         C.POINTER(LoParticles), C.POINTER(_dp), C.c_int]
     L.lo_collisions.restype = C.c_int
     L.lo_collisions.argtypes = [C.POINTER(LoDomain), C.POINTER(LoMccSetup),
                                 C.POINTER(LoReactionTable), C.POINTER(LoParticles),
@@ -612,3 +618,45 @@ def time_push_deposit(dom: Domain, charge, mass, Nm, Ei, vxp, reps=1):
         lib().lo_charge_deposition(C.byref(d), C.byref(sp), _p(work), npt.value, _p(k), _p(rho))
         total += time.perf_counter() - t0
     return total / reps
+
+
+def time_push_deposit_omp(dom: Domain, charge, mass, Nm, Ei, vxp, nthreads, reps=1):
+    """Seconds of `reps` steps of push + deposit of one species with the
+    particles split into nthreads partitions, one OpenMP thread each with its
+    own density array (the reference's main.f90:1024-1089); vxp unchanged."""
+    Ei = np.ascontiguousarray(Ei, dtype=np.float64)
+    src = np.ascontiguousarray(vxp, dtype=np.float64).reshape(-1, 6)
+    d = dom.struct()
+    sp = LoSpecies(charge, mass, Nm, 0)
+    k = kq(dom)
+    return lib().lo_time_push_deposit_omp(C.byref(d), C.byref(sp), _p(Ei), _p(k), _p(src), src.shape[0],
+                                          int(nthreads), int(reps))
+
+
+def time_collisions_omp(dom: Domain, setup: dict, table: dict, parts: dict, np_red, nthreads):
+    """Seconds of one collisions call with every species' particles split
+    into nthreads partitions, each thread colliding its own partition with
+    its own seed (collision_OMP per thread, collisions.f90:160-181)."""
+    s = LoMccSetup()
+    for k in ("ntype", "ntype_all", "ns_coll", "plt_src", "n_cell", "n_rg", "iyl_rg", "iyr_rg",
+              "flag_ionrep"):
+        setattr(s, k, int(setup[k]))
+    for k in ("charge", "mass", "Nm", "vt0", "np_mx", "nu_uplim"):
+        arr = getattr(s, k)
+        for p, v in setup[k].items():
+            arr[p] = v
+    for j in range(setup["n_rg"]):
+        s.ixl_rg[j] = setup["ixl_rg"][j]
+        s.ixr_rg[j] = setup["ixr_rg"][j]
+    P = LoParticles()
+    keep = []
+    for p, dd in parts.items():
+        v = np.ascontiguousarray(dd["vxp"], dtype=np.float64)
+        keep.append(v)
+        P.vxp[p] = _p(v)
+        P.np_tot[p] = dd["np_tot"]
+        P.cap[p] = v.shape[0]
+    reds = [np.ascontiguousarray(a, dtype=np.float64) for a in np_red]
+    d = dom.struct()
+    return lib().lo_time_collisions_omp(C.byref(d), C.byref(s), C.byref(table["struct"]), C.byref(P),
+                                        (_dp * len(reds))(*[_p(a) for a in reds]), int(nthreads))
