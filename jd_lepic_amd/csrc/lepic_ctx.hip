@@ -291,6 +291,8 @@ int alloc_species(lepic_ctx* ctx, int p) {
   s.packed = true;
   LEPIC_HIP(hipMalloc(&s.np, ctx->nodes * 8));
   LEPIC_HIP(hipMemsetAsync(s.np, 0, ctx->nodes * 8, ctx->stream));
+  LEPIC_HIP(hipMalloc(&s.acc, ctx->nodes * 8));
+  LEPIC_HIP(hipMemsetAsync(s.acc, 0, ctx->nodes * 8, ctx->stream));
   LEPIC_HIP(hipMalloc(&s.np_red, ctx->nodes * 8));
   LEPIC_HIP(hipMemsetAsync(s.np_red, 0, ctx->nodes * 8, ctx->stream));
   LEPIC_HIP(hipMalloc(&s.sink, ctx->nodes * 8));
@@ -397,14 +399,15 @@ int lepic_destroy(lepic_ctx* ctx) {
   for (int p = 0; p < ctx->nsp; ++p) {
     Species& s = ctx->sp[p];
     for (int b = 0; b < 2; ++b) (void)hipFree(s.mem[b]);
-    (void)hipFree(s.flag_dead); (void)hipFree(s.np); (void)hipFree(s.np_red); (void)hipFree(s.sink); (void)hipFree(s.src);
+    (void)hipFree(s.flag_dead); (void)hipFree(s.np); (void)hipFree(s.acc); (void)hipFree(s.np_red); (void)hipFree(s.sink); (void)hipFree(s.src);
     (void)hipFree(s.seg[0]); (void)hipFree(s.seg[1]);
   }
   void* ptrs[] = {ctx->bc, ctx->phi, ctx->phi_given, ctx->rhs, ctx->E, ctx->Ef, ctx->Bi, ctx->Bf,
                   ctx->d_ctrl, ctx->d_scalar, ctx->d_resid64, ctx->d_corr,
                   ctx->d_counts, ctx->d_tally, ctx->d_segoff, ctx->d_ticket, ctx->d_error,
                   ctx->s_key[0], ctx->s_key[1], ctx->s_val[0], ctx->s_val[1], ctx->s_hist,
-                  ctx->d_plist, ctx->d_scan_tmp, ctx->rx.d_sig_Er, ctx->rx.d_sig,
+                  ctx->d_plist, ctx->d_scan_tmp, ctx->s_meta, ctx->s_rkey, ctx->s_rcnt, ctx->s_rcntd,
+                  ctx->s_nrun, ctx->rx.d_sig_Er, ctx->rx.d_sig,
                   ctx->rx.d_sig_Er_f, ctx->rx.d_sig_f, ctx->rx.d_cnt_col, ctx->d_claim,
                   ctx->d_mcc_stage, ctx->d_mcc_i64, ctx->d_mcc_f64, ctx->d_mcc_tab, ctx->d_heat,
                   ctx->d_stage6, ctx->d_stype, ctx->d_srank, ctx->d_srctally, ctx->d_cellwall,
@@ -609,6 +612,38 @@ int lepic_get_count(lepic_ctx* ctx, int32_t ptype, int64_t* np) {
   return 0;
 }
 
+namespace {
+
+// particles [first, first+n) of a packed species into host vxp(6,n)
+int download_span(lepic_ctx* ctx, int32_t ptype, int64_t first, int64_t n, double* vxp) {
+  Species& s = ctx->sp[ptype - 1];
+  int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), kStage);
+  double* stage = nullptr;
+  LEPIC_HIP(hipMalloc(&stage, chunk * 48));
+  const int b = s.cur;
+  for (int64_t off = 0; off < n; off += chunk) {
+    const int64_t m = std::min(chunk, n - off), g = first + off;
+    dim3 grid((unsigned)((m + 255) / 256));
+    if (ctx->g.precision == LEPIC_MIXED) {
+      hipLaunchKernelGGL(k_mixed_to_aos, grid, dim3(256), 0, ctx->stream, stage, m, ctx->g.h1, ctx->g.h2,
+                         s.cell[b] + g, s.f[b][0] + g, s.f[b][1] + g,
+                         s.f[b][2] ? s.f[b][2] + g : nullptr, s.f[b][3] + g, s.f[b][4] + g,
+                         s.f[b][5] + g);
+    } else {
+      hipLaunchKernelGGL(k_soa64_to_aos, grid, dim3(256), 0, ctx->stream, stage, m, s.d[b][0] + g,
+                         s.d[b][1] + g, s.d[b][2] + g, s.d[b][3] + g, s.d[b][4] + g,
+                         s.d[b][5] + g);
+    }
+    LEPIC_HIP(hipGetLastError());
+    LEPIC_HIP(hipMemcpyAsync(vxp + 6 * off, stage, m * 48, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  LEPIC_HIP(hipFree(stage));
+  return 0;
+}
+
+}  // namespace
+
 int lepic_download_particles(lepic_ctx* ctx, int32_t ptype, double* vxp, int64_t cap, int64_t* np) {
   if (!ctx || !valid_ptype(ctx, ptype) || !np) return LEPIC_ERR_ARG;
   if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
@@ -617,30 +652,16 @@ int lepic_download_particles(lepic_ctx* ctx, int32_t ptype, double* vxp, int64_t
   *np = n;
   if (n > cap) return ctx->fail(LEPIC_ERR_CAPACITY, "host buffer too small");
   if (n > 0 && !vxp) return LEPIC_ERR_ARG;
-  Species& s = ctx->sp[ptype - 1];
-  int64_t chunk = std::min<int64_t>(std::max<int64_t>(n, 1), kStage);
-  double* stage = nullptr;
-  LEPIC_HIP(hipMalloc(&stage, chunk * 48));
-  const int b = s.cur;
-  for (int64_t off = 0; off < n; off += chunk) {
-    int64_t m = std::min(chunk, n - off);
-    dim3 grid((unsigned)((m + 255) / 256));
-    if (ctx->g.precision == LEPIC_MIXED) {
-      hipLaunchKernelGGL(k_mixed_to_aos, grid, dim3(256), 0, ctx->stream, stage, m, ctx->g.h1, ctx->g.h2,
-                         s.cell[b] + off, s.f[b][0] + off, s.f[b][1] + off,
-                         s.f[b][2] ? s.f[b][2] + off : nullptr, s.f[b][3] + off, s.f[b][4] + off,
-                         s.f[b][5] + off);
-    } else {
-      hipLaunchKernelGGL(k_soa64_to_aos, grid, dim3(256), 0, ctx->stream, stage, m, s.d[b][0] + off,
-                         s.d[b][1] + off, s.d[b][2] + off, s.d[b][3] + off, s.d[b][4] + off,
-                         s.d[b][5] + off);
-    }
-    LEPIC_HIP(hipGetLastError());
-    LEPIC_HIP(hipMemcpyAsync(vxp + 6 * off, stage, m * 48, hipMemcpyDeviceToHost, ctx->stream));
-  }
-  LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipFree(stage));
-  return 0;
+  return download_span(ctx, ptype, 0, n, vxp);
+}
+
+int lepic_download_particles_range(lepic_ctx* ctx, int32_t ptype, int64_t first, int64_t count, double* vxp) {
+  if (!ctx || !valid_ptype(ctx, ptype) || first < 0 || count < 0 || (count > 0 && !vxp)) return LEPIC_ERR_ARG;
+  if (int rc = ensure_packed(ctx, ptype - 1)) return rc;
+  int64_t n = 0;
+  if (int rc = lepic_get_count(ctx, ptype, &n)) return rc;
+  if (first + count > n) return ctx->fail(LEPIC_ERR_ARG, "range beyond the species' particles");
+  return download_span(ctx, ptype, first, count, vxp);
 }
 
 int lepic_set_flag_dead(lepic_ctx* ctx, int32_t ptype, const int8_t* flags, int64_t np) {

@@ -85,6 +85,7 @@ struct Species {
   float* f[2][6] = {};               // mixed: fx fy z vx vy vz
   int8_t* flag_dead = nullptr;
   double* np = nullptr;              // deposited density (nodes)
+  int64_t* acc = nullptr;            // fixed-point deposition accumulator (nodes), scaled into np
   double* np_red = nullptr;
   double* sink = nullptr;            // ss2D(2): sink counts (nodes)
   double* src = nullptr;             // ss2D(1): source counts (nodes)
@@ -187,6 +188,15 @@ struct lepic_ctx {
   int32_t* d_plist = nullptr;        // cell-end list (pl_max+1)
   uint32_t* d_scan_tmp = nullptr;
   int64_t plist_cap = 0;           // entries of d_plist
+  // segment sort: per-slot (run, rank) of the segment-local sort, the runs
+  // of every segment (cell key, particle count) at a stride of kChunk, the
+  // dense run counts, and the per-segment run counts / bases
+  uint32_t* s_meta = nullptr;
+  uint32_t* s_rkey = nullptr;
+  uint32_t* s_rcnt = nullptr;
+  uint32_t* s_rcntd = nullptr;
+  uint32_t* s_nrun = nullptr;
+  int64_t s_seg_cap = 0;           // slots covered by s_meta / s_rkey / s_rcnt / s_rcntd (segments x kChunk)
   double* d_diag = nullptr;          // per-block partials of the charge/energy totals
 
   // collisions
@@ -210,6 +220,10 @@ struct lepic_ctx {
   uint32_t* d_srank = nullptr;
   int64_t stage6_cap = 0;
   void* d_srctally = nullptr;
+  // lepic_step with opt_inj = +-2: wall losses of every push since the last
+  // injection, per species (N_inj of main.f90:1044-1056, reset after the
+  // injection at :1063-1071)
+  double inj_lost[lepic::kMaxSpecies] = {};
 
   int fail(int code, const std::string& msg) {
     err = msg;

@@ -9,13 +9,22 @@
 // buffer; the segments' contents in segment order are the species'
 // particles, so no prefix across blocks is needed and blocks never wait for
 // each other.  Deposition merges equal cells within a thread and in runs of
-// adjacent lanes (segmented DPP sums), accumulates in an LDS window of 16 rows
-// x 64 columns of nodes around the segment's first cell, and flushes the
-// window to the global density with FP64 atomics; nodes outside the window go
-// to the global density directly.  The FP64 path computes exactly the
+// adjacent lanes (segmented DPP sums, a fixed order), accumulates in an LDS
+// window of 16 rows x 64 columns of nodes around the segment's first cell,
+// and flushes the window to the species' global accumulator; nodes outside
+// the window go to the accumulator directly.  Window and accumulator hold
+// fixed-point integers (CIC weight x 2^kFixShift64 in FP64 contexts,
+// x 2^kFixShiftMx in mixed ones): integer addition is associative, so the
+// density is the same bit for bit whatever order the atomics land in, the
+// same as a flush in a fixed order.  k_np_from_acc then scales the
+// accumulator to np = Nm/(h1 h2 zmax) * kq * sum (charge_deposition,
+// part_expmover.f90:555-584).  The FP64 path computes exactly the
 // reference's expressions (no contraction: the library is built with
 // -ffp-contract=off).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
 
 #include "lepic_internal.h"
 
@@ -33,6 +42,16 @@ constexpr int kWinCols = 64;
 constexpr int kWinX0 = 8;
 constexpr int kWinY0 = 7;
 constexpr int kWin = kWinRows * kWinCols;
+// Fixed-point units of the deposition: a particle's four CIC weights sum to
+// 1.  FP64 contexts round each run sum to 2^-42 of a particle (an absolute
+// error below 1.2e-13 of one particle's charge per run; a node holds up to
+// 2^21 particles); mixed contexts to 2^-20 (finer than FP32 accumulation
+// once a node holds more than 8 particles; an int32 LDS window node holds
+// up to 2^11 particles, more than a 1024-slot segment can bring).
+constexpr int kFixShift64 = 42;
+constexpr int kFixShiftMx = 20;
+constexpr double kFix64 = 4398046511104.0;   // 2^42
+constexpr float kFixMx = 1048576.0f;         // 2^20
 // threads per mover block; a block owns one segment of kChunk slots
 constexpr int kMoverThreadsMixed = kBlock;
 constexpr int kMoverThreads64 = 2 * kBlock;   // two particles per thread: FP64 state of 4 would cost occupancy
@@ -48,7 +67,7 @@ struct MoverCommon {
   int nB1, nB2;
   double hB1, hB2, Bc0, Bc1, Bc2;
   int deposit;
-  double* np;
+  unsigned long long* acc;      // fixed-point density accumulator (int64 per node)
   double k1dep;                 // Nm/(h1*h2*zmax)
   Tally* tally;
   double* sink;
@@ -69,39 +88,43 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // ---- deposition helpers ----------------------------------------------------
-// Deposit the four corner sums of cell (ix,iy) (corners 00,10,11,01 in the
-// order of part_expmover.f90:571-584).  kq = 2 on non-interior nodes
-// (main.f90:334-344) is applied here or at the flush (a factor 2 is exact).
-template <typename T>
-__device__ __forceinline__ void deposit_cell(const MoverCommon& a, T* win, int wx0, int wy0, int ix,
-                                             int iy, const T w[4], double scale) {
+__device__ __forceinline__ long long fix64(double w) { return __double2ll_rn(w * kFix64); }
+__device__ __forceinline__ int fixmx(float w) { return __float2int_rn(w * kFixMx); }
+__device__ __forceinline__ void acc_add(unsigned long long* p, long long v) {
+  atomicAdd(p, (unsigned long long)v);
+}
+
+// Deposit the four corner sums of FP64 cell (ix,iy) (corners 00,10,11,01 in
+// the order of part_expmover.f90:571-584), as fixed-point integers.  kq = 2
+// on non-interior nodes (main.f90:334-344) is applied by k_np_from_acc (a
+// factor 2 is exact).
+__device__ __forceinline__ void deposit_cell64(const MoverCommon& a, unsigned long long* win, int wx0, int wy0,
+                                               int ix, int iy, const double w[4]) {
+  const long long q[4] = {fix64(w[0]), fix64(w[1]), fix64(w[2]), fix64(w[3])};
   const int cx = ix - wx0, cy = iy - wy0;
   if (cx >= 0 && cx + 1 < kWinCols && cy >= 0 && cy + 1 < kWinRows) {
-    atomicAdd(&win[cy * kWinCols + cx], w[0]);
-    atomicAdd(&win[cy * kWinCols + cx + 1], w[1]);
-    atomicAdd(&win[(cy + 1) * kWinCols + cx + 1], w[2]);
-    atomicAdd(&win[(cy + 1) * kWinCols + cx], w[3]);
+    atomicAdd(&win[cy * kWinCols + cx], (unsigned long long)q[0]);
+    atomicAdd(&win[cy * kWinCols + cx + 1], (unsigned long long)q[1]);
+    atomicAdd(&win[(cy + 1) * kWinCols + cx + 1], (unsigned long long)q[2]);
+    atomicAdd(&win[(cy + 1) * kWinCols + cx], (unsigned long long)q[3]);
   } else {
     const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
-    const int64_t nd[4] = {n00, n00 + 1, n01 + 1, n01};
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      double kq = a.bc[nd[c]] != -1 ? 2.0 : 1.0;
-      atomic_add_f64(&a.np[nd[c]], kq * ((double)w[c] * scale));
-    }
+    acc_add(&a.acc[n00], q[0]);
+    acc_add(&a.acc[n00 + 1], q[1]);
+    acc_add(&a.acc[n01 + 1], q[2]);
+    acc_add(&a.acc[n01], q[3]);
   }
 }
 
+// The window's nonzero nodes into the global accumulator (int64 for FP64
+// windows, int32 sign-extended for mixed ones).
 template <typename T>
-__device__ __forceinline__ void flush_window(const MoverCommon& a, T* win, int wx0, int wy0,
-                                             double scale) {
+__device__ __forceinline__ void flush_window(const MoverCommon& a, const T* win, int wx0, int wy0) {
   for (int e = threadIdx.x; e < kWin; e += blockDim.x) {
-    T v = win[e];
-    if (v != (T)0) {
-      int ix = wx0 + e % kWinCols, iy = wy0 + e / kWinCols;
-      int64_t nd = node_index(ix, iy, a.n1);
-      double kq = a.bc[nd] != -1 ? 2.0 : 1.0;
-      atomic_add_f64(&a.np[nd], kq * ((double)v * scale));
+    const long long v = (long long)win[e];
+    if (v != 0) {
+      const int ix = wx0 + e % kWinCols, iy = wy0 + e / kWinCols;
+      acc_add(&a.acc[node_index(ix, iy, a.n1)], v);
     }
   }
 }
@@ -266,7 +289,7 @@ __device__ __forceinline__ void seg_sum4(double w[4], int s) {
 // (waves whose particles all sit in different cells skip the scan).  Every
 // lane of the wave must call it; ok marks a lane with a particle, key its
 // node index.
-__device__ __forceinline__ void deposit_runs64(const MoverCommon& a, double* win, int wx0, int wy0, bool ok,
+__device__ __forceinline__ void deposit_runs64(const MoverCommon& a, unsigned long long* win, int wx0, int wy0, bool ok,
                                                int key, int ix, int iy, double w[4]) {
   const int lane = threadIdx.x & 63;
   const int kk = ok ? key : -2 - lane;
@@ -276,21 +299,18 @@ __device__ __forceinline__ void deposit_runs64(const MoverCommon& a, double* win
   const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
   if (__popcll(heads & act) != __popcll(act)) seg_sum4(w, s);
   const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-  if (tail && ok) deposit_cell(a, win, wx0, wy0, ix, iy, w, 1.0);
+  if (tail && ok) deposit_cell64(a, win, wx0, wy0, ix, iy, w);
 }
 
-// Weights of one cell straight to the global density (nodes outside the LDS
-// window), with the kq factor of the flush.
+// Mixed weights of one cell straight to the global accumulator (nodes
+// outside the LDS window).
 __device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int iy, float q0, float q1, float q2,
-                                            float q3, double scale) {
+                                            float q3) {
   const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
-  const int64_t nd[4] = {n00, n00 + 1, n01 + 1, n01};
-  const float q[4] = {q0, q1, q2, q3};
-#pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const double kq = a.bc[nd[c]] != -1 ? 2.0 : 1.0;
-    atomic_add_f64(&a.np[nd[c]], kq * ((double)q[c] * scale));
-  }
+  acc_add(&a.acc[n00], fixmx(q0));
+  acc_add(&a.acc[n00 + 1], fixmx(q1));
+  acc_add(&a.acc[n01 + 1], fixmx(q2));
+  acc_add(&a.acc[n01], fixmx(q3));
 }
 
 // CIC deposition of thread-major items (consecutive particles per thread):
@@ -300,7 +320,7 @@ __device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int
 // whole wave are skipped.  Window-local cell indices keep the node
 // arithmetic to adds; cells whose nodes leave the window take the global
 // path.
-__device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win, int wx0, int wy0,
+__device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, int wx0, int wy0,
                                                const int C[kItems], const float FX[kItems],
                                                const float FY[kItems], const bool ok[kItems]) {
   const int lane = threadIdx.x & 63;
@@ -318,7 +338,7 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
     const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
     const int cx = ix - wx0, cy = iy - wy0;
     if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
-      deposit_global(a, ix, iy, q0, q1, q2, q3, a.k1dep);
+      deposit_global(a, ix, iy, q0, q1, q2, q3);
       continue;
     }
     const int kk = cy * kWinCols + cx;
@@ -348,10 +368,10 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, float* win,
     if (__popcll(heads & act) != __popcll(act)) seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
     if (tail && key[k] >= 0) {
-      atomicAdd(&win[kk], v[0]);
-      atomicAdd(&win[kk + 1], v[1]);
-      atomicAdd(&win[kk + kWinCols + 1], v[2]);
-      atomicAdd(&win[kk + kWinCols], v[3]);
+      atomicAdd(&win[kk], fixmx(v[0]));
+      atomicAdd(&win[kk + 1], fixmx(v[1]));
+      atomicAdd(&win[kk + kWinCols + 1], fixmx(v[2]));
+      atomicAdd(&win[kk + kWinCols], fixmx(v[3]));
     }
   }
 }
@@ -398,7 +418,7 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
     float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
     seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-    if (tail && key[k] >= 0) deposit_global(a, key[k] & 0xFFFF, key[k] >> 16, v[0], v[1], v[2], v[3], a.k1dep);
+    if (tail && key[k] >= 0) deposit_global(a, key[k] & 0xFFFF, key[k] >> 16, v[0], v[1], v[2], v[3]);
   }
 }
 
@@ -410,7 +430,7 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
 // sorts); k_mover_mixed chooses it per block from the run count.
 // key[k]: window-local cell, -1 for no item; s_hist: kWin words of LDS.
 template <int BT>
-__device__ __forceinline__ void deposit_sorted(const MoverCommon& a, float* win, int wx0, int wy0,
+__device__ __forceinline__ void deposit_sorted(const MoverCommon& a, int* win, int wx0, int wy0,
                                                const int key[kItems], const float FX[kItems],
                                                const float FY[kItems], uint32_t* s_hist) {
   static_assert(kWin == 4 * BT, "one thread per four window cells");
@@ -498,7 +518,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
                                                     int8_t* flag_dead, const double* __restrict__ E,
                                                     const double* __restrict__ B) {
   constexpr int IT = kChunk / BT;   // particles per thread
-  __shared__ double win[kWin];
+  __shared__ unsigned long long win[kWin];
   __shared__ int s_wx0, s_wy0;
   const int64_t bid = blockIdx.x;
   const int cnt_in = a.seg_in[bid];
@@ -508,7 +528,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
   const double h1 = a.h1, h2 = a.h2, dt = a.dt, k1 = a.k1;
 
   if (a.deposit) {
-    for (int e = t; e < kWin; e += BT) win[e] = 0.0;
+    for (int e = t; e < kWin; e += BT) win[e] = 0ull;
     if (t == 0) {
       int ix0 = 1, iy0 = 1;
       if (cnt_in > 0) { ix0 = (int)(in.x[base] / h1) + 1; iy0 = (int)(in.y[base] / h2) + 1; }
@@ -628,22 +648,23 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
       iy = (int)(Y[k] / h2) + 1;
       double px = (ix * h1 - X[k]) / h1;
       double py = (iy * h2 - Y[k]) / h2;
-      w[0] = a.k1dep * px * py;
-      w[1] = a.k1dep * (1.0 - px) * py;
-      w[2] = a.k1dep * (1.0 - px) * (1.0 - py);
-      w[3] = a.k1dep * px * (1.0 - py);
+      w[0] = px * py;
+      w[1] = (1.0 - px) * py;
+      w[2] = (1.0 - px) * (1.0 - py);
+      w[3] = px * (1.0 - py);
       key = node_index(ix, iy, n1);
     }
     deposit_runs64(a, win, wx0, wy0, surv[k], (int)key, ix, iy, w);
   }
   __syncthreads();
-  flush_window(a, win, wx0, wy0, 1.0);
+  flush_window(a, win, wx0, wy0);
 }
 
 // ---------------------------------------------------------------------------
 // Mixed-precision mover: int32 cell + FP32 in-cell offset, FP32 velocity,
-// FP32 E/B gathers; deposition of FP32 weights into an FP32 LDS window,
-// flushed to the FP64 density with the FP64 factor Nm/(h1 h2 zmax).
+// FP32 E/B gathers; deposition of FP32 run sums into a fixed-point int32 LDS
+// window, flushed to the int64 accumulator (scaled to the FP64 density by
+// k_np_from_acc with the FP64 factor Nm/(h1 h2 zmax)).
 // ---------------------------------------------------------------------------
 struct MixedParams {
   float k1f, dtoh1, dtoh2, dtf;
@@ -655,7 +676,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
                                                         SoaMxW out, int8_t* flag_dead,
                                                         const float* __restrict__ E,
                                                         const float* __restrict__ B) {
-  __shared__ float win[kWin];
+  __shared__ int win[kWin];
   __shared__ uint32_t s_hist[kWin];
   const int64_t bid = blockIdx.x;
   const int cnt_in = a.seg_in[bid];
@@ -667,7 +688,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   const int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
   const int wx0 = max((c0 & 0xFFFF) - kWinX0, 0), wy0 = max((c0 >> 16) - kWinY0, 0);
   if (a.deposit)
-    for (int e = t; e < kWin; e += BT) win[e] = 0.0f;
+    for (int e = t; e < kWin; e += BT) win[e] = 0;
   // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
   // the arrays are padded to whole segments, so the 16-byte loads stay inside
   // them also past the segment's count
@@ -858,19 +879,19 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     deposit_tmajor(a, win, wx0, wy0, C, FX, FY, ok);
   }
   __syncthreads();
-  flush_window(a, win, wx0, wy0, a.k1dep);
+  flush_window(a, win, wx0, wy0);
 }
 
 // ---------------------------------------------------------------------------
 // Deposition only (charge_deposition without a push)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
-  __shared__ double win[kWin];
+  __shared__ unsigned long long win[kWin];
   __shared__ int s_wx0, s_wy0;
   const int cnt_in = a.seg_in[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kChunk;
   const int t = threadIdx.x;
-  for (int e = t; e < kWin; e += kBlock) win[e] = 0.0;
+  for (int e = t; e < kWin; e += kBlock) win[e] = 0ull;
   if (t == 0) {
     int ix0 = 1, iy0 = 1;
     if (cnt_in > 0) { ix0 = (int)(in.x[base] / a.h1) + 1; iy0 = (int)(in.y[base] / a.h2) + 1; }
@@ -893,25 +914,25 @@ __global__ __launch_bounds__(kBlock) void k_deposit64(MoverCommon a, Soa64 in) {
       clamp_cell(ix, iy, a.n1, a.n2);
       double px = (ix * a.h1 - xp) / a.h1;
       double py = (iy * a.h2 - yp) / a.h2;
-      w[0] = a.k1dep * px * py;
-      w[1] = a.k1dep * (1.0 - px) * py;
-      w[2] = a.k1dep * (1.0 - px) * (1.0 - py);
-      w[3] = a.k1dep * px * (1.0 - py);
+      w[0] = px * py;
+      w[1] = (1.0 - px) * py;
+      w[2] = (1.0 - px) * (1.0 - py);
+      w[3] = px * (1.0 - py);
       key = node_index(ix, iy, a.n1);
     }
     deposit_runs64(a, win, wx0, wy0, ok, (int)key, ix, iy, w);
   }
   __syncthreads();
-  flush_window(a, win, wx0, wy0, 1.0);
+  flush_window(a, win, wx0, wy0);
 }
 
 __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx in) {
-  __shared__ float win[kWin];
+  __shared__ int win[kWin];
   __shared__ int s_wx0, s_wy0;
   const int cnt_in = a.seg_in[blockIdx.x];
   const int64_t base = (int64_t)blockIdx.x * kChunk;
   const int t = threadIdx.x;
-  for (int e = t; e < kWin; e += kBlock) win[e] = 0.0f;
+  for (int e = t; e < kWin; e += kBlock) win[e] = 0;
   if (t == 0) {
     int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
     s_wx0 = (c0 & 0xFFFF) - kWinX0 < 0 ? 0 : (c0 & 0xFFFF) - kWinX0;
@@ -934,7 +955,18 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx i
   __syncthreads();
   deposit_tmajor(a, win, s_wx0, s_wy0, C, FX, FY, ok);
   __syncthreads();
-  flush_window(a, win, s_wx0, s_wy0, a.k1dep);
+  flush_window(a, win, s_wx0, s_wy0);
+}
+
+// np = Nm/(h1 h2 zmax) * kq * accumulator (kq = 2 on non-interior nodes,
+// main.f90:334-344); scale = Nm/(h1 h2 zmax) * 2^-shift.
+__global__ __launch_bounds__(256) void k_np_from_acc(const unsigned long long* __restrict__ acc,
+                                                     const int8_t* __restrict__ bc, double* __restrict__ np,
+                                                     int64_t n, double scale) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const double kq = bc[i] != -1 ? 2.0 : 1.0;
+    np[i] = kq * ((double)(long long)acc[i] * scale);
+  }
 }
 
 MoverCommon common_args(lepic_ctx* ctx, int p) {
@@ -951,7 +983,7 @@ MoverCommon common_args(lepic_ctx* ctx, int p) {
   a.magnetize = !(ctx->g.flag_B_pos == 1 && s.charge > 0.0);
   a.nB1 = ctx->nB1; a.nB2 = ctx->nB2; a.hB1 = ctx->hB1; a.hB2 = ctx->hB2;
   a.Bc0 = ctx->Bc[0]; a.Bc1 = ctx->Bc[1]; a.Bc2 = ctx->Bc[2];
-  a.np = s.np;
+  a.acc = (unsigned long long*)s.acc;
   a.k1dep = s.Nm / (ctx->g.h1 * ctx->g.h2 * ctx->g.zmax);  // part_expmover.f90:564
   a.tally = ctx->d_tally + p;
   a.sink = s.sink;
@@ -961,6 +993,18 @@ MoverCommon common_args(lepic_ctx* ctx, int p) {
   a.count = (unsigned long long*)(ctx->d_counts + p);
   a.err = ctx->d_error;
   return a;
+}
+
+// np(:,:,ptype) from the species' fixed-point accumulator
+int np_from_acc(lepic_ctx* ctx, int p) {
+  Species& s = ctx->sp[p];
+  const int shift = ctx->g.precision == LEPIC_MIXED ? kFixShiftMx : kFixShift64;
+  const double scale = std::ldexp(s.Nm / (ctx->g.h1 * ctx->g.h2 * ctx->g.zmax), -shift);
+  const int grid = (int)std::min<int64_t>((ctx->nodes + 255) / 256, 2048);
+  hipLaunchKernelGGL(k_np_from_acc, dim3(grid), dim3(256), 0, ctx->stream, (const unsigned long long*)s.acc,
+                     (const int8_t*)ctx->bc, s.np, ctx->nodes, scale);
+  LEPIC_HIP(hipGetLastError());
+  return 0;
 }
 
 }  // namespace
@@ -974,7 +1018,7 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
   const int64_t nb = std::max<int64_t>(nseg_of(s), 1);  // one block per segment
   if (deposit)
-    if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
+    if (int rc = zero_async(ctx, s.acc, ctx->nodes * 8)) return rc;
   const int b = s.cur, o = 1 - s.cur;
   if (ctx->timing) LEPIC_HIP(hipEventRecord(ctx->ev_mover[p][0], ctx->stream));
   if (mixed) {
@@ -1004,6 +1048,8 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   }
   s.cur = o;
   s.packed = false;
+  if (deposit)
+    if (int rc = np_from_acc(ctx, p)) return rc;
   return 0;
 }
 
@@ -1011,7 +1057,7 @@ int deposit_launch(lepic_ctx* ctx, int p) {
   Species& s = ctx->sp[p];
   MoverCommon a = common_args(ctx, p);
   a.deposit = 1;
-  if (int rc = zero_async(ctx, s.np, ctx->nodes * 8)) return rc;
+  if (int rc = zero_async(ctx, s.acc, ctx->nodes * 8)) return rc;
   const int64_t nb = std::max<int64_t>(nseg_of(s), 1);
   const int b = s.cur;
   if (ctx->g.precision == LEPIC_MIXED) {
@@ -1022,11 +1068,12 @@ int deposit_launch(lepic_ctx* ctx, int p) {
     hipLaunchKernelGGL(k_deposit64, dim3((unsigned)nb), dim3(kBlock), 0, ctx->stream, a, in);
   }
   LEPIC_HIP(hipGetLastError());
-  return 0;
+  return np_from_acc(ctx, p);
 }
 
 // Adds the charge of segments [seg0, seg0+nseg) of the current buffer to
-// np(:,:,ptype) without clearing it: particles appended after the mover
+// the accumulator of the last deposition without clearing it, and rescales
+// np(:,:,ptype) from it: particles appended after the mover
 // (volume injection, part_injection.f90, runs between the push and the
 // deposition, main.f90:1063-1071) join the step's density this way.
 int deposit_range_launch(lepic_ctx* ctx, int p, int64_t seg0, int64_t nseg) {
@@ -1046,7 +1093,7 @@ int deposit_range_launch(lepic_ctx* ctx, int p, int64_t seg0, int64_t nseg) {
     hipLaunchKernelGGL(k_deposit64, dim3((unsigned)nseg), dim3(kBlock), 0, ctx->stream, a, in);
   }
   LEPIC_HIP(hipGetLastError());
-  return 0;
+  return np_from_acc(ctx, p);
 }
 
 }  // namespace lepic
