@@ -1,22 +1,33 @@
-// sort.hip -- cell sort of one species (sorting.f90:92-251): a stable LSD
-// radix sort of the y-major cell index in as many passes of at most 11 bits
-// (2048 buckets) as the key needs -- two below 2^22 cells, three for
-// 2048 x 2048 cells and beyond -- then a gather of the particle payload into
-// the other buffer and, when the caller asks for it, the cell-end list
-// Plist.  The sort reads the species as the mover leaves it, segment by
-// segment: slots past a segment's count carry a key above every cell and
-// sort behind the particles, so the output is packed without a separate
-// packing pass.
+// sort.hip -- cell sort of one species (sorting.f90:92-251), written for
+// particles that are nearly sorted already.
 //
-// With one thread the reference's counting sort is stable (particles keep
-// their order inside a cell); LSD radix sort with stable per-digit scatter
-// gives exactly that permutation.  Per pass: a digit histogram per 16384-key
-// tile (wave-aggregated LDS counts), a device-wide exclusive scan of the
-// digit-major histogram, and a scatter that ranks keys inside the tile in
-// index order.
+// The reference's counting sort with one thread is a stable sort of the
+// particles by the y-major cell index ic.  The species is stored as
+// segments of kChunk slots (see lepic_internal.h); since the last sort the
+// particles of a segment have moved at most a few cells, so a segment
+// covers few distinct cells.  The sort therefore works on runs:
+//  1. k_seg_sort: one block per segment sorts the segment's keys stably in
+//     LDS (LSD passes of 8-bit digits of the key minus the segment's
+//     smallest key, as many as the segment's key span needs) and records,
+//     for every slot, its run (a maximal group of equal keys in the sorted
+//     segment) and its rank inside the run, and for every run its key and
+//     particle count.
+//  2. The runs of all segments, in segment order, are sorted stably by key
+//     (global LSD radix passes of up to 11 bits over the runs only), and an
+//     exclusive scan of their counts in sorted order gives every run its
+//     first output position.
+//  3. k_seg_scatter: one block per segment writes every particle to its
+//     run's position plus its rank, into the other particle buffer.
+// A particle's output position is then the number of particles in lower
+// cells, plus those of its cell in earlier segments, plus those of its cell
+// earlier in its own segment: exactly the stable sort's.  The payload moves
+// once (read and written), the keys are read once.  The cell-end list
+// Plist comes from the sorted run keys and offsets when the caller asks for
+// it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "lepic_internal.h"
 
@@ -26,76 +37,232 @@ namespace {
 
 constexpr int kSortItems = 64;
 constexpr int kTile = 256 * kSortItems;
-constexpr int kMaxRadix = 2048;   // buckets of the widest digit (11 bits)
+constexpr int kMaxRadix = 2048;   // buckets of the widest global digit (11 bits)
 constexpr int kMaxDigitBits = 11;
+constexpr int kSegThreads = 256;  // one block per segment: 4 waves x 4 rounds of 64 slots
+constexpr int kSegRounds = kChunk / kSegThreads;
+constexpr int kRankBits = 10;     // rank inside a run and run index inside a segment: < kChunk
+static_assert(kChunk == (1 << kRankBits), "meta packing assumes 1024-slot segments");
 
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << (threadIdx.x & 63)) - 1ull; }
 
-// Per-wave match of equal digits: returns my rank among earlier lanes with the
-// same digit; the first lane of each digit group adds the group size to cnt.
+// Lanes of the wave holding my digit, matched by one ballot per digit bit
+// (cost independent of how many distinct digits the wave holds).  Returns my
+// rank among the valid earlier lanes with my digit; the first lane of each
+// digit group adds the group size to cnt[d] (the wave's own LDS row).
+template <int NB>
 __device__ __forceinline__ uint32_t wave_match(bool valid, uint32_t d, uint32_t* cnt) {
-  const int lane = threadIdx.x & 63;
-  uint64_t remaining = __ballot(valid);
-  uint32_t rank = 0;
-  while (remaining) {
-    const int leader = __builtin_ctzll(remaining);
-    const uint32_t dl = __shfl(d, leader, 64);
-    const uint64_t mask = __ballot(valid && d == dl) & remaining;
-    if (valid && d == dl) rank = (uint32_t)__popcll(mask & lanemask_lt());
-    if (lane == leader) cnt[dl] += (uint32_t)__popcll(mask);
-    remaining &= ~mask;
+  uint64_t m = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t ones = __ballot(bit);
+    m &= bit ? ones : ~ones;
   }
+  const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+  if (valid && rank == 0) cnt[d] += (uint32_t)__popcll(m);
   return rank;
 }
 
-// keys: 0-based cell index, kEmpty for an empty slot; slot i is a particle
-// when i % kChunk < seg[i / kChunk]
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+template <int R> struct Log2;
+template <> struct Log2<256> { static constexpr int v = 8; };
+template <> struct Log2<512> { static constexpr int v = 9; };
+template <> struct Log2<1024> { static constexpr int v = 10; };
+template <> struct Log2<2048> { static constexpr int v = 11; };
 
-__global__ void k_keys64(const double* __restrict__ x, const double* __restrict__ y, const int32_t* __restrict__ seg,
-                         int64_t nslots, double h1, double h2, int n1, int n2, uint32_t* __restrict__ key) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < nslots && (int)(i & (kChunk - 1)) < seg[i / kChunk];
-  uint32_t k = kEmpty;
-  if (valid) {
-    int ix = (int)(x[i] / h1), iy = (int)(y[i] / h2);  // INT(x/h) (sorting.f90:147-151)
-    ix = ix < 0 ? 0 : (ix > n1 - 1 ? n1 - 1 : ix);
-    iy = iy < 0 ? 0 : (iy > n2 - 1 ? n2 - 1 : iy);
-    k = (uint32_t)ix + (uint32_t)n1 * (uint32_t)iy;
+// ---------------------------------------------------------------------------
+// 1. segment-local stable sort and runs
+// ---------------------------------------------------------------------------
+struct SegSortArgs {
+  const int32_t* seg;         // particles per segment
+  const int32_t* cell;        // mixed: (iy<<16)|ix, 1-based
+  const double* x;            // FP64 positions
+  const double* y;
+  double h1, h2;
+  int n1, n2;
+  uint32_t* meta;             // [slot] (run << kRankBits) | rank in run
+  uint32_t* rkey;             // [segment * kChunk + run] cell key of the run
+  uint32_t* rcnt;             // [segment * kChunk + run] particles of the run
+  uint32_t* nrun;             // [segment] runs of the segment
+};
+
+// y-major 0-based cell index of slot i: INT(x/h) with the clamp of
+// sorting.f90:147-151 (FP64), or the stored cell (mixed)
+template <bool kMixed>
+__device__ __forceinline__ uint32_t cell_key(const SegSortArgs& A, int64_t i) {
+  int ix, iy;
+  if (kMixed) {
+    const int c = A.cell[i];
+    ix = (c & 0xFFFF) - 1;
+    iy = (c >> 16) - 1;
+  } else {
+    ix = (int)(A.x[i] / A.h1);
+    iy = (int)(A.y[i] / A.h2);
   }
-  if (i < nslots) key[i] = k;
+  ix = ix < 0 ? 0 : (ix > A.n1 - 1 ? A.n1 - 1 : ix);
+  iy = iy < 0 ? 0 : (iy > A.n2 - 1 ? A.n2 - 1 : iy);
+  return (uint32_t)ix + (uint32_t)A.n1 * (uint32_t)iy;
 }
 
-__global__ void k_keys_mixed(const int32_t* __restrict__ cell, const int32_t* __restrict__ seg, int64_t nslots,
-                             int n1, int n2, uint32_t* __restrict__ key) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < nslots && (int)(i & (kChunk - 1)) < seg[i / kChunk];
-  uint32_t k = kEmpty;
-  if (valid) {
-    int c = cell[i];
-    int ix = (c & 0xFFFF) - 1, iy = (c >> 16) - 1;
-    ix = ix < 0 ? 0 : (ix > n1 - 1 ? n1 - 1 : ix);
-    iy = iy < 0 ? 0 : (iy > n2 - 1 ? n2 - 1 : iy);
-    k = (uint32_t)ix + (uint32_t)n1 * (uint32_t)iy;
+// exclusive prefix of v over the block's 256 threads in thread order; the
+// block total in *total.  s_tot: 4 words of LDS.  Contains barriers.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tot, uint32_t* total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
   }
-  if (i < nslots) key[i] = k;
+  if (lane == 63) s_tot[w] = inc;
+  __syncthreads();
+  uint32_t pre = inc - v, tot = 0;
+#pragma unroll
+  for (int q = 0; q < kSegThreads / 64; ++q) {
+    const uint32_t c = s_tot[q];
+    if (q < w) pre += c;
+    tot += c;
+  }
+  *total = tot;
+  __syncthreads();
+  return pre;
 }
 
-// Plist from the sorted keys K[0..nslots) (empty slots, kEmpty, at the end):
-// plist[c] = number of particles in cells before c = the first position
-// whose key is at least c, for c = 0..pl_max.  Position i (0..nslots) owns
-// the cells (K[i-1], K[i]], so every entry is written once; past the last
-// particle the rest of the list takes the particle count.
-__global__ void k_plist_from_keys(const uint32_t* __restrict__ K, int64_t nslots, int64_t pl_max,
-                                  uint32_t* __restrict__ plist) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > nslots) return;
-  const int64_t prev = i == 0 ? -1 : (K[i - 1] == kEmpty ? pl_max : (int64_t)K[i - 1]);
-  if (prev >= pl_max) return;
-  const int64_t cur = (i == nslots || K[i] == kEmpty) ? pl_max : std::min((int64_t)K[i], pl_max);
-  for (int64_t c = prev + 1; c <= cur; ++c) plist[c] = (uint32_t)i;
+template <bool kMixed>
+__global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
+  __shared__ uint32_t s_k[2][kChunk];
+  __shared__ uint16_t s_i[2][kChunk];
+  __shared__ uint32_t s_wc[4][256];
+  __shared__ uint16_t s_head[kChunk + 1];
+  __shared__ uint32_t s_mm[2][4];
+  __shared__ uint32_t s_tot[4];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int64_t sg = blockIdx.x;
+  const int cnt = A.seg[sg];
+  const int64_t base = sg * (int64_t)kChunk;
+  if (cnt <= 0) {                       // block-uniform
+    if (t == 0) A.nrun[sg] = 0;
+    return;
+  }
+  // keys in slot order; wave w owns positions [256 w, 256 w + 256)
+  uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
+#pragma unroll
+  for (int r = 0; r < kSegRounds; ++r) {
+    const int q = w * 256 + r * 64 + lane;
+    if (q < cnt) {
+      const uint32_t k = cell_key<kMixed>(A, base + q);
+      s_k[0][q] = k;
+      s_i[0][q] = (uint16_t)q;
+      kmin = min(kmin, k);
+      kmax = max(kmax, k);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, o, 64));
+    kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, o, 64));
+  }
+  if (lane == 0) { s_mm[0][w] = kmin; s_mm[1][w] = kmax; }
+  __syncthreads();
+  kmin = min(min(s_mm[0][0], s_mm[0][1]), min(s_mm[0][2], s_mm[0][3]));
+  kmax = max(max(s_mm[1][0], s_mm[1][1]), max(s_mm[1][2], s_mm[1][3]));
+  const uint32_t span = kmax - kmin;
+  int npass = 0;
+  while (npass < 4 && (span >> (8 * npass)) != 0u) ++npass;
+  // stable LSD passes over the key relative to kmin
+  int cur = 0;
+  for (int p = 0; p < npass; ++p) {
+    const int shift = 8 * p;
+    for (int e = t; e < 4 * 256; e += kSegThreads) (&s_wc[0][0])[e] = 0u;
+    __syncthreads();
+    uint32_t d[kSegRounds];
+    bool v[kSegRounds];
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+      const int q = w * 256 + r * 64 + lane;
+      v[r] = q < cnt;
+      d[r] = v[r] ? ((s_k[cur][q] - kmin) >> shift) & 255u : 0u;
+      wave_match<8>(v[r], d[r], s_wc[w]);
+    }
+    __syncthreads();
+    // digit t: first position of each wave's items with that digit
+    const uint32_t c0 = s_wc[0][t], c1 = s_wc[1][t], c2 = s_wc[2][t], c3 = s_wc[3][t];
+    uint32_t tot;
+    const uint32_t pre = block_excl_scan(c0 + c1 + c2 + c3, s_tot, &tot);
+    s_wc[0][t] = pre;
+    s_wc[1][t] = pre + c0;
+    s_wc[2][t] = pre + c0 + c1;
+    s_wc[3][t] = pre + c0 + c1 + c2;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+      const int q = w * 256 + r * 64 + lane;
+      const uint32_t b = v[r] ? s_wc[w][d[r]] : 0u;   // read before this round's counts are added
+      const uint32_t rk = wave_match<8>(v[r], d[r], s_wc[w]);
+      if (v[r]) {
+        s_k[1 - cur][b + rk] = s_k[cur][q];
+        s_i[1 - cur][b + rk] = s_i[cur][q];
+      }
+    }
+    __syncthreads();
+    cur = 1 - cur;
+  }
+  // runs of equal keys over the sorted positions; thread t owns 4t .. 4t+3
+  uint32_t kq[4];
+  uint32_t heads = 0, nh = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int q = 4 * t + j;
+    kq[j] = q < cnt ? s_k[cur][q] : 0u;
+    const bool hd = q < cnt && (q == 0 || s_k[cur][q - 1] != kq[j]);
+    heads |= hd ? 1u << j : 0u;
+    nh += hd ? 1u : 0u;
+  }
+  uint32_t nrun;
+  const uint32_t rid0 = block_excl_scan(nh, s_tot, &nrun);
+  uint32_t rid[4];
+  {
+    uint32_t r = rid0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if ((heads >> j) & 1u) {
+        s_head[r] = (uint16_t)(4 * t + j);
+        ++r;
+      }
+      rid[j] = r - 1;   // the run of position 4t+j (its head is at or before it)
+    }
+  }
+  if (t == 0) s_head[nrun] = (uint16_t)cnt;
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int q = 4 * t + j;
+    if (q < cnt) A.meta[base + s_i[cur][q]] = (rid[j] << kRankBits) | (uint32_t)(q - s_head[rid[j]]);
+  }
+  for (uint32_t r = t; r < nrun; r += kSegThreads) {
+    const int h = s_head[r];
+    A.rkey[base + r] = s_k[cur][h];
+    A.rcnt[base + r] = (uint32_t)(s_head[r + 1] - h);
+  }
+  if (t == 0) A.nrun[sg] = nrun;
 }
 
+// the runs of every segment into dense arrays, segment order (run q of the
+// species = base[s] + r); base is the exclusive scan of the run counts
+__global__ __launch_bounds__(64) void k_runs_pack(const uint32_t* __restrict__ rkey, const uint32_t* __restrict__ rcnt,
+                                                  const uint32_t* __restrict__ rbase, uint32_t* __restrict__ key,
+                                                  uint32_t* __restrict__ cnt) {
+  const int64_t sg = blockIdx.x;
+  const uint32_t b = rbase[sg], n = min(rbase[sg + 1] - b, (uint32_t)kChunk);
+  for (uint32_t r = threadIdx.x; r < n; r += 64) {
+    key[b + r] = rkey[sg * (int64_t)kChunk + r];
+    cnt[b + r] = rcnt[sg * (int64_t)kChunk + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. global stable radix sort of the runs by key
+// ---------------------------------------------------------------------------
 // Digit counts of a wave's kSortItems x 64 keys from base, added to cnt.  The
 // keys are loaded kLoadBatch per lane at a time, so a wave waits for one
 // memory latency per batch rather than one per 64 keys.
@@ -103,18 +270,19 @@ constexpr int kLoadBatch = 16;
 static_assert(kSortItems % kLoadBatch == 0, "whole load batches per wave");
 
 template <int R>
-__device__ __forceinline__ void count_digits(const uint32_t* __restrict__ key, int64_t base, int64_t nslots,
-                                             int shift, uint32_t* cnt) {
+__device__ __forceinline__ void count_digits(const uint32_t* __restrict__ key, int64_t base, int64_t n, int shift,
+                                             uint32_t* cnt) {
   const int lane = threadIdx.x & 63;
   for (int j0 = 0; j0 < kSortItems; j0 += kLoadBatch) {
     uint32_t d[kLoadBatch];
 #pragma unroll
     for (int u = 0; u < kLoadBatch; ++u) {
       const int64_t i = base + (int64_t)(j0 + u) * 64 + lane;
-      d[u] = i < nslots ? (key[i] >> shift) & (uint32_t)(R - 1) : 0u;
+      d[u] = i < n ? (key[i] >> shift) & (uint32_t)(R - 1) : 0u;
     }
 #pragma unroll
-    for (int u = 0; u < kLoadBatch; ++u) wave_match(base + (int64_t)(j0 + u) * 64 + lane < nslots, d[u], cnt);
+    for (int u = 0; u < kLoadBatch; ++u)
+      wave_match<Log2<R>::v>(base + (int64_t)(j0 + u) * 64 + lane < n, d[u], cnt);
   }
 }
 
@@ -122,14 +290,14 @@ __device__ __forceinline__ void count_digits(const uint32_t* __restrict__ key, i
 // time), so stability across waves is a per-digit prefix over the four
 // waves and no barrier is needed inside the loops.
 template <int R>   // buckets per pass
-__global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, int64_t nslots, int shift,
+__global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, int64_t n, int shift,
                                               uint32_t* __restrict__ hist, int nb) {
   __shared__ uint32_t wc[4][R];
-  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int t = threadIdx.x, w = t >> 6;
   for (int q = t; q < R; q += 256) wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)w * (kTile / 4);
-  count_digits<R>(key, base, nslots, shift, wc[w]);
+  count_digits<R>(key, base, n, shift, wc[w]);
   __syncthreads();
   for (int q = t; q < R; q += 256) hist[(int64_t)q * nb + blockIdx.x] = wc[0][q] + wc[1][q] + wc[2][q] + wc[3][q];
 }
@@ -137,17 +305,14 @@ __global__ __launch_bounds__(256) void k_hist(const uint32_t* __restrict__ key, 
 template <int R>
 __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                  uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                 const uint32_t* __restrict__ hist, int64_t nslots, int shift,
-                                                 int nb) {
+                                                 const uint32_t* __restrict__ hist, int64_t n, int shift, int nb) {
   __shared__ uint32_t wc[4][R];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   for (int q = t; q < R; q += 256) wc[0][q] = wc[1][q] = wc[2][q] = wc[3][q] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)w * (kTile / 4);
-  // the wave's digit counts
-  count_digits<R>(kin, base, nslots, shift, wc[w]);
+  count_digits<R>(kin, base, n, shift, wc[w]);
   __syncthreads();
-  // first output position of each digit for each wave
   for (int q = t; q < R; q += 256) {
     const uint32_t g = hist[(int64_t)q * nb + blockIdx.x];
     const uint32_t c0 = wc[0][q], c1 = wc[1][q], c2 = wc[2][q];
@@ -162,16 +327,16 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ki
 #pragma unroll
     for (int u = 0; u < kLoadBatch; ++u) {
       const int64_t i = base + (int64_t)(j0 + u) * 64 + lane;
-      const bool valid = i < nslots;
+      const bool valid = i < n;
       kk[u] = valid ? kin[i] : 0u;
       vv[u] = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kLoadBatch; ++u) {
-      const bool valid = base + (int64_t)(j0 + u) * 64 + lane < nslots;
+      const bool valid = base + (int64_t)(j0 + u) * 64 + lane < n;
       const uint32_t d = (kk[u] >> shift) & (uint32_t)(R - 1);
       const uint32_t b = valid ? wc[w][d] : 0u;          // read before this round's counts are added
-      const uint32_t r = wave_match(valid, d, wc[w]);
+      const uint32_t r = wave_match<Log2<R>::v>(valid, d, wc[w]);
       if (valid) {
         kout[b + r] = kk[u];
         vout[b + r] = vv[u];
@@ -180,50 +345,109 @@ __global__ __launch_bounds__(256) void k_scatter(const uint32_t* __restrict__ ki
   }
 }
 
-__global__ void k_gather64(const uint32_t* __restrict__ perm, const int64_t* nptr, const double* __restrict__ a0,
-                           const double* __restrict__ a1, const double* __restrict__ a2, const double* __restrict__ a3,
-                           const double* __restrict__ a4, const double* __restrict__ a5, double* b0, double* b1,
-                           double* b2, double* b3, double* b4, double* b5) {
-  const int64_t N = *nptr;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint32_t s = perm[i];
-  b0[i] = a0[s]; b1[i] = a1[s]; b2[i] = a2[s]; b3[i] = a3[s]; b4[i] = a4[s]; b5[i] = a5[s];
+// counts of the runs in sorted order (for the offset scan) and, after the
+// scan, each run's first output position by its dense index
+__global__ void k_runs_sorted_counts(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ cnt,
+                                     uint32_t* __restrict__ out, int64_t M) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M) out[j] = cnt[perm[j]];
 }
 
-__global__ void k_gather_mixed(const uint32_t* __restrict__ perm, const int64_t* nptr,
-                               const int32_t* __restrict__ c0, const float* __restrict__ a0,
-                               const float* __restrict__ a1, const float* __restrict__ a2,
-                               const float* __restrict__ a3, const float* __restrict__ a4,
-                               const float* __restrict__ a5, int32_t* d0, float* b0, float* b1, float* b2,
-                               float* b3, float* b4, float* b5) {
-  const int64_t N = *nptr;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const uint32_t s = perm[i];
-  d0[i] = c0[s]; b0[i] = a0[s]; b1[i] = a1[s];
-  if (b2) b2[i] = a2[s];
-  b3[i] = a3[s]; b4[i] = a4[s]; b5[i] = a5[s];
+__global__ void k_runs_offsets(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ off,
+                               uint32_t* __restrict__ roff, int64_t M) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < M && perm[j] < M) roff[perm[j]] = off[j];
 }
 
-int ensure_sort_scratch(lepic_ctx* ctx, int64_t n, int64_t nb, int64_t ncnt) {
-  if (n > ctx->s_cap) {
-    for (int b = 0; b < 2; ++b) {
-      if (ctx->s_key[b]) LEPIC_HIP(hipFree(ctx->s_key[b]));
-      if (ctx->s_val[b]) LEPIC_HIP(hipFree(ctx->s_val[b]));
-      LEPIC_HIP(hipMalloc(&ctx->s_key[b], n * 4));
-      LEPIC_HIP(hipMalloc(&ctx->s_val[b], n * 4));
+// Plist from the sorted run keys K[0..M) and their first positions off:
+// plist[c] = number of particles in 0-based cells below c, for c = 0..pl_max.
+// Position j (0..M) owns the cells (K[j-1], K[j]], past the last run the
+// rest of the list takes the particle count.
+__global__ void k_plist_from_runs(const uint32_t* __restrict__ K, const uint32_t* __restrict__ off, int64_t M,
+                                  int64_t pl_max, const int64_t* __restrict__ nptr, uint32_t* __restrict__ plist) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > M) return;
+  const int64_t prev = j == 0 ? -1 : (int64_t)K[j - 1];
+  if (prev >= pl_max) return;
+  const int64_t cur = j == M ? pl_max : std::min((int64_t)K[j], pl_max);
+  const uint32_t v = j == M ? (uint32_t)*nptr : off[j];
+  for (int64_t c = prev + 1; c <= cur; ++c) plist[c] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 3. payload scatter
+// ---------------------------------------------------------------------------
+struct SegScatterArgs {
+  const int32_t* seg;
+  const uint32_t* meta;
+  const uint32_t* rbase;      // first dense run of each segment
+  const uint32_t* roff;       // first output position of each dense run
+  const double* a[6];         // FP64 payload in
+  double* b[6];               // FP64 payload out
+  const int32_t* ci;          // mixed payload in: cell + 6 FP32 (z may be null)
+  const float* f[6];
+  int32_t* co;
+  float* g[6];
+  int64_t cap;                // slots of the output buffer
+  int32_t* err;               // set when a destination falls outside it
+};
+
+template <bool kMixed>
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(SegScatterArgs A) {
+  const int64_t sg = blockIdx.x;
+  const int cnt = A.seg[sg];
+  const int64_t base = sg * (int64_t)kChunk;
+  const uint32_t rb = A.rbase[sg];
+  for (int j = threadIdx.x; j < cnt; j += kSegThreads) {
+    const int64_t i = base + j;
+    const uint32_t m = A.meta[i];
+    const int64_t o = (int64_t)A.roff[rb + (m >> kRankBits)] + (m & (kChunk - 1));
+    if (o >= A.cap) {        // cannot happen for consistent runs; never write past the buffer
+      *A.err = 1;
+      continue;
     }
-    ctx->s_cap = n;
+    if (kMixed) {
+      A.co[o] = A.ci[i];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        if (k != 2 || A.g[2]) A.g[k][o] = A.f[k][i];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) A.b[k][o] = A.a[k][i];
+    }
+  }
+}
+
+template <typename T>
+int grow(lepic_ctx* ctx, T** p, int64_t n) {
+  if (*p) LEPIC_HIP(hipFree(*p));
+  *p = nullptr;
+  LEPIC_HIP(hipMalloc(p, std::max<int64_t>(n, 1) * sizeof(T)));
+  return 0;
+}
+
+int ensure_sort_scratch(lepic_ctx* ctx, int64_t nslots, int64_t nseg, int64_t nb, int64_t ncnt) {
+  if (nslots > ctx->s_cap) {
+    for (int b = 0; b < 2; ++b) {
+      if (int rc = grow(ctx, &ctx->s_key[b], nslots)) return rc;
+      if (int rc = grow(ctx, &ctx->s_val[b], nslots)) return rc;
+    }
+    ctx->s_cap = nslots;
+  }
+  if (nslots > ctx->s_seg_cap) {
+    if (int rc = grow(ctx, &ctx->s_meta, nslots)) return rc;
+    if (int rc = grow(ctx, &ctx->s_rkey, nslots)) return rc;
+    if (int rc = grow(ctx, &ctx->s_rcnt, nslots)) return rc;
+    if (int rc = grow(ctx, &ctx->s_rcntd, nslots)) return rc;
+    if (int rc = grow(ctx, &ctx->s_nrun, nslots / kChunk + 2)) return rc;
+    ctx->s_seg_cap = nslots;
   }
   if (kMaxRadix * nb > ctx->s_hist_cap) {
-    if (ctx->s_hist) LEPIC_HIP(hipFree(ctx->s_hist));
-    LEPIC_HIP(hipMalloc(&ctx->s_hist, kMaxRadix * nb * 4));
+    if (int rc = grow(ctx, &ctx->s_hist, kMaxRadix * nb)) return rc;
     ctx->s_hist_cap = kMaxRadix * nb;
   }
   if (ncnt > 0 && (!ctx->d_plist || ncnt > ctx->plist_cap)) {
-    if (ctx->d_plist) LEPIC_HIP(hipFree(ctx->d_plist));
-    LEPIC_HIP(hipMalloc(&ctx->d_plist, ncnt * 4));
+    if (int rc = grow(ctx, (uint32_t**)&ctx->d_plist, ncnt)) return rc;
     ctx->plist_cap = ncnt;
   }
   return 0;
@@ -234,14 +458,14 @@ int ensure_sort_scratch(lepic_ctx* ctx, int64_t n, int64_t nb, int64_t ncnt) {
 namespace lepic {
 
 template <int R>
-int radix_pass(lepic_ctx* ctx, int cur, int pass, int shift, int64_t nslots, int64_t nb) {
-  hipLaunchKernelGGL(k_hist<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur], nslots, shift,
+int radix_pass(lepic_ctx* ctx, int cur, int pass, int shift, int64_t n, int64_t nb) {
+  hipLaunchKernelGGL(k_hist<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur], n, shift,
                      ctx->s_hist, (int)nb);
   LEPIC_HIP(hipGetLastError());
   if (int rc = scan_exclusive_u32(ctx, ctx->s_hist, (int64_t)R * nb)) return rc;
   hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur],
                      pass == 0 ? (const uint32_t*)nullptr : ctx->s_val[cur], ctx->s_key[1 - cur],
-                     ctx->s_val[1 - cur], ctx->s_hist, nslots, shift, (int)nb);
+                     ctx->s_val[1 - cur], ctx->s_hist, n, shift, (int)nb);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -252,58 +476,94 @@ int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
   const int64_t ncell = (int64_t)n1 * n2;
   if (ncell >= (1ll << 31)) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "2^31 cells or more");
   const int64_t pl_max = (int64_t)n1 + ncell + 1;
-  // every slot of every segment; empty slots sort behind the particles
-  const int64_t nslots = nseg_of(s) * kChunk;
-  const int64_t nb = std::max<int64_t>((nslots + kTile - 1) / kTile, 1);
-  if (int rc = ensure_sort_scratch(ctx, std::max<int64_t>(nslots, 1), nb, Plist_host ? pl_max + 1 : 0)) return rc;
+  const int64_t nseg = std::min<int64_t>(nseg_of(s), s.nseg_cap);
+  const int64_t nslots = std::max<int64_t>(nseg, 1) * kChunk;
+  if (int rc = ensure_sort_scratch(ctx, nslots, nseg, std::max<int64_t>((nslots + kTile - 1) / kTile, 1),
+                                   Plist_host ? pl_max + 1 : 0))
+    return rc;
+  const bool mixed = ctx->g.precision == LEPIC_MIXED;
   const int b = s.cur, o = 1 - s.cur;
-  const int64_t* nptr = ctx->d_counts + p;
-  const dim3 gs((unsigned)std::max<int64_t>((nslots + 255) / 256, 1));
-  if (ctx->g.precision == LEPIC_MIXED)
-    hipLaunchKernelGGL(k_keys_mixed, gs, dim3(256), 0, ctx->stream, s.cell[b], (const int32_t*)s.seg[b], nslots,
-                       n1, n2, ctx->s_key[0]);
-  else
-    hipLaunchKernelGGL(k_keys64, gs, dim3(256), 0, ctx->stream, s.d[b][0], s.d[b][1], (const int32_t*)s.seg[b],
-                       nslots, ctx->g.h1, ctx->g.h2, n1, n2, ctx->s_key[0]);
-  LEPIC_HIP(hipGetLastError());
-  // bits of the largest cell index plus the empty-slot key's place above it,
-  // split into the fewest digits of at most 11 bits, widest first
-  int bits = 1;
-  while ((1ll << bits) <= ncell) ++bits;
-  const int npass = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
-  const int lo = (bits + npass - 1) / npass;
-  int cur = 0;
-  for (int pass = 0; pass < npass; ++pass) {
-    const int shift = pass * lo, width = std::min(lo, bits - shift);
-    int rc;
-    if (width <= 8) rc = radix_pass<256>(ctx, cur, pass, shift, nslots, nb);
-    else if (width <= 9) rc = radix_pass<512>(ctx, cur, pass, shift, nslots, nb);
-    else if (width <= 10) rc = radix_pass<1024>(ctx, cur, pass, shift, nslots, nb);
-    else rc = radix_pass<2048>(ctx, cur, pass, shift, nslots, nb);
-    if (rc) return rc;
-    cur = 1 - cur;
+  int64_t M = 0;
+  if (nseg > 0) {
+    // 1. segment-local sorts and their runs
+    SegSortArgs A{};
+    A.seg = s.seg[b];
+    A.cell = s.cell[b];
+    A.x = s.d[b][0];
+    A.y = s.d[b][1];
+    A.h1 = ctx->g.h1; A.h2 = ctx->g.h2; A.n1 = n1; A.n2 = n2;
+    A.meta = ctx->s_meta; A.rkey = ctx->s_rkey; A.rcnt = ctx->s_rcnt; A.nrun = ctx->s_nrun;
+    if (mixed) hipLaunchKernelGGL(k_seg_sort<true>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, A);
+    else hipLaunchKernelGGL(k_seg_sort<false>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, A);
+    LEPIC_HIP(hipGetLastError());
+    // run bases (exclusive scan over the segments; entry nseg = M)
+    if (int rc = zero_async(ctx, ctx->s_nrun + nseg, 4)) return rc;
+    if (int rc = scan_exclusive_u32(ctx, ctx->s_nrun, nseg + 1)) return rc;
+    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->s_nrun + nseg, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    uint32_t m32;
+    std::memcpy(&m32, ctx->h_pinned, 4);
+    M = m32;
+    hipLaunchKernelGGL(k_runs_pack, dim3((unsigned)nseg), dim3(64), 0, ctx->stream, (const uint32_t*)ctx->s_rkey,
+                       (const uint32_t*)ctx->s_rcnt, (const uint32_t*)ctx->s_nrun, ctx->s_key[0], ctx->s_rcntd);
+    LEPIC_HIP(hipGetLastError());
   }
-  int64_t N = 0;
-  if (int rc = lepic_get_count(ctx, p + 1, &N)) return rc;
-  const dim3 g1((unsigned)std::max<int64_t>((N + 255) / 256, 1));
-  const uint32_t* perm = ctx->s_val[cur];
-  if (ctx->g.precision == LEPIC_MIXED)
-    hipLaunchKernelGGL(k_gather_mixed, g1, dim3(256), 0, ctx->stream, perm, nptr, s.cell[b], s.f[b][0],
-                       s.f[b][1], s.f[b][2], s.f[b][3], s.f[b][4], s.f[b][5], s.cell[o], s.f[o][0],
-                       s.f[o][1], s.f[o][2], s.f[o][3], s.f[o][4], s.f[o][5]);
-  else
-    hipLaunchKernelGGL(k_gather64, g1, dim3(256), 0, ctx->stream, perm, nptr, s.d[b][0], s.d[b][1],
-                       s.d[b][2], s.d[b][3], s.d[b][4], s.d[b][5], s.d[o][0], s.d[o][1], s.d[o][2],
-                       s.d[o][3], s.d[o][4], s.d[o][5]);
-  LEPIC_HIP(hipGetLastError());
+  // 2. the runs sorted by key (stable), in the fewest digits of at most 11 bits
+  int cur = 0;
+  if (M > 0) {
+    const int64_t nb = (M + kTile - 1) / kTile;
+    int bits = 1;
+    while ((1ll << bits) < ncell) ++bits;
+    const int npass = (bits + kMaxDigitBits - 1) / kMaxDigitBits;
+    const int lo = (bits + npass - 1) / npass;
+    for (int pass = 0; pass < npass; ++pass) {
+      const int shift = pass * lo, width = std::min(lo, bits - shift);
+      int rc;
+      if (width <= 8) rc = radix_pass<256>(ctx, cur, pass, shift, M, nb);
+      else if (width <= 9) rc = radix_pass<512>(ctx, cur, pass, shift, M, nb);
+      else if (width <= 10) rc = radix_pass<1024>(ctx, cur, pass, shift, M, nb);
+      else rc = radix_pass<2048>(ctx, cur, pass, shift, M, nb);
+      if (rc) return rc;
+      cur = 1 - cur;
+    }
+    // first output position of every run: scan of the counts in sorted order
+    const uint32_t* perm = ctx->s_val[cur];
+    uint32_t* off = ctx->s_val[1 - cur];
+    const dim3 gm((unsigned)((M + 255) / 256));
+    hipLaunchKernelGGL(k_runs_sorted_counts, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)ctx->s_rcntd, off,
+                       M);
+    LEPIC_HIP(hipGetLastError());
+    if (int rc = scan_exclusive_u32(ctx, off, M)) return rc;
+    hipLaunchKernelGGL(k_runs_offsets, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)off, ctx->s_rkey, M);
+    LEPIC_HIP(hipGetLastError());
+    // 3. the payload, once
+    SegScatterArgs S{};
+    S.seg = s.seg[b]; S.meta = ctx->s_meta; S.rbase = ctx->s_nrun; S.roff = ctx->s_rkey;
+    for (int k = 0; k < 6; ++k) {
+      S.a[k] = s.d[b][k]; S.b[k] = s.d[o][k];
+      S.f[k] = s.f[b][k]; S.g[k] = s.f[o][k];
+    }
+    S.ci = s.cell[b]; S.co = s.cell[o];
+    S.cap = (s.nseg_cap - 1) * kChunk;   // alloc_species: nseg_cap = slots / kChunk + 1
+    S.err = ctx->d_error + 12;
+    if (mixed) hipLaunchKernelGGL(k_seg_scatter<true>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, S);
+    else hipLaunchKernelGGL(k_seg_scatter<false>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, S);
+    LEPIC_HIP(hipGetLastError());
+  }
   s.cur = o;
   if (int rc = set_packed_counts(ctx, p)) return rc;
+  if (M > 0) {
+    int32_t err = 0;
+    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->d_error + 12, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(&err, ctx->h_pinned, 4);
+    if (err) return ctx->fail(LEPIC_ERR_STATE, "cell sort: inconsistent runs");
+  }
   if (Plist_host) {
-    // Plist(c) = number of particles in cells 1..c (1-based), i.e. in the
-    // 0-based cells below c: the lower bound of c in the sorted keys
     uint32_t* plist = (uint32_t*)ctx->d_plist;
-    hipLaunchKernelGGL(k_plist_from_keys, dim3((unsigned)((nslots + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
-                       ctx->s_key[cur], nslots, pl_max, plist);
+    hipLaunchKernelGGL(k_plist_from_runs, dim3((unsigned)((M + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (const uint32_t*)ctx->s_key[cur], (const uint32_t*)ctx->s_val[1 - cur], M, pl_max,
+                       (const int64_t*)(ctx->d_counts + p), plist);
     LEPIC_HIP(hipGetLastError());
     LEPIC_HIP(hipMemcpyAsync(Plist_host, plist, (pl_max + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     LEPIC_HIP(hipStreamSynchronize(ctx->stream));
