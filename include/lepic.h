@@ -129,6 +129,11 @@ int lepic_set_bfield(lepic_ctx *ctx, int32_t nB1, int32_t nB2, double hB1, doubl
 int lepic_upload_particles(lepic_ctx *ctx, int32_t ptype, const double *vxp, int64_t np);
 int lepic_download_particles(lepic_ctx *ctx, int32_t ptype, double *vxp, int64_t cap,
                              int64_t *np);
+/* particles first .. first+count-1 (0-based, the species' order) only, e.g.
+ * a sample of a large run; no counterpart routine in the reference, whose
+ * diagnostics read vxp(:,i) in place */
+int lepic_download_particles_range(lepic_ctx *ctx, int32_t ptype, int64_t first, int64_t count,
+                                   double *vxp);
 int lepic_get_count(lepic_ctx *ctx, int32_t ptype, int64_t *np);
 /* flag_dead(nmax,ptype) as int8, for the first np particles */
 int lepic_set_flag_dead(lepic_ctx *ctx, int32_t ptype, const int8_t *flags, int64_t np);

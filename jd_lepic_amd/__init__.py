@@ -142,6 +142,12 @@ class Context:
                                                       C.byref(got)))
         return out[:got.value]
 
+    def download_range(self, ptype, first, count):
+        """Particles first .. first+count-1 of the species, (count, 6)."""
+        out = np.zeros((max(count, 1), 6))
+        self._check(self.lib.lepic_download_particles_range(self._ctx, ptype, first, count, _dp(out)))
+        return out[:count]
+
     # ---- averaged moments (part_moments / calc_avg) ----
     def moments_accumulate(self, size_na=1):
         self._check(self.lib.lepic_moments_accumulate(self._ctx, size_na))

@@ -139,6 +139,7 @@ SIGNATURES = {
     "lepic_set_bfield": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_double, C.c_double, _dp]),
     "lepic_upload_particles": (C.c_int, [_ctxp, C.c_int32, _dp, C.c_int64]),
     "lepic_download_particles": (C.c_int, [_ctxp, C.c_int32, _dp, C.c_int64, C.POINTER(C.c_int64)]),
+    "lepic_download_particles_range": (C.c_int, [_ctxp, C.c_int32, C.c_int64, C.c_int64, _dp]),
     "lepic_get_count": (C.c_int, [_ctxp, C.c_int32, C.POINTER(C.c_int64)]),
     "lepic_set_flag_dead": (C.c_int, [_ctxp, C.c_int32, _bp, C.c_int64]),
     "lepic_get_flag_dead": (C.c_int, [_ctxp, C.c_int32, _bp, C.c_int64]),

@@ -174,19 +174,26 @@ extern "C" int lepic_step(lepic_ctx* ctx, int32_t it, lepic_step_args* a) {
     rc = lepic_eheating(ctx, a->heat, a->heat_mode, a->seed, it, &a->iseed, &a->P_heat);
   LEPIC_HIP(hipEventRecord(ev[3], ctx->stream));
   const bool inject = a->inj && a->inj->ns_inj > 0 && it % a->inj->ns_inj == 0;
+  // opt_inj = +-2 re-injects the wall losses of every push since the last
+  // injection: sav_np before and after each push (main.f90:1044-1056)
+  const bool count_losses = a->inj && (a->inj->opt_inj == 2 || a->inj->opt_inj == -2);
   double lost0[kMaxSpecies] = {};
-  if (!rc && inject) rc = lost_counts(ctx, lost0);   // sav_np, main.f90:1044
+  if (!rc && count_losses) rc = lost_counts(ctx, lost0);
   for (int p = 1; p <= ctx->nsp && !rc; ++p) rc = lepic_part_mover(ctx, p, 1);  // :1041-1086
-  a->injected = 0;
-  if (!rc && inject) {  // :1063-1071
+  if (!rc && count_losses) {
     double lost1[kMaxSpecies] = {};
     rc = lost_counts(ctx, lost1);
+    for (int q = 0; q < ctx->nsp; ++q) ctx->inj_lost[q] += lost1[q] - lost0[q];
+  }
+  a->injected = 0;
+  if (!rc && inject) {  // :1063-1071
     int64_t N_inj[kMaxSpecies] = {};
-    for (int q = 0; q < ctx->nsp; ++q) N_inj[q] = (int64_t)(lost1[q] - lost0[q]);
+    for (int q = 0; q < ctx->nsp; ++q) N_inj[q] = (int64_t)ctx->inj_lost[q];
     lepic_injection_info info;
-    if (!rc) rc = lepic_part_injection(ctx, a->inj, N_inj, it, a->heat_mode, a->seed, &a->iseed, 1, &info);
+    rc = lepic_part_injection(ctx, a->inj, N_inj, it, a->heat_mode, a->seed, &a->iseed, 1, &info);
     if (!rc)
       for (int q = 0; q < ctx->nsp; ++q) a->injected += info.injected[q];
+    for (int q = 0; q < ctx->nsp; ++q) ctx->inj_lost[q] = 0.0;   // "Reset counter", :1070
   }
   a->emitted = 0;
   if (!rc && a->emit && a->emit->jne > 0.0 && (a->emit->ns_Hm == 1 || it % a->emit->ns_Hm == 1)) {  // :1092-1096

@@ -246,3 +246,27 @@ def test_step_reinjects_losses():
     assert r["injected"] > 0
     lost_ions = int(round(lost["p_mac"][0].sum()))
     assert lost_ions > 0 and n2 == 4000            # every lost ion came back
+
+
+def test_step_reinjects_losses_of_skipped_steps():
+    """opt_inj = 2 with ns_inj = 2: the ions lost in the push of a step without
+    injection are re-injected at the next injection step together with that
+    step's own losses (N_inj accumulates, main.f90:1044-1070)."""
+    inj = dict(injection(2), ns_inj=2)
+    res = {p: residents(4000, 60 + p) for p in (1, 2, 3)}
+    rng = np.random.default_rng(6)
+    res[2][:, 0] = rng.uniform(0.01 * H, 0.2 * H, 4000)
+    res[2][:2000, 3] = -3e6
+    with context(L.FP64, res) as ctx:
+        ctx.charge_deposition(1)
+        ctx.charge_deposition(2)
+        ctx.reset_tallies()
+        r1 = ctx.step(1, mcc_mode=-1, nsort=0, inj=gpu_args(inj))
+        n_mid = ctx.count(2)
+        lost1 = int(round(ctx.mover_tally(2, ngrid=2)["p_mac"][0].sum()))
+        r2 = ctx.step(2, mcc_mode=-1, nsort=0, inj=gpu_args(inj))
+        lost2 = int(round(ctx.mover_tally(2, ngrid=2)["p_mac"][0].sum()))
+        n_end = ctx.count(2)
+    assert r1["injected"] == 0 and lost1 > 0 and n_mid == 4000 - lost1
+    assert r2["injected"] > 0
+    assert n_end == 4000, (lost1, lost2, n_end)     # losses of both steps came back
