@@ -1,22 +1,25 @@
 """Benchmark of the per-timestep hot path on one MI355X.
 
-Workload (one GPU): 512 x 512 cells, dx = dy = 5e-5 m, grounded box, e- and
-Ar+ (39.95 u) at 100 macroparticles per cell each (52,428,800 particles),
-n0 = 1e16 m^-3, Maxwellian Te = 3 eV and Ti = 0.03 eV, Bz(x) Gaussian filter
-of 10 mT peak at L/2 and width L/8, background Ar at 0.3 Pa / 300 K with
-analytic cross sections (constant elastic 1e-19 m^2, excitation at 11.5 eV,
-Lotz ionization at 15.76 eV), dt = 5e-12 s, MG Poisson (9 levels) to 1e-10,
-sort and null-collision MCC every 10 steps.  Synthetic inputs from the
-device loader (counter-based RNG, recorded seed).
+Workload (one GPU, default): 1024 x 1024 cells, dx = dy = 5e-5 m, grounded
+box, e- and Ar+ (39.95 u) at 100 macroparticles per cell each (209,715,200
+particles), n0 = 1e16 m^-3, Maxwellian Te = 3 eV and Ti = 0.03 eV, Bz(x)
+Gaussian filter of 10 mT peak at L/2 and width L/8, background Ar at 0.3 Pa /
+300 K with analytic cross sections (constant elastic 1e-19 m^2, excitation at
+11.5 eV, Lotz ionization at 15.76 eV), dt = 5e-12 s, MG Poisson (10 levels)
+to 1e-10, sort and null-collision MCC every 10 steps.  Synthetic inputs from
+the device loader (counter-based RNG, recorded seed).  Other workloads:
+--workload argon-128 / argon-512 / ion-source / argon-2048.
 
-One step = rhs -> Poisson -> E -> [sort] -> [MCC] -> push+deposit of both
+One step = rhs -> Poisson -> E -> [sort] -> [MCC] -> push+deposit of every
 species (main.f90:689-1090 order).  The headline value is particles/s of
 push + deposit + MCC in the mixed-precision form with the learned warm start
-(FP32 push/deposit with FP64 grid sums, FP32 MG smoothing in FP64
-refinement, Poisson started from the network's guess when it lowers the
-initial residual; the network is trained on the run's own first solves,
-untimed); plain mixed and plain FP64 are measured on the same workload
-beside it.
+(FP32 push/deposit with fixed-point tile sums flushed to the FP64 grid, FP32
+MG smoothing in FP64 refinement, Poisson started from the network's guess
+when it lowers the initial residual; the network is trained on the run's own
+first solves, untimed); plain mixed and plain FP64 are measured on the same
+workload beside it.  Untimed beside the loop: the mixed form's densities,
+phi, E and moments against the FP64 oracle on the same state, and mixed
+against FP64 after ten steps.
 
 Prints one JSON line.
 """
@@ -35,14 +38,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import jd_lepic_amd as L  # noqa: E402
-from jd_lepic_amd import reactions as RX, tables  # noqa: E402
+from jd_lepic_amd import workloads as WL  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per particle of one mixed mover launch at this workload, measured
-# with rocprofv3 PMC counters (FETCH_SIZE and WRITE_SIZE, each scaled by what
-# it reads for a 1 GiB device copy): 51.4 B against 48 algorithmic bytes
-# (profiles/mover_mixed_traffic_512x512_100ppc.md)
-TRAFFIC_BYTES_PER_PARTICLE = 51.4
 
 
 class Hip:
@@ -87,107 +85,6 @@ class Hip:
         return nbytes
 
 
-def workload(n=512, ppc=100):
-    """e-/Ar+ in a grounded n x n box at ppc per cell each, Bz filter field."""
-    h = 5.0e-5
-    L1 = n * h
-    x = (np.arange(n + 3) - 1) * h
-    Bz = 0.01 * np.exp(-(x - L1 / 2) ** 2 / (2 * (L1 / 8) ** 2))   # Bfield.f90:54-57
-    Bi = np.zeros((n + 3, n + 3, 4))
-    Bi[..., 2] = Bz[None, :]
-    Bi[..., 3] = np.abs(Bi[..., 2])
-    bcnd = -np.ones((n + 3, n + 3), np.int32)
-    bcnd[:2, :] = bcnd[n + 1:, :] = bcnd[:, :2] = bcnd[:, n + 1:] = 1
-    ngas = 0.3 / (1.380649e-23 * 300.0)
-    chem = RX.argon(ngas)
-    n0, Vc = 1.0e16, h * h * 1.0
-    Nm = n0 * Vc / ppc
-    vte = np.sqrt(2.0 * L.QE * 3.0 / L.ME)
-    vti = np.sqrt(2.0 * L.QE * 0.03 / (39.95 * L.AMU))
-    vtg = np.sqrt(2.0 * L.QE * 0.02585 / (39.95 * L.AMU))
-    species = [dict(name="e", charge=-L.QE, mass=L.ME, Nm=Nm, vt=vte, loads=[(ppc, 0.0, L1)]),
-               dict(name="i", charge=L.QE, mass=39.95 * L.AMU, Nm=Nm, vt=vti, loads=[(ppc, 0.0, L1)])]
-    return dict(kind="e-Ar", n=n, n1=n, n2=n, h=h, Bi=Bi, bcnd=bcnd, chem=chem, Nm=Nm, ppc=ppc, vte=vte,
-                vti=vti, vtg=vtg, dt=5.0e-12, ng=int(np.log2(n)), ngas=ngas, species=species,
-                vt0=[vte, vti, vtg], ngrid=1, tag_neg=0,
-                desc=f"{n}x{n} cells, e-/Ar+ {ppc} ppc each ({2 * n * n * ppc} particles), B filter 10 mT, "
-                     "Ar 0.3 Pa analytic MCC, dt 5e-12, MG 1e-10")
-
-
-def workload_ion_source(n1=1024, n2=512, ppc=124, strip=16):
-    """Negative-ion-source-like domain: e-, H+, H2+, H- over background H2
-    (analytic elastic, dissociation and ionization cross sections); density
-    1e17 m^-3 over the driver half falling exponentially to 1e15 at the
-    extraction wall (x = L), loaded in strips of `strip` columns with the
-    strip's particles per cell; species fractions e- 0.9, H+ 0.5, H2+ 0.5,
-    H- 0.1 of that density (quasi-neutral), one weight Nm for all, ppc the
-    particles per cell of a species at unit fraction where the density is
-    1e17; grounded box (label 1), a grid electrode of bars a few cells before
-    the extraction wall (label 2), the extraction wall itself (label 3); Bz
-    filter field of 10 mT at L/2."""
-    h = 5.0e-5
-    L1 = n1 * h
-    bcnd = -np.ones((n2 + 3, n1 + 3), np.int32)
-    bcnd[:2, :] = bcnd[n2 + 1:, :] = bcnd[:, :2] = 1
-    bcnd[:, n1 + 1:] = 3
-    c0 = n1 - 11 if (n1 - 11) % 2 == 1 else n1 - 10
-    for r0 in range(5, n2 - 3, 8):
-        bcnd[r0:r0 + 5, c0:c0 + 3] = 2
-    x = (np.arange(n1 + 3) - 1) * h
-    Bi = np.zeros((n2 + 3, n1 + 3, 4))
-    Bi[..., 2] = 0.01 * np.exp(-(x - L1 / 2) ** 2 / (2 * (L1 / 8) ** 2))[None, :]
-    Bi[..., 3] = np.abs(Bi[..., 2])
-    n_hi, n_lo = 1.0e17, 1.0e15
-    lam = (L1 / 2) / np.log(n_hi / n_lo)
-
-    def density(xm):
-        return n_hi if xm < L1 / 2 else n_hi * np.exp(-(xm - L1 / 2) / lam)
-
-    Nm = n_hi * h * h * 1.0 / ppc
-    ngas = 0.3 / (1.380649e-23 * 300.0)
-    chem = RX.hydrogen(ngas)
-    temps = {"e": 2.0, "Hp": 0.1, "H2p": 0.1, "Hm": 0.1}
-    masses = {"e": L.ME, "Hp": 1.00794 * L.AMU, "H2p": 2.01588 * L.AMU, "Hm": 1.00794 * L.AMU}
-    charges = {"e": -L.QE, "Hp": L.QE, "H2p": L.QE, "Hm": -L.QE}
-    frac = {"e": 0.9, "Hp": 0.5, "H2p": 0.5, "Hm": 0.1}
-    species, total = [], 0
-    for name in ("e", "Hp", "H2p", "Hm"):
-        loads = []
-        for c in range(0, n1, strip):
-            k = int(round(ppc * frac[name] * density((c + strip / 2) * h) / n_hi))
-            if k > 0:
-                loads.append((k, c * h, (c + strip) * h))
-                total += k * strip * n2
-        species.append(dict(name=name, charge=charges[name], mass=masses[name], Nm=Nm,
-                            vt=np.sqrt(2.0 * L.QE * temps[name] / masses[name]), loads=loads))
-    vt0 = [np.sqrt(2.0 * L.QE * (2.0 if k == 0 else sp.T) / abs(sp.mass)) for k, sp in enumerate(chem.species)]
-    return dict(kind="ion-source", n=n1, n1=n1, n2=n2, h=h, Bi=Bi, bcnd=bcnd, chem=chem, Nm=Nm, ppc=ppc,
-                dt=5.0e-12, ng=int(np.log2(min(n1, n2))), ngas=ngas, species=species, vt0=vt0, ngrid=3,
-                tag_neg=4,
-                desc=f"{n1}x{n2} cells, e-/H+/H2+/H- (about {total} particles), density 1e17 m^-3 over the "
-                     "driver half falling exponentially to 1e15 at the extraction wall, grid electrode, "
-                     "B filter 10 mT, H2 0.3 Pa analytic MCC, dt 5e-12, MG 1e-10")
-
-
-def build_ctx(W, precision, seed=20261202):
-    n1, n2 = W["n1"], W["n2"]
-    sp = []
-    for s in W["species"]:
-        nload = sum(k * n2 * int(round((xr - xl) / W["h"])) for k, xl, xr in s["loads"])
-        sp.append(L.Species(s["charge"], s["mass"], s["Nm"], int(nload * 1.05) + 4096))
-    ctx = L.Context(n1, n2, W["h"], W["h"], W["dt"], sp, ng=W["ng"], ngrid=W["ngrid"], precision=precision,
-                    track_z=0, tag_neg=W["tag_neg"])
-    ctx.set_boundary(W["bcnd"], np.zeros((n2 + 3, n1 + 3)))
-    ctx.set_bfield(W["Bi"], (n1, n2), (W["h"], W["h"]))
-    for p, s in enumerate(W["species"], start=1):
-        for k, xl, xr in s["loads"]:
-            ctx.load_uniform(p, k, s["vt"], xl, xr, seed)
-    ctx.set_reactions(W["chem"], tables.ordering(W["chem"]), vt0=W["vt0"])
-    for p in range(1, len(W["species"]) + 1):
-        ctx.charge_deposition(p)
-    return ctx
-
-
 def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_steps=40, train_iters=300):
     """One mode of the step loop: learn_steps untimed steps, warmup untimed
     steps, then `steps` timed steps, so every mode times the same steps of
@@ -196,7 +93,7 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
     and every later solve starts from the network's guess when that lowers
     the initial residual (the library skips the network for a growing
     back-off after a guess that gains too little)."""
-    ctx = build_ctx(W, precision)
+    ctx = WL.build_context(W, precision)
     ctx.kernel_timing(True)      # mover kernels bracketed by events inside the library
     stream = C.c_void_p()
     ctx.lib.lepic_stream(ctx._ctx, C.byref(stream))
@@ -300,7 +197,7 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
     n1, n2 = W["n1"], W["n2"]
     names = [sp["name"] for sp in W["species"]]
     nsp = len(names)
-    ctx = {p: build_ctx(W, p) for p in (L.FP64, L.MIXED)}
+    ctx = {p: WL.build_context(W, p) for p in (L.FP64, L.MIXED)}
     out, rhs_seq = {}, []
     stock0 = {}
     for c in ctx.values():
@@ -394,111 +291,121 @@ def accuracy_and_sor(W, steps=10, sor_solves=3):
     return out, sor
 
 
-def cpu_baseline(W, budget_s=10.0):
-    """The oracle (scalar C restatement, one core) on a bounded sample of the
-    same workload: 64 x 64 cells of it at 100 ppc for both species."""
+def host_threads():
+    """Host cores this process may use (capped at the 16 a one-GPU share has)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(W, budget_s=15.0):
+    """The oracle's C restatement of part_mover + charge_deposition and of
+    collisions, on the host cores with OpenMP, partitioned the way the
+    reference's main loop runs them (one partition per thread with its own
+    density array, main.f90:1024-1089; collision_OMP per thread), on a
+    bounded sample of the workload: a 128 x 128-cell box of the same plasma
+    at the workload's particles per cell for both species, push + deposit
+    every step and the MCC every 10 steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    n, ppc, h = 64, W["ppc"], W["h"]
+    nth = host_threads()
+    Ws = WL.argon_box(128, W.get("ppc", 100))
+    n, h = Ws["n1"], Ws["h"]
+    dom = O.Domain(n1=n, n2=n, h1=h, h2=h, dt=Ws["dt"], bcnd=Ws["bcnd"], Bi=Ws["Bi"], nB=(n, n), hB=(h, h))
     rng = np.random.default_rng(3)
-    N = n * n * ppc
+    N = n * n * Ws["ppc"]
     pos = np.column_stack([rng.uniform(0, n * h, N), rng.uniform(0, n * h, N), np.zeros(N)])
-    ve = rng.normal(0, W["vte"] / np.sqrt(2), (N, 3))
-    vi = rng.normal(0, W["vti"] / np.sqrt(2), (N, 3))
-    bc = O.box_bcnd(n, n)
-    x = (np.arange(n + 3) - 1) * h
-    Bi = np.zeros((n + 3, n + 3, 4))
-    Bi[..., 2] = 0.01 * np.exp(-(x - n * h / 2) ** 2 / (2 * (n * h / 8) ** 2))[None, :]
-    Bi[..., 3] = np.abs(Bi[..., 2])
-    dom = O.Domain(n1=n, n2=n, h1=h, h2=h, dt=W["dt"], bcnd=bc, Bi=Bi, nB=(n, n), hB=(h, h))
+    e = np.column_stack([pos, rng.normal(0, Ws["vte"] / np.sqrt(2), (N, 3))])
+    i = np.column_stack([pos, rng.normal(0, Ws["vti"] / np.sqrt(2), (N, 3))])
     Ei = np.random.default_rng(4).normal(0, 1e3, (n + 3, n + 3, 2))
-    chem = W["chem"]
+    M = 39.95 * L.AMU
+    probe = O.time_push_deposit_omp(dom, -L.QE, L.ME, Ws["Nm"], Ei, e, nth, 1) + \
+        O.time_push_deposit_omp(dom, L.QE, M, Ws["Nm"], Ei, i, nth, 1)
+    reps = max(2, int(0.8 * budget_s / max(probe, 1e-6)))
+    t_push = O.time_push_deposit_omp(dom, -L.QE, L.ME, Ws["Nm"], Ei, e, nth, reps) + \
+        O.time_push_deposit_omp(dom, L.QE, M, Ws["Nm"], Ei, i, nth, reps)
+    chem = Ws["chem"]
     tab = O.ordering([dict(col_info=r.col_info(), Eth=r.Eth, E=r.E, S=r.S) for r in chem.reactions],
                      chem.masses(), chem.ntype_all)
-    setup = dict(ntype=2, ntype_all=3, ns_coll=10, plt_src=0, n_cell=n * n, n_rg=1, ixl_rg=[1],
-                 ixr_rg=[n + 1], iyl_rg=1, iyr_rg=n + 1, flag_ionrep=0,
-                 charge={1: -L.QE, 2: L.QE, 3: 0.0}, mass=chem.masses(),
-                 Nm={1: W["Nm"], 2: W["Nm"], 3: W["Nm"]},
-                 vt0={1: W["vte"], 2: W["vti"], 3: W["vtg"]},
-                 np_mx={1: 0.0, 2: 0.0, 3: W["ngas"]}, nu_uplim={1: 6e7, 2: 5e6, 3: 5e6})
-    harness = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    if os.path.exists(harness):
-        return cpu_baseline_reference(harness, W, dom, Ei, pos, ve, vi, N, O, setup, tab, budget_s)
-    t_push = t_mcc = 0.0
-    reps = 0
-    start = time.perf_counter()
-    while time.perf_counter() - start < budget_s or reps < 1:
-        e = np.column_stack([pos, ve])
-        i = np.column_stack([pos, vi])
-        t_push += O.time_push_deposit(dom, -L.QE, L.ME, W["Nm"], Ei, e)
-        t_push += O.time_push_deposit(dom, L.QE, 39.95 * L.AMU, W["Nm"], Ei, i)
-        cap = N + N // 10
-        pe = np.zeros((cap, 6)); pe[:N] = e
-        pi_ = np.zeros((cap, 6)); pi_[:N] = i
-        parts = {1: dict(vxp=pe, np_tot=N, flag_dead=np.zeros(cap, np.int8)),
-                 2: dict(vxp=pi_, np_tot=N, flag_dead=np.zeros(cap, np.int8))}
-        np_red = [np.full((n + 3, n + 3), 1e16), np.full((n + 3, n + 3), 1e16)]
-        t = time.perf_counter()
-        O.collisions(dom, setup, tab, parts, np_red, 123456)
-        t_mcc += time.perf_counter() - t
-        reps += 1
-    per_step = (t_push + t_mcc / 10.0) / reps     # MCC runs every 10 steps
-    return dict(value=2 * N / per_step, unit="particles/s", cores=1, kind="port",
-                sample=f"oracle C restatement, 1 core: {n}x{n} cells x 100 ppc x 2 species "
-                       f"({2 * N} particles), push+deposit each step + MCC/10, {reps} reps")
-
-
-def _ref_push_deposit(harness, W, dom, Ei, vxp, charge, mass, reps, O):
-    """Seconds of reps x (part_mover + charge_deposition) of the reference's
-    own Fortran (part_expmover.f90, compiled by oracle/Makefile), one core."""
-    import subprocess
-    import tempfile
-    n, h = dom.n1, dom.h1
-    N = vxp.shape[0]
-    ints = [n, n, dom.nB[0], dom.nB[1], N, dom.ngrid, 1, dom.flag_gridB, dom.flag_B_pos, 0, reps]
-    dbls = [float(np.float32(1.60217646e-19)), 1.66053886e-27, h, h, dom.hB[0], dom.hB[1], dom.dt,
-            charge, mass, W["Nm"], dom.zmax, n * h, n * h]
-    kq = O.kq(dom)
-    with tempfile.TemporaryDirectory() as td:
-        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
-        with open(fin, "wb") as f:
-            f.write(np.int64(len(ints)).tobytes() + np.asarray(ints, np.int64).tobytes())
-            f.write(np.int64(len(dbls)).tobytes() + np.asarray(dbls, np.float64).tobytes())
-            for a in (np.asarray(Ei, np.float64), np.asarray(dom.Bi, np.float64),
-                      np.asarray(dom.bcnd, np.int32), np.ascontiguousarray(vxp, np.float64),
-                      np.zeros(N, np.int8), np.zeros((n + 3, n + 3)), np.asarray(kq, np.float64)):
-                f.write(np.ascontiguousarray(a).tobytes())
-        subprocess.run([harness, "bench", fin, fout], check=True)
-        sec = float(np.frombuffer(open(fout, "rb").read(8), np.float64)[0])
-    return sec
-
-
-def cpu_baseline_reference(harness, W, dom, Ei, pos, ve, vi, N, O, setup, tab, budget_s):
-    """The reference's own push + deposition (Fortran, one core) on the
-    sample; MCC every 10 steps from the C restatement (collisions.f90 needs
-    MPI, so it cannot be compiled here)."""
-    e = np.column_stack([pos, ve])
-    i = np.column_stack([pos, vi])
-    probe = _ref_push_deposit(harness, W, dom, Ei, e, -L.QE, L.ME, 4, O) + \
-        _ref_push_deposit(harness, W, dom, Ei, i, L.QE, 39.95 * L.AMU, 4, O)
-    reps = max(4, int(budget_s / max(probe / 4, 1e-6)))
-    t_push = _ref_push_deposit(harness, W, dom, Ei, e, -L.QE, L.ME, reps, O) + \
-        _ref_push_deposit(harness, W, dom, Ei, i, L.QE, 39.95 * L.AMU, reps, O)
-    n = dom.n1
-    cap = N + N // 10
-    pe = np.zeros((cap, 6)); pe[:N] = e
-    pi_ = np.zeros((cap, 6)); pi_[:N] = i
-    parts = {1: dict(vxp=pe, np_tot=N, flag_dead=np.zeros(cap, np.int8)),
-             2: dict(vxp=pi_, np_tot=N, flag_dead=np.zeros(cap, np.int8))}
+    setup = WL.oracle_setup(Ws)
+    parts = {1: dict(vxp=e, np_tot=N), 2: dict(vxp=i, np_tot=N)}
     np_red = [np.full((n + 3, n + 3), 1e16), np.full((n + 3, n + 3), 1e16)]
-    t = time.perf_counter()
-    O.collisions(dom, setup, tab, parts, np_red, 123456)
-    t_mcc = time.perf_counter() - t
+    t_mcc = O.time_collisions_omp(dom, setup, tab, parts, np_red, nth)
     per_step = t_push / reps + t_mcc / 10.0
-    return dict(value=2 * N / per_step, unit="particles/s", cores=1, kind="reference",
-                sample=f"reference part_mover + charge_deposition (part_expmover.f90 built by amdflang -O2, "
-                       f"1 core) on {n}x{n} cells x 100 ppc x 2 species ({2 * N} particles), {reps} steps; "
-                       f"plus MCC/10 from the C restatement")
+    return dict(value=2 * N / per_step, unit="particles/s", cores=nth, kind="port",
+                sample=f"C restatement of part_mover + charge_deposition and collisions (OpenMP, {nth} threads, "
+                       f"one particle partition per thread as in the reference's main loop) on {n}x{n} cells x "
+                       f"{Ws['ppc']} ppc x 2 species ({2 * N} particles): {reps} push+deposit steps + one MCC "
+                       f"call counted as 1/10 per step",
+                push_deposit_s_per_step=t_push / reps, mcc_s_per_call=t_mcc)
+
+
+def accuracy_vs_oracle(W, steps=3):
+    """The mixed form against the FP64 oracle on the same state (untimed):
+    after `steps` steps of the mixed context, each species' deposited density
+    against the oracle's charge_deposition of the same particles, phi of the
+    next solve against the oracle's iterative_solver on the same right-hand
+    side from the same starting phi, E from phi, and the density and
+    temperature maps of part_moments/calc_avg against the oracle's on the
+    same particles, densities and sink maps (relative to the largest
+    magnitude; moments on nodes above a tenth of the peak density)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n1, n2, h = W["n1"], W["n2"], W["h"]
+    nsp = len(W["species"])
+    dom = O.Domain(n1=n1, n2=n2, h1=h, h2=h, dt=W["dt"], bcnd=W["bcnd"], ngrid=W["ngrid"], Bi=W["Bi"],
+                   nB=(n1, n2), hB=(h, h))
+    out = {}
+    t0 = time.perf_counter()
+    with WL.build_context(W, L.MIXED) as c:
+        for it in range(1, steps + 1):
+            c.calc_rhs()
+            c.iterative_solver(eps=1e-10, start=L.START_COLD if it == 1 else L.START_PREV)
+            c.calc_efield()
+            for p in range(1, nsp + 1):
+                c.part_mover(p, deposit=True)
+        parts = [c.download(p) for p in range(1, nsp + 1)]
+        nps = [c.get_grid(L.GRID_NP, p) for p in range(1, nsp + 1)]
+        sinks = [c.get_grid(L.GRID_SINK, p) for p in range(1, nsp + 1)]
+        srcs = [c.get_grid(L.GRID_SRC, p) for p in range(1, nsp + 1)]
+        phi0 = c.get_grid(L.GRID_PHI)
+        c.calc_rhs()
+        rhs = c.get_grid(L.GRID_RHS)
+        info = c.iterative_solver(eps=1e-10, start=L.START_PREV)
+        phi = c.get_grid(L.GRID_PHI)
+        c.calc_efield()
+        E = c.get_grid(L.GRID_E)
+        c.moments_reset()
+        c.moments_accumulate(1)
+        mom = c.moments()[0]
+    dens = {}
+    for p in range(nsp):
+        ref = O.charge_deposition(dom, W["species"][p]["Nm"], parts[p])
+        dens[W["species"][p]["name"]] = _rel(nps[p], ref)
+    ref = O.iterative_solver(phi0, rhs, W["bcnd"], (h, h), n1, n2, W["ng"], eps=1e-10)
+    E_ref = O.calc_efield(n1, n2, (h, h), ref["phi"], W["bcnd"], np.zeros((n2 + 3, n1 + 3, 2)))
+    out["density"] = dens
+    out["phi"] = _rel(phi, ref["phi"])
+    out["phi_max_abs_V"] = float(np.max(np.abs(ref["phi"])))
+    out["E"] = _rel(E, E_ref)
+    out["vcycles_mixed_vs_oracle"] = [int(info["vcycles"]), int(ref["vcycles"])]
+    pm = [O.part_moments(dom, W["species"][p]["Nm"], parts[p], 1) for p in range(nsp)]
+    want = O.calc_avg(dom, [s["mass"] for s in W["species"]], [s["Nm"] for s in W["species"]], nps, pm, sinks, 1,
+                      srcs=srcs)
+    mo = {}
+    for p, sp in enumerate(W["species"]):
+        d = want[p]
+        m = d[..., 0] > 0.1 * d[..., 0].max()
+        for j, q in ((0, "density"), (1, "temperature")):
+            mj = m & (d[..., j] != 0)
+            mo[f"{q}_{sp['name']}"] = float(np.max(np.abs(mom[p][..., j][mj] - d[..., j][mj])) /
+                                            np.max(np.abs(d[..., j][mj])))
+    out["moments"] = mo
+    out["state"] = f"after {steps} mixed steps from the load"
+    out["seconds"] = time.perf_counter() - t0
+    return out
 
 
 def main():
@@ -506,21 +413,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", choices=("e-Ar", "ion-source"), default="e-Ar",
-                    help="e-/Ar+ box (--n, --ppc) or the 1024x512 four-species negative-ion-source domain")
-    ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--ppc", type=int, default=100, help="macroparticles per cell of each species")
+    ap.add_argument("--workload", choices=tuple(WL.CONFIGS), default="argon-1024",
+                    help="argon-N: e-/Ar+ in an N x N box; ion-source: the 1024x512 four-species domain")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-plain-mixed", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
     ap.add_argument("--learn-steps", type=int, default=40)
     ap.add_argument("--train-iters", type=int, default=300)
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-accuracy", action="store_true",
-                    help="skip the untimed mixed-vs-FP64 comparison and the ng=1 SOR Poisson run")
-    ap.add_argument("--sor-solves", type=int, default=3,
-                    help="Poisson solves of the ng=1 SOR run beside MG (0: none)")
+                    help="skip the untimed comparisons (mixed against the FP64 oracle on the same state, "
+                         "mixed against FP64 after ten steps, SOR beside MG)")
+    ap.add_argument("--accuracy-only", action="store_true",
+                    help="only the mixed form against the FP64 oracle on the same state (no timed loop)")
+    ap.add_argument("--sor-solves", type=int, default=0,
+                    help="Poisson solves of the ng=1 red-black SOR run beside MG (0: none)")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="after the runs, one device-to-device copy of GIB GiB (a plain copy kernel "
                          "whose HBM counters calibrate those of the mover under rocprofv3 --pmc)")
@@ -529,10 +437,13 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
               "split is a design only)", file=sys.stderr)
         sys.exit(2)
+    W = WL.CONFIGS[args.workload]()
+    if args.accuracy_only:
+        print(json.dumps({"workload": W["desc"], "mixed_vs_fp64_oracle": accuracy_vs_oracle(W)}))
+        return
     hip = Hip()
-    W = workload(args.n, args.ppc) if args.workload == "e-Ar" else workload_ion_source()
     nsp = len(W["species"])
-    N = sum(k * W["n2"] * int(round((xr - xl) / W["h"])) for sp in W["species"] for k, xl, xr in sp["loads"])
+    N = sum(WL.load_counts(W))
     K = args.steps
 
     def summary(a, wall, cnt):
@@ -553,35 +464,37 @@ def main():
 
     res = {}
     if not args.skip_learned:
-        print(f"[bench] mixed precision + learned warm start, {N} particles, "
+        print(f"[bench] {W['name']}: mixed precision + learned warm start, {N} particles, "
               f"{args.learn_steps} recorded + {args.warmup}+{K} steps", file=sys.stderr, flush=True)
         r = run_mode(W, L.MIXED, K, args.warmup, hip, learned=True, learn_steps=args.learn_steps,
                      train_iters=args.train_iters)
         res["mixed_learned"] = (r, summary(*r))
     if not args.skip_plain_mixed:
-        print(f"[bench] mixed precision, {N} particles, {args.learn_steps}+{args.warmup}+{K} steps", file=sys.stderr,
+        print(f"[bench] {W['name']}: mixed precision, {args.learn_steps}+{args.warmup}+{K} steps", file=sys.stderr,
               flush=True)
         r = run_mode(W, L.MIXED, K, args.warmup, hip, learn_steps=args.learn_steps)
         res["mixed"] = (r, summary(*r))
     if not args.skip_fp64:
-        print("[bench] FP64", file=sys.stderr, flush=True)
+        print(f"[bench] {W['name']}: FP64", file=sys.stderr, flush=True)
         r = run_mode(W, L.FP64, K, args.warmup, hip, learn_steps=args.learn_steps)
         res["fp64"] = (r, summary(*r))
     head = "mixed_learned" if "mixed_learned" in res else "mixed"
     (acc_h, _, cnt_h), s_h = res[head]
     # roofline of the dominant kernel, the mixed-precision mover with fused
-    # deposition (one instantiation serves electrons and ions): algorithmic
-    # bytes = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities, read
-    # and written) per particle and launch; launch durations come from HIP
-    # events around that kernel alone, on the library's stream
+    # deposition (one instantiation serves every species): algorithmic bytes
+    # = 2 x 24 B (int32 cell + 2 FP32 offsets + 3 FP32 velocities, read and
+    # written) per particle and launch; launch durations come from HIP events
+    # around that kernel alone, on the library's stream.  HBM traffic needs
+    # PMC counters (rocprofv3 --pmc), which a timed run cannot read: the
+    # measured figures are in profiles/, not in this line.
     bpp = 48.0
     kms = [float(np.mean(x)) for x in acc_h["push_kernel_ms"]]
     alg = [c * bpp for c in cnt_h]
     achieved = sum(alg) / (sum(kms) * 1e-3) / 1e9
-    # HBM traffic per particle was measured with PMC counters on the default
-    # workload only
-    measured = W["kind"] == "e-Ar" and args.n == 512 and args.ppc == 100
-    cpu = None if (args.skip_cpu or W["kind"] != "e-Ar") else cpu_baseline(W, args.cpu_budget)
+    cpu = None
+    if not args.skip_cpu and W["kind"] == "e-Ar":
+        print(f"[bench] CPU baseline ({host_threads()} host threads)", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(W, args.cpu_budget)
     out = {
         "metric": "ms/step & particles/s (push+deposit+MCC), mixed+learned vs FP64 GPU",
         "value": s_h["particles_per_s"],
@@ -595,8 +508,7 @@ def main():
                                                    else "mixed precision"),
                    "global_batch": N, "seq_len": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_PARTICLE * sum(cnt_h) / nsp if measured else None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_mover_mixed<256> (one launch per species)",
                      "alg_bytes_per_launch": sum(alg) / nsp,
                      "avg_launch_ms": sum(kms) / nsp,
@@ -612,8 +524,12 @@ def main():
         out["speedup_vs_fp64_step"] = s_f["ms_per_step"] / s_h["ms_per_step"]
         out["speedup_vs_fp64_pdm"] = s_h["particles_per_s"] / s_f["particles_per_s"]
     if not args.skip_accuracy:
-        print("[bench] mixed against FP64 (untimed), ng=1 SOR beside MG", file=sys.stderr, flush=True)
-        out["accuracy_mixed_vs_fp64"], out["poisson_sor_vs_mg"] = accuracy_and_sor(W, sor_solves=args.sor_solves)
+        print("[bench] mixed against the FP64 oracle on the same state (untimed)", file=sys.stderr, flush=True)
+        out["accuracy_mixed_vs_fp64_oracle"] = accuracy_vs_oracle(W)
+        print("[bench] mixed against FP64 after ten steps (untimed)", file=sys.stderr, flush=True)
+        out["accuracy_mixed_vs_fp64"], sor = accuracy_and_sor(W, sor_solves=args.sor_solves)
+        if sor is not None:
+            out["poisson_sor_vs_mg"] = sor
     if args.copy_calibration > 0:
         out["copy_calibration_bytes"] = hip.copy_once(int(args.copy_calibration * (1 << 30)))
     print(json.dumps(out))

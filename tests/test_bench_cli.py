@@ -1,6 +1,7 @@
 """CPU checks of bench.py's host logic: the one-GPU contract (--gpus above 1
 exits with a message and status 2 before touching a GPU) and the workload
-builders (particle counts, species loads, wall labels, multigrid depth)."""
+builders of jd_lepic_amd.workloads (particle counts, species loads, wall
+labels, multigrid depth)."""
 import os
 import subprocess
 import sys
@@ -10,6 +11,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench as B  # noqa: E402
+from jd_lepic_amd import workloads as WL  # noqa: E402
 
 
 def test_more_than_one_gpu_is_refused():
@@ -22,8 +24,13 @@ def loaded(W):
     return sum(k * W["n2"] * int(round((xr - xl) / W["h"])) for s in W["species"] for k, xl, xr in s["loads"])
 
 
-def test_default_workload():
-    W = B.workload(512)
+def test_argon_workloads():
+    for name, n, ppc, ng in (("argon-128", 128, 200, 7), ("argon-512", 512, 100, 9), ("argon-1024", 1024, 100, 10),
+                             ("argon-2048", 2048, 120, 11)):
+        W = WL.CONFIGS[name]()
+        assert W["name"] == name and W["ng"] == ng and loaded(W) == 2 * n * n * ppc == sum(WL.load_counts(W))
+        assert W["Nm"] == 1e16 * (5e-5) ** 2 / ppc
+    W = WL.CONFIGS["argon-512"]()
     assert W["kind"] == "e-Ar" and W["ng"] == 9 and loaded(W) == 2 * 512 * 512 * 100
     assert [s["name"] for s in W["species"]] == ["e", "i"] and len(W["vt0"]) == 3
     bc = W["bcnd"]
@@ -32,7 +39,7 @@ def test_default_workload():
 
 
 def test_ion_source_workload():
-    W = B.workload_ion_source()
+    W = WL.CONFIGS["ion-source"]()
     assert (W["n1"], W["n2"], W["ng"], W["ngrid"], W["tag_neg"]) == (1024, 512, 9, 3, 4)
     assert [s["name"] for s in W["species"]] == ["e", "Hp", "H2p", "Hm"] and len(W["vt0"]) == 6
     assert 7.5e7 < loaded(W) < 8.5e7
@@ -48,3 +55,10 @@ def test_ion_source_workload():
     # quasi-neutral where the density is 1e17: e + H- = H+ + H2+
     k0 = {s["name"]: s["loads"][0][0] for s in W["species"]}
     assert abs(k0["e"] + k0["Hm"] - k0["Hp"] - k0["H2p"]) <= 2
+
+
+def test_default_bench_workload_is_the_1024_box():
+    import argparse  # noqa: F401
+    src = open(os.path.join(ROOT, "bench.py")).read()
+    assert 'default="argon-1024"' in src
+    assert B.host_threads() >= 1
