@@ -8,8 +8,9 @@
  * Usage: ref_harness <cmd> <in.bin> <out.bin>
  *   in.bin  = int64 ni, int64[ni], int64 nd, double[nd], then raw arrays
  *   out.bin = raw arrays (order per command, see tests/refharness.py)
- * Commands: ran2, gauss, mover, deposit, efield, sort, heat, emit, bench (the timed
- * push + deposition loop of bench.py's CPU baseline).
+ * Commands: ran2, gauss, mover, deposit, efield, sort, heat, emit, bench (a timed
+ * push + deposition loop, one thread) and ompbench (the same with one particle
+ * partition per OpenMP thread, as the reference's main loop runs it).
  *
  * The routines take every argument by reference (implicit interfaces) and
  * read their scalars from the COMMON blocks declared in particle_info.h:56-73
@@ -23,6 +24,8 @@
 #include <string.h>
 #include <math.h>
 #include <time.h>
+
+#include <omp.h>
 
 #define NPART 32
 
@@ -299,6 +302,74 @@ int main(int argc, char **argv) {
     clock_gettime(CLOCK_MONOTONIC, &t1);
     double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
     int64_t npt = np_tot;
+    wr(&sec, 8);
+    wr(&npt, 8);
+  } else if (!strcmp(cmd, "ompbench")) {
+    /* As "bench" with I[11] = nthreads: the particles split into nthreads
+     * partitions vxp(6,nmax,1,nproc), and inside one OpenMP region per step
+     * thread iproc calls part_mover and charge_deposition on partition iproc
+     * with its own density np(:,:,1,iproc), the way main.f90:1024-1089 runs
+     * them.  Writes the seconds spent and the final particle count. */
+    set_constants();
+    int32_t n[3] = {(int32_t)I[0], (int32_t)I[1], 1}, na[3] = {n[0], n[1], 1};
+    int32_t nB[2] = {(int32_t)I[2], (int32_t)I[3]};
+    const int64_t N = I[4], reps = I[10];
+    const int32_t T = (int32_t)I[11], ngrid = (int32_t)I[5];
+    part_info_int_.flag_B = (int32_t)I[6];
+    part_info_int_.flag_gridB = (int32_t)I[7];
+    part_info_int_.flag_B_pos = (int32_t)I[8];
+    part_info_int_.tag_neg = I[9] ? 1 : 0;
+    part_info_int_.size_na = 1;
+    double h[3] = {D[2], D[3], 1.0}, hB[2] = {D[4], D[5]};
+    part_info_dp_2_.dt = D[6];
+    part_info_dp_1_.charge[0] = D[7];
+    part_info_dp_1_.mass[0] = D[8];
+    part_info_dp_2_.Nm[0] = D[9];
+    part_info_dp_2_.zmax = D[10];
+    part_info_dp_2_.xmax = D[11];
+    part_info_dp_2_.ymax = D[12];
+    part_info_dp_2_.hz = -1.0;
+    size_t nodes = (size_t)(n[0] + 3) * (n[1] + 3), bnodes = (size_t)(nB[0] + 3) * (nB[1] + 3);
+    double *Ei = rd(16 * nodes), *Bi = rd(32 * bnodes);
+    int32_t *bcnd = rd(4 * nodes);
+    double *src = rd(48 * (size_t)(N > 0 ? N : 1));
+    int8_t *fd_in = rd((size_t)(N > 0 ? N : 1));
+    double *phi = rd(8 * nodes), *kq = rd(8 * nodes);
+    (void)fd_in;
+    int32_t nmax = (int32_t)((N + T - 1) / T);
+    if (nmax < 1) nmax = 1;
+    double *vxp = calloc((size_t)6 * nmax * T, 8);
+    int8_t *fd = calloc((size_t)nmax * T, 1);
+    int32_t *np_tot = calloc((size_t)T, 4), *ip_ps = calloc((size_t)T, 4), *Nh = calloc((size_t)T, 4);
+    double *p_mac = calloc(2 * (size_t)(ngrid + 1) * T, 8), *P_loss = calloc(4 * (size_t)T, 8);
+    double *uB = calloc((size_t)(na[0] + 1) * (na[1] + 1), 8), *cnt_dead = calloc((size_t)T, 8);
+    double *sum_q = calloc((size_t)T, 8), *sum_dEk = calloc((size_t)T, 8);
+    double *sum_q_y = calloc((size_t)(ngrid > 0 ? ngrid : 1) * (n[1] + 3) * T, 8);
+    double *ps = calloc(4 * (size_t)T, 8);
+    double *ss2D = calloc(2 * nodes * T, 8), *np = calloc(nodes * T, 8);
+    for (int32_t t = 0; t < T; ++t) {
+      const int64_t a = N * t / T, b = N * (t + 1) / T;
+      memcpy(vxp + (size_t)6 * nmax * t, src + 6 * a, 48 * (size_t)(b - a));
+      np_tot[t] = (int32_t)(b - a);
+    }
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int64_t k = 0; k < reps; ++k) {
+#pragma omp parallel num_threads(T)
+      {
+        int32_t iproc = omp_get_thread_num() + 1, istep = 1, ntype = 1, ptype = 1, nproc = T,
+                iseed = 123456 * iproc, nmax_ps = 1, ng = ngrid, nmx = nmax, n_icp = 1;
+        part_mover_(&istep, n, na, h, Ei, Bi, p_mac, P_loss, vxp, bcnd, &nmx, &ntype, &ng, fd, &nproc, np_tot,
+                    phi, uB, &iproc, &ptype, &iseed, cnt_dead, sum_q, sum_q_y, ps, ip_ps, &nmax_ps, nB, hB, ss2D);
+        memset(np + nodes * (iproc - 1), 0, 8 * nodes);
+        charge_deposition_(n, h, vxp, &nmx, &ntype, kq, np, &nproc, np_tot, sum_dEk, Nh, &iproc, &ptype,
+                           &n_icp);
+      }
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    double sec = (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+    int64_t npt = 0;
+    for (int32_t t = 0; t < T; ++t) npt += np_tot[t];
     wr(&sec, 8);
     wr(&npt, 8);
   } else if (!strcmp(cmd, "deposit")) {
