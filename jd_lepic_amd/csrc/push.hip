@@ -45,9 +45,12 @@ constexpr int kWin = kWinRows * kWinCols;
 // Fixed-point units of the deposition: a particle's four CIC weights sum to
 // 1.  FP64 contexts round each run sum to 2^-42 of a particle (an absolute
 // error below 1.2e-13 of one particle's charge per run; a node holds up to
-// 2^21 particles); mixed contexts to 2^-20 (finer than FP32 accumulation
-// once a node holds more than 8 particles; an int32 LDS window node holds
-// up to 2^11 particles, more than a 1024-slot segment can bring).
+// 2^21 particles; the runs of an FP64 segment are fixed by the particle
+// order alone).  Mixed contexts round every particle's weights to 2^-20
+// before any sum (finer than FP32 accumulation once a node holds more than
+// 8 particles), so no run, merge or atomic order changes a bit; an int32
+// LDS window node holds up to 2^11 particles, more than a 1024-slot segment
+// can bring.
 constexpr int kFixShift64 = 42;
 constexpr int kFixShiftMx = 20;
 constexpr double kFix64 = 4398046511104.0;   // 2^42
@@ -90,6 +93,16 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // ---- deposition helpers ----------------------------------------------------
 __device__ __forceinline__ long long fix64(double w) { return __double2ll_rn(w * kFix64); }
 __device__ __forceinline__ int fixmx(float w) { return __float2int_rn(w * kFixMx); }
+// A mixed particle's four CIC weights (corners 00,10,11,01 of
+// part_expmover.f90:571-584) from its in-cell offsets, each rounded to
+// fixed point before any summation.
+__device__ __forceinline__ void cic_fix(float fx, float fy, int& q0, int& q1, int& q2, int& q3) {
+  const float px = 1.0f - fx, py = 1.0f - fy;
+  q0 = fixmx(px * py);
+  q1 = fixmx(fx * py);
+  q2 = fixmx(fx * fy);
+  q3 = fixmx(px * fy);
+}
 __device__ __forceinline__ void acc_add(unsigned long long* p, long long v) {
   atomicAdd(p, (unsigned long long)v);
 }
@@ -230,29 +243,32 @@ __device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t 
   return total;
 }
 
-// DPP lane move of a float (see dpp_move for the controls)
+// DPP lane move of a 32-bit integer (row shifts 0x111..0x118, row
+// broadcasts 0x142/0x143 with their row masks)
 template <int CTRL, int ROWS>
-__device__ __forceinline__ float dpp_f32(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xf, false));
+__device__ __forceinline__ int dpp_i32(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xf, false);
 }
 
 // Segmented inclusive sum of w[0..3] over runs of equal key in adjacent
-// lanes, in a fixed order: afterwards the last lane of each run holds the
-// run's sums.  s = first lane of this lane's run.  Every lane must be active.
-__device__ __forceinline__ void seg_sum4(float w[4], int s) {
+// lanes: afterwards the last lane of each run holds the run's sums.  s =
+// first lane of this lane's run.  Every lane must be active.  The mixed
+// deposition sums fixed-point integers, so the result does not depend on
+// which particles share a run or in what order they arrive.
+__device__ __forceinline__ void seg_sum4(int w[4], int s) {
   const int lane = threadIdx.x & 63;
   const bool a1 = lane - 1 >= s, a2 = lane - 2 >= s, a4 = lane - 4 >= s, a8 = lane - 8 >= s;
   const bool b15 = (lane & 16) && (lane & ~15) - 1 >= s;   // rows 1 and 3
   const bool b31 = lane >= 32 && s <= 31;                  // rows 2 and 3
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    float v = w[c], o;
-    o = dpp_f32<0x111, 0xf>(v); if (a1) v += o;
-    o = dpp_f32<0x112, 0xf>(v); if (a2) v += o;
-    o = dpp_f32<0x114, 0xf>(v); if (a4) v += o;
-    o = dpp_f32<0x118, 0xf>(v); if (a8) v += o;
-    o = dpp_f32<0x142, 0xa>(v); if (b15) v += o;
-    o = dpp_f32<0x143, 0xc>(v); if (b31) v += o;
+    int v = w[c], o;
+    o = dpp_i32<0x111, 0xf>(v); if (a1) v += o;
+    o = dpp_i32<0x112, 0xf>(v); if (a2) v += o;
+    o = dpp_i32<0x114, 0xf>(v); if (a4) v += o;
+    o = dpp_i32<0x118, 0xf>(v); if (a8) v += o;
+    o = dpp_i32<0x142, 0xa>(v); if (b15) v += o;
+    o = dpp_i32<0x143, 0xc>(v); if (b31) v += o;
     w[c] = v;
   }
 }
@@ -304,13 +320,13 @@ __device__ __forceinline__ void deposit_runs64(const MoverCommon& a, unsigned lo
 
 // Mixed weights of one cell straight to the global accumulator (nodes
 // outside the LDS window).
-__device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int iy, float q0, float q1, float q2,
-                                            float q3) {
+__device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int iy, int q0, int q1, int q2,
+                                            int q3) {
   const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
-  acc_add(&a.acc[n00], fixmx(q0));
-  acc_add(&a.acc[n00 + 1], fixmx(q1));
-  acc_add(&a.acc[n01 + 1], fixmx(q2));
-  acc_add(&a.acc[n01], fixmx(q3));
+  acc_add(&a.acc[n00], q0);
+  acc_add(&a.acc[n00 + 1], q1);
+  acc_add(&a.acc[n01 + 1], q2);
+  acc_add(&a.acc[n01], q3);
 }
 
 // CIC deposition of thread-major items (consecutive particles per thread):
@@ -325,17 +341,17 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, i
                                                const float FY[kItems], const bool ok[kItems]) {
   const int lane = threadIdx.x & 63;
   int key[kItems];          // window-local cell, -1: nothing in this slot
-  float w[kItems][4];
+  int w[kItems][4];
   int prev = -1;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     key[k] = -1;
-    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0.f;
+    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
     if (!ok[k]) continue;
     int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
     clamp_cell(ix, iy, a.n1, a.n2);
-    const float px = 1.0f - FX[k], py = 1.0f - FY[k];
-    const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
+    int q0, q1, q2, q3;
+    cic_fix(FX[k], FY[k], q0, q1, q2, q3);
     const int cx = ix - wx0, cy = iy - wy0;
     if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
       deposit_global(a, ix, iy, q0, q1, q2, q3);
@@ -362,16 +378,16 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, i
     const int kp = __shfl_up(kk, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || kp != kk);
     const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
-    float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
+    int v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
     // every active lane its own run (a slot holding only the cell changes of
     // a few threads): nothing to merge
     if (__popcll(heads & act) != __popcll(act)) seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
     if (tail && key[k] >= 0) {
-      atomicAdd(&win[kk], fixmx(v[0]));
-      atomicAdd(&win[kk + 1], fixmx(v[1]));
-      atomicAdd(&win[kk + kWinCols + 1], fixmx(v[2]));
-      atomicAdd(&win[kk + kWinCols], fixmx(v[3]));
+      atomicAdd(&win[kk], v[0]);
+      atomicAdd(&win[kk + 1], v[1]);
+      atomicAdd(&win[kk + kWinCols + 1], v[2]);
+      atomicAdd(&win[kk + kWinCols], v[3]);
     }
   }
 }
@@ -389,15 +405,15 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
   if (__ballot(any) == 0) return;                 // wave-uniform
   const int lane = threadIdx.x & 63;
   int key[kItems];
-  float w[kItems][4];
+  int w[kItems][4];
   int prev = -1;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     key[k] = -1;
-    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0.f;
+    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
     if (gk[k] < 0) continue;
-    const float px = 1.0f - FX[k], py = 1.0f - FY[k];
-    const float q0 = px * py, q1 = FX[k] * py, q2 = FX[k] * FY[k], q3 = px * FY[k];
+    int q0, q1, q2, q3;
+    cic_fix(FX[k], FY[k], q0, q1, q2, q3);
     if (gk[k] == prev) {
 #pragma unroll
       for (int j = k - 1; j >= 0; --j)
@@ -415,7 +431,7 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
     const int kp = __shfl_up(kk, 1, 64);
     const uint64_t heads = __ballot(lane == 0 || kp != kk);
     const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
-    float v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
+    int v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
     seg_sum4(v, s);
     const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
     if (tail && key[k] >= 0) deposit_global(a, key[k] & 0xFFFF, key[k] >> 16, v[0], v[1], v[2], v[3]);
@@ -662,9 +678,9 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
 
 // ---------------------------------------------------------------------------
 // Mixed-precision mover: int32 cell + FP32 in-cell offset, FP32 velocity,
-// FP32 E/B gathers; deposition of FP32 run sums into a fixed-point int32 LDS
-// window, flushed to the int64 accumulator (scaled to the FP64 density by
-// k_np_from_acc with the FP64 factor Nm/(h1 h2 zmax)).
+// FP32 E/B gathers; deposition of fixed-point weights (int32 run sums) into
+// an int32 LDS window, flushed to the int64 accumulator (scaled to the FP64
+// density by k_np_from_acc with the FP64 factor Nm/(h1 h2 zmax)).
 // ---------------------------------------------------------------------------
 struct MixedParams {
   float k1f, dtoh1, dtoh2, dtf;

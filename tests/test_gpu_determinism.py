@@ -52,7 +52,7 @@ def test_steps_bit_identical_across_runs(precision):
     i[:, 3:] = np.random.default_rng(74).normal(0, 300.0, (N, 3))
     runs = []
     for run in range(2):
-        with ctx_for(dom, precision, N + 1024, 7) as ctx:
+        with ctx_for(dom, precision, N + 1024, 6) as ctx:
             ctx.upload(1, e)
             ctx.upload(2, i)
             for p in (1, 2):
