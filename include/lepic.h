@@ -168,6 +168,17 @@ int lepic_iterative_solver(lepic_ctx *ctx, double eps, int32_t ncycl, int32_t st
  * (optional, n1+n1*n2+2 int32) receives the cell-end list, which is only
  * computed when Plist is not NULL (the device MCC does not need it). */
 int lepic_part_sorting(lepic_ctx *ctx, int32_t ptype, int32_t *Plist);
+/* lepic_part_sorting without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same kernels and results as lepic_part_sorting, with the scratch in a
+ * static device array of the library (about 21 MiB per device, held until
+ * the process ends): species of at most 512 segments of 1024 slots (524288
+ * particles) and, when Plist is requested, grids with n1 + n1*n2 + 2 <=
+ * 1052672 (up to 1024 x 1024 cells); larger inputs return
+ * LEPIC_ERR_CAPACITY.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (two host threads, two contexts) race. */
+int lepic_part_sorting_noalloc(lepic_ctx *ctx, int32_t ptype, int32_t *Plist);
 /* dens_red (calc_rho.f90:134-199): np_red and np_mx per charged species */
 int lepic_dens_red(lepic_ctx *ctx, double *np_mx);
 

@@ -154,6 +154,7 @@ SIGNATURES = {
     "lepic_iterative_solver": (C.c_int, [_ctxp, C.c_double, C.c_int32, C.c_int32,
                                          C.POINTER(SolveInfo), _ip, C.c_int32]),
     "lepic_part_sorting": (C.c_int, [_ctxp, C.c_int32, _ip]),
+    "lepic_part_sorting_noalloc": (C.c_int, [_ctxp, C.c_int32, _ip]),
     "lepic_dens_red": (C.c_int, [_ctxp, _dp]),
     "lepic_set_reactions": (C.c_int, [_ctxp, C.POINTER(Reactions)]),
     "lepic_collisions": (C.c_int, [_ctxp, C.c_int32, C.c_int32, C.c_uint64, _ip,

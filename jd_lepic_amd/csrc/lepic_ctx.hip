@@ -230,7 +230,13 @@ int ensure_packed(lepic_ctx* ctx, int p) {
   return set_packed_counts(ctx, p);
 }
 
-int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n) {
+int64_t scan_workspace_words(int64_t n) {
+  int64_t w = 0;
+  for (int64_t nb = (n + kScanTile - 1) / kScanTile; nb > 1; nb = (nb + kScanTile - 1) / kScanTile) w += nb;
+  return w;
+}
+
+int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n, uint32_t* ws) {
   if (n <= 0) return 0;
   int64_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 1) {
@@ -239,15 +245,17 @@ int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n) {
     LEPIC_HIP(hipGetLastError());
     return 0;
   }
-  uint32_t* part = nullptr;
-  LEPIC_HIP(hipMallocAsync((void**)&part, nb * sizeof(uint32_t), ctx->stream));
+  // block partials: from the caller's workspace (scan_workspace_words(n)
+  // words, the levels one after another) or stream-ordered allocations
+  uint32_t* part = ws;
+  if (!ws) LEPIC_HIP(hipMallocAsync((void**)&part, nb * sizeof(uint32_t), ctx->stream));
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, ctx->stream, data, n, part);
   LEPIC_HIP(hipGetLastError());
-  if (int rc = scan_exclusive_u32(ctx, part, nb)) return rc;
+  if (int rc = scan_exclusive_u32(ctx, part, nb, ws ? ws + nb : nullptr)) return rc;
   hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(256), 0, ctx->stream, data, n,
                      (const uint32_t*)part);
   LEPIC_HIP(hipGetLastError());
-  LEPIC_HIP(hipFreeAsync(part, ctx->stream));
+  if (!ws) LEPIC_HIP(hipFreeAsync(part, ctx->stream));
   return 0;
 }
 
@@ -381,6 +389,7 @@ int lepic_create(const lepic_grid_desc* g, const lepic_species_desc* sp, lepic_c
   H(hipHostMalloc((void**)&ctx->h_pinned, kPinnedDoubles * 8, hipHostMallocDefault), "pinned");
   ctx->bc_host.assign(ctx->nodes, 1);
   if (!rc) rc = poisson_setup(ctx);
+  if (!rc) rc = sort_scratch_resolve(ctx);
   *out = ctx;
   if (!rc) rc = (hipDeviceSynchronize() == hipSuccess) ? 0 : ctx->fail(LEPIC_ERR_HIP, "sync");
   return rc;
@@ -793,6 +802,11 @@ int lepic_iterative_solver(lepic_ctx* ctx, double eps, int32_t ncycl, int32_t st
 int lepic_part_sorting(lepic_ctx* ctx, int32_t ptype, int32_t* Plist) {
   if (!ctx || !valid_ptype(ctx, ptype)) return LEPIC_ERR_ARG;
   return sort_launch(ctx, ptype - 1, Plist);
+}
+
+int lepic_part_sorting_noalloc(lepic_ctx* ctx, int32_t ptype, int32_t* Plist) {
+  if (!ctx || !valid_ptype(ctx, ptype)) return LEPIC_ERR_ARG;
+  return sort_noalloc_launch(ctx, ptype - 1, Plist);
 }
 
 int lepic_dens_red(lepic_ctx* ctx, double* np_mx) {

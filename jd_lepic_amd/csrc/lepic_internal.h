@@ -198,6 +198,9 @@ struct lepic_ctx {
   uint32_t* s_nrun = nullptr;
   int64_t s_seg_cap = 0;           // slots covered by s_meta / s_rkey / s_rcnt / s_rcntd (segments x kChunk)
   double* d_diag = nullptr;          // per-block partials of the charge/energy totals
+  // this device's copy of the static sort scratch array (device address
+  // from hipGetSymbolAddress, resolved at creation)
+  void* sort_static = nullptr;
 
   // collisions
   lepic::Reactions rx;
@@ -237,7 +240,10 @@ namespace lepic {
 int ensure_packed(lepic_ctx* ctx, int p);        // 0-based species
 int set_packed_counts(lepic_ctx* ctx, int p);    // segments from the device count
 inline int64_t nseg_of(const Species& s) { return (s.ub + kChunk - 1) / kChunk; }
-int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n);   // in place
+// in place; ws: optional workspace of scan_workspace_words(n) words for the
+// block partials (none allocated then)
+int scan_exclusive_u32(lepic_ctx* ctx, uint32_t* data, int64_t n, uint32_t* ws = nullptr);
+int64_t scan_workspace_words(int64_t n);
 int zero_async(lepic_ctx* ctx, void* p, size_t bytes);
 int deposit_range_launch(lepic_ctx* ctx, int p, int64_t seg0, int64_t nseg);   // push.hip
 
@@ -252,6 +258,11 @@ void poisson_teardown(lepic_ctx* ctx);
 int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_info* info,
                   int32_t* log, int32_t log_cap);
 int sort_launch(lepic_ctx* ctx, int ptype, int32_t* Plist_host);
+// the same sort with its scratch in the library's static device array
+// (no allocation); sort_scratch_resolve finds that array's address on the
+// current device (once per device, at context creation)
+int sort_noalloc_launch(lepic_ctx* ctx, int ptype, int32_t* Plist_host);
+int sort_scratch_resolve(lepic_ctx* ctx);
 int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed,
             lepic_mcc_info* info);
 int load_uniform_launch(lepic_ctx* ctx, int ptype, int ppc, double vt, double xl, double xr,

@@ -28,6 +28,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "lepic_internal.h"
 
@@ -426,7 +429,24 @@ int grow(lepic_ctx* ctx, T** p, int64_t n) {
   return 0;
 }
 
-int ensure_sort_scratch(lepic_ctx* ctx, int64_t nslots, int64_t nseg, int64_t nb, int64_t ncnt) {
+// Device buffers of one sort: keys and values of the run sort (two of each),
+// the per-slot (run, rank), the per-segment runs (key, count) at a stride
+// of kChunk, the dense run counts, the run bases, the digit histograms, the
+// cell-end list and the scans' block partials (null: allocated by the scan).
+struct SortWork {
+  uint32_t* key[2];
+  uint32_t* val[2];
+  uint32_t* meta;
+  uint32_t* rkey;
+  uint32_t* rcnt;
+  uint32_t* rcntd;
+  uint32_t* nrun;
+  uint32_t* hist;
+  uint32_t* plist;
+  uint32_t* scan_ws;
+};
+
+int ensure_sort_scratch(lepic_ctx* ctx, int64_t nslots, int64_t nb, int64_t ncnt) {
   if (nslots > ctx->s_cap) {
     for (int b = 0; b < 2; ++b) {
       if (int rc = grow(ctx, &ctx->s_key[b], nslots)) return rc;
@@ -453,34 +473,46 @@ int ensure_sort_scratch(lepic_ctx* ctx, int64_t nslots, int64_t nseg, int64_t nb
   return 0;
 }
 
+// The static scratch of lepic_part_sorting_noalloc: a species of at most
+// kStaticSlots slots (kStaticSlots / kChunk segments) and a cell-end list of
+// at most kStaticPlist entries (grids of up to 1024 x 1024 cells).  Every
+// call on a device shares that device's copy; a call leaves its contents
+// behind, and the sort writes every word it reads before reading it (the
+// one counter it needs zeroed, the run-count terminator, is cleared by a
+// fill kernel).
+constexpr int64_t kStaticSlots = int64_t(1) << 19;
+constexpr int64_t kStaticSeg = kStaticSlots / kChunk;
+constexpr int64_t kStaticHist = int64_t(kMaxRadix) * (kStaticSlots / kTile);
+constexpr int64_t kStaticPlist = (int64_t(1) << 20) + 4096;
+constexpr int64_t kStaticScanWs = 2 * (kStaticSlots / 4096 + 64);   // >= scan_workspace_words of any scan here
+constexpr int64_t kStaticWords = 8 * kStaticSlots + (kStaticSeg + 2) + kStaticHist + kStaticPlist + kStaticScanWs;
+static_assert(kStaticSlots % kTile == 0, "whole histogram tiles");
+
+static __device__ __align__(256) uint32_t scratch[kStaticWords];
+
 }  // namespace
 
 namespace lepic {
 
 template <int R>
-int radix_pass(lepic_ctx* ctx, int cur, int pass, int shift, int64_t n, int64_t nb) {
-  hipLaunchKernelGGL(k_hist<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur], n, shift,
-                     ctx->s_hist, (int)nb);
+int radix_pass(lepic_ctx* ctx, const SortWork& W, int cur, int pass, int shift, int64_t n, int64_t nb) {
+  hipLaunchKernelGGL(k_hist<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, W.key[cur], n, shift, W.hist,
+                     (int)nb);
   LEPIC_HIP(hipGetLastError());
-  if (int rc = scan_exclusive_u32(ctx, ctx->s_hist, (int64_t)R * nb)) return rc;
-  hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, ctx->s_key[cur],
-                     pass == 0 ? (const uint32_t*)nullptr : ctx->s_val[cur], ctx->s_key[1 - cur],
-                     ctx->s_val[1 - cur], ctx->s_hist, n, shift, (int)nb);
+  if (int rc = scan_exclusive_u32(ctx, W.hist, (int64_t)R * nb, W.scan_ws)) return rc;
+  hipLaunchKernelGGL(k_scatter<R>, dim3((unsigned)nb), dim3(256), 0, ctx->stream, W.key[cur],
+                     pass == 0 ? (const uint32_t*)nullptr : W.val[cur], W.key[1 - cur], W.val[1 - cur], W.hist, n,
+                     shift, (int)nb);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
 
-int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
+// The sort of species p with its device buffers in W (sized for nseg
+// segments and, with Plist_host, pl_max + 1 list entries).
+int sort_core(lepic_ctx* ctx, int p, int32_t* Plist_host, const SortWork& W, int64_t nseg, int64_t pl_max) {
   Species& s = ctx->sp[p];
   const int n1 = ctx->g.n1, n2 = ctx->g.n2;
   const int64_t ncell = (int64_t)n1 * n2;
-  if (ncell >= (1ll << 31)) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "2^31 cells or more");
-  const int64_t pl_max = (int64_t)n1 + ncell + 1;
-  const int64_t nseg = std::min<int64_t>(nseg_of(s), s.nseg_cap);
-  const int64_t nslots = std::max<int64_t>(nseg, 1) * kChunk;
-  if (int rc = ensure_sort_scratch(ctx, nslots, nseg, std::max<int64_t>((nslots + kTile - 1) / kTile, 1),
-                                   Plist_host ? pl_max + 1 : 0))
-    return rc;
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
   const int b = s.cur, o = 1 - s.cur;
   int64_t M = 0;
@@ -492,20 +524,20 @@ int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
     A.x = s.d[b][0];
     A.y = s.d[b][1];
     A.h1 = ctx->g.h1; A.h2 = ctx->g.h2; A.n1 = n1; A.n2 = n2;
-    A.meta = ctx->s_meta; A.rkey = ctx->s_rkey; A.rcnt = ctx->s_rcnt; A.nrun = ctx->s_nrun;
+    A.meta = W.meta; A.rkey = W.rkey; A.rcnt = W.rcnt; A.nrun = W.nrun;
     if (mixed) hipLaunchKernelGGL(k_seg_sort<true>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, A);
     else hipLaunchKernelGGL(k_seg_sort<false>, dim3((unsigned)nseg), dim3(kSegThreads), 0, ctx->stream, A);
     LEPIC_HIP(hipGetLastError());
     // run bases (exclusive scan over the segments; entry nseg = M)
-    if (int rc = zero_async(ctx, ctx->s_nrun + nseg, 4)) return rc;
-    if (int rc = scan_exclusive_u32(ctx, ctx->s_nrun, nseg + 1)) return rc;
-    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, ctx->s_nrun + nseg, 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (int rc = zero_async(ctx, W.nrun + nseg, 4)) return rc;
+    if (int rc = scan_exclusive_u32(ctx, W.nrun, nseg + 1, W.scan_ws)) return rc;
+    LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, W.nrun + nseg, 4, hipMemcpyDeviceToHost, ctx->stream));
     LEPIC_HIP(hipStreamSynchronize(ctx->stream));
     uint32_t m32;
     std::memcpy(&m32, ctx->h_pinned, 4);
     M = m32;
-    hipLaunchKernelGGL(k_runs_pack, dim3((unsigned)nseg), dim3(64), 0, ctx->stream, (const uint32_t*)ctx->s_rkey,
-                       (const uint32_t*)ctx->s_rcnt, (const uint32_t*)ctx->s_nrun, ctx->s_key[0], ctx->s_rcntd);
+    hipLaunchKernelGGL(k_runs_pack, dim3((unsigned)nseg), dim3(64), 0, ctx->stream, (const uint32_t*)W.rkey,
+                       (const uint32_t*)W.rcnt, (const uint32_t*)W.nrun, W.key[0], W.rcntd);
     LEPIC_HIP(hipGetLastError());
   }
   // 2. the runs sorted by key (stable), in the fewest digits of at most 11 bits
@@ -519,26 +551,25 @@ int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
     for (int pass = 0; pass < npass; ++pass) {
       const int shift = pass * lo, width = std::min(lo, bits - shift);
       int rc;
-      if (width <= 8) rc = radix_pass<256>(ctx, cur, pass, shift, M, nb);
-      else if (width <= 9) rc = radix_pass<512>(ctx, cur, pass, shift, M, nb);
-      else if (width <= 10) rc = radix_pass<1024>(ctx, cur, pass, shift, M, nb);
-      else rc = radix_pass<2048>(ctx, cur, pass, shift, M, nb);
+      if (width <= 8) rc = radix_pass<256>(ctx, W, cur, pass, shift, M, nb);
+      else if (width <= 9) rc = radix_pass<512>(ctx, W, cur, pass, shift, M, nb);
+      else if (width <= 10) rc = radix_pass<1024>(ctx, W, cur, pass, shift, M, nb);
+      else rc = radix_pass<2048>(ctx, W, cur, pass, shift, M, nb);
       if (rc) return rc;
       cur = 1 - cur;
     }
     // first output position of every run: scan of the counts in sorted order
-    const uint32_t* perm = ctx->s_val[cur];
-    uint32_t* off = ctx->s_val[1 - cur];
+    const uint32_t* perm = W.val[cur];
+    uint32_t* off = W.val[1 - cur];
     const dim3 gm((unsigned)((M + 255) / 256));
-    hipLaunchKernelGGL(k_runs_sorted_counts, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)ctx->s_rcntd, off,
-                       M);
+    hipLaunchKernelGGL(k_runs_sorted_counts, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)W.rcntd, off, M);
     LEPIC_HIP(hipGetLastError());
-    if (int rc = scan_exclusive_u32(ctx, off, M)) return rc;
-    hipLaunchKernelGGL(k_runs_offsets, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)off, ctx->s_rkey, M);
+    if (int rc = scan_exclusive_u32(ctx, off, M, W.scan_ws)) return rc;
+    hipLaunchKernelGGL(k_runs_offsets, gm, dim3(256), 0, ctx->stream, perm, (const uint32_t*)off, W.rkey, M);
     LEPIC_HIP(hipGetLastError());
     // 3. the payload, once
     SegScatterArgs S{};
-    S.seg = s.seg[b]; S.meta = ctx->s_meta; S.rbase = ctx->s_nrun; S.roff = ctx->s_rkey;
+    S.seg = s.seg[b]; S.meta = W.meta; S.rbase = W.nrun; S.roff = W.rkey;
     for (int k = 0; k < 6; ++k) {
       S.a[k] = s.d[b][k]; S.b[k] = s.d[o][k];
       S.f[k] = s.f[b][k]; S.g[k] = s.f[o][k];
@@ -560,15 +591,82 @@ int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
     if (err) return ctx->fail(LEPIC_ERR_STATE, "cell sort: inconsistent runs");
   }
   if (Plist_host) {
-    uint32_t* plist = (uint32_t*)ctx->d_plist;
     hipLaunchKernelGGL(k_plist_from_runs, dim3((unsigned)((M + 1 + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (const uint32_t*)ctx->s_key[cur], (const uint32_t*)ctx->s_val[1 - cur], M, pl_max,
-                       (const int64_t*)(ctx->d_counts + p), plist);
+                       (const uint32_t*)W.key[cur], (const uint32_t*)W.val[1 - cur], M, pl_max,
+                       (const int64_t*)(ctx->d_counts + p), W.plist);
     LEPIC_HIP(hipGetLastError());
-    LEPIC_HIP(hipMemcpyAsync(Plist_host, plist, (pl_max + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
+    LEPIC_HIP(hipMemcpyAsync(Plist_host, W.plist, (pl_max + 1) * 4, hipMemcpyDeviceToHost, ctx->stream));
     LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   }
   return 0;
+}
+
+int sort_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
+  Species& s = ctx->sp[p];
+  const int64_t ncell = (int64_t)ctx->g.n1 * ctx->g.n2;
+  if (ncell >= (1ll << 31)) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "2^31 cells or more");
+  const int64_t pl_max = (int64_t)ctx->g.n1 + ncell + 1;
+  const int64_t nseg = std::min<int64_t>(nseg_of(s), s.nseg_cap);
+  const int64_t nslots = std::max<int64_t>(nseg, 1) * kChunk;
+  if (int rc = ensure_sort_scratch(ctx, nslots, std::max<int64_t>((nslots + kTile - 1) / kTile, 1),
+                                   Plist_host ? pl_max + 1 : 0))
+    return rc;
+  SortWork W{};
+  for (int k = 0; k < 2; ++k) { W.key[k] = ctx->s_key[k]; W.val[k] = ctx->s_val[k]; }
+  W.meta = ctx->s_meta; W.rkey = ctx->s_rkey; W.rcnt = ctx->s_rcnt; W.rcntd = ctx->s_rcntd;
+  W.nrun = ctx->s_nrun; W.hist = ctx->s_hist; W.plist = (uint32_t*)ctx->d_plist; W.scan_ws = nullptr;
+  return sort_core(ctx, p, Plist_host, W, nseg, pl_max);
+}
+
+int sort_scratch_resolve(lepic_ctx* ctx) {
+  // one lookup per device, kept for the process (each device holds its own
+  // copy of the array at its own address)
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  int dev = -1;
+  LEPIC_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(dev);
+  if (it == addr.end()) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return ctx->fail(LEPIC_ERR_HIP, std::string("hipGetSymbolAddress(sort scratch): ") + hipGetErrorString(e));
+    it = addr.emplace(dev, ptr).first;
+  }
+  ctx->sort_static = it->second;
+  return 0;
+}
+
+int sort_noalloc_launch(lepic_ctx* ctx, int p, int32_t* Plist_host) {
+  Species& s = ctx->sp[p];
+  if (!ctx->sort_static) return ctx->fail(LEPIC_ERR_STATE, "static sort scratch not resolved on this device");
+  const int64_t ncell = (int64_t)ctx->g.n1 * ctx->g.n2;
+  const int64_t pl_max = (int64_t)ctx->g.n1 + ncell + 1;
+  const int64_t nseg = std::min<int64_t>(nseg_of(s), s.nseg_cap);
+  if (nseg > kStaticSeg)
+    return ctx->fail(LEPIC_ERR_CAPACITY, "lepic_part_sorting_noalloc: the species spans " + std::to_string(nseg) +
+                                             " segments of 1024 slots, the static scratch holds " +
+                                             std::to_string(kStaticSeg) + " (524288 particles)");
+  if (Plist_host && pl_max + 1 > kStaticPlist)
+    return ctx->fail(LEPIC_ERR_CAPACITY, "lepic_part_sorting_noalloc: the cell-end list needs " +
+                                             std::to_string(pl_max + 1) + " entries, the static scratch holds " +
+                                             std::to_string(kStaticPlist));
+  uint32_t* w = (uint32_t*)ctx->sort_static;
+  SortWork W{};
+  for (int k = 0; k < 2; ++k) {
+    W.key[k] = w; w += kStaticSlots;
+    W.val[k] = w; w += kStaticSlots;
+  }
+  W.meta = w; w += kStaticSlots;
+  W.rkey = w; w += kStaticSlots;
+  W.rcnt = w; w += kStaticSlots;
+  W.rcntd = w; w += kStaticSlots;
+  W.nrun = w; w += kStaticSeg + 2;
+  W.hist = w; w += kStaticHist;
+  W.plist = w; w += kStaticPlist;
+  W.scan_ws = w;
+  return sort_core(ctx, p, Plist_host, W, nseg, pl_max);
 }
 
 }  // namespace lepic
