@@ -8,11 +8,16 @@
 // 495-503) by a block ballot scan into the same segment of the other particle
 // buffer; the segments' contents in segment order are the species'
 // particles, so no prefix across blocks is needed and blocks never wait for
-// each other.  Deposition merges equal cells within a thread and in runs of
-// adjacent lanes (segmented DPP sums, a fixed order), accumulates in an LDS
-// window of 16 rows x 64 columns of nodes around the segment's first cell,
-// and flushes the window to the species' global accumulator; nodes outside
-// the window go to the accumulator directly.  Window and accumulator hold
+// each other.  The mixed mover reads E, B and the cell wall flags from a
+// field tile of 10 x 32 nodes staged in LDS around the segment's first cell
+// (global memory for particles outside it).  Deposition accumulates in an
+// LDS window of 16 rows x 64 columns of nodes around the segment's first
+// cell and flushes the window to the species' global accumulator; nodes
+// outside the window go to the accumulator directly.  While a segment's
+// particles are in cell order, equal cells are merged first within a thread
+// and in runs of adjacent lanes (segmented DPP sums); once pushes have
+// scattered them, each particle's weights go to the window directly.
+// Window and accumulator hold
 // fixed-point integers (CIC weight x 2^kFixShift64 in FP64 contexts,
 // x 2^kFixShiftMx in mixed ones): integer addition is associative, so the
 // density is the same bit for bit whatever order the atomics land in, the
@@ -42,6 +47,17 @@ constexpr int kWinCols = 64;
 constexpr int kWinX0 = 8;
 constexpr int kWinY0 = 7;
 constexpr int kWin = kWinRows * kWinCols;
+// Field tile of the mixed mover: E, B and the cell wall flags of kTileRows x
+// kTileCols nodes staged in LDS per segment, from kTileX0 columns left and
+// kTileY0 rows below the segment's first cell.  A sorted segment of 1024
+// particles at 100 per cell spans about 10 cells of one row, and particles
+// drift a few cells between sorts; particles whose cell leaves the tile
+// gather from global memory instead.
+constexpr int kTileRows = 10;
+constexpr int kTileCols = 32;
+constexpr int kTileX0 = 6;
+constexpr int kTileY0 = 4;
+constexpr int kFieldTile = kTileRows * kTileCols;
 // Fixed-point units of the deposition: a particle's four CIC weights sum to
 // 1.  FP64 contexts round each run sum to 2^-42 of a particle (an absolute
 // error below 1.2e-13 of one particle's charge per run; a node holds up to
@@ -438,94 +454,6 @@ __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const 
   }
 }
 
-// Block-level counting sort of the segment's items by window-local cell
-// before the deposition (LDS: histogram, exclusive scan, stable-per-run
-// scatter), so that equal cells sit in adjacent lanes and slots whatever
-// order the particles are in; the particles themselves keep their order.
-// Pays off once pushes have scattered a sorted segment (electrons between
-// sorts); k_mover_mixed chooses it per block from the run count.
-// key[k]: window-local cell, -1 for no item; s_hist: kWin words of LDS.
-template <int BT>
-__device__ __forceinline__ void deposit_sorted(const MoverCommon& a, int* win, int wx0, int wy0,
-                                               const int key[kItems], const float FX[kItems],
-                                               const float FY[kItems], uint32_t* s_hist) {
-  static_assert(kWin == 4 * BT, "one thread per four window cells");
-  __shared__ float s_fx[BT * kItems], s_fy[BT * kItems];
-  __shared__ uint16_t s_key[BT * kItems];
-  __shared__ uint32_t s_tot[BT / 64];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int e = t; e < kWin; e += BT) s_hist[e] = 0u;
-  uint64_t heads[kItems];
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const int kk = key[k] >= 0 ? key[k] : -2 - lane;
-    const int kp = __shfl_up(kk, 1, 64);
-    heads[k] = __ballot(lane == 0 || kp != kk);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const uint64_t above = heads[k] & ~((2ull << lane) - 1ull);
-    const int next = above ? __builtin_ctzll(above) : 64;
-    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) atomicAdd(&s_hist[key[k]], (uint32_t)(next - lane));
-  }
-  __syncthreads();
-  const uint32_t h0 = s_hist[4 * t], h1 = s_hist[4 * t + 1], h2 = s_hist[4 * t + 2], h3 = s_hist[4 * t + 3];
-  const uint32_t mine = h0 + h1 + h2 + h3;
-  uint32_t incl = mine;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += o;
-  }
-  if (lane == 63) s_tot[wv] = incl;
-  __syncthreads();
-  uint32_t base = incl - mine, M = 0;
-#pragma unroll
-  for (int v = 0; v < BT / 64; ++v) {
-    const uint32_t c = s_tot[v];
-    if (v < wv) base += c;
-    M += c;
-  }
-  s_hist[4 * t] = base;
-  s_hist[4 * t + 1] = base + h0;
-  s_hist[4 * t + 2] = base + h0 + h1;
-  s_hist[4 * t + 3] = base + h0 + h1 + h2;
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const uint64_t above = heads[k] & ~((2ull << lane) - 1ull);
-    const int next = above ? __builtin_ctzll(above) : 64;
-    uint32_t start = 0;
-    if (key[k] >= 0 && ((heads[k] >> lane) & 1ull)) start = atomicAdd(&s_hist[key[k]], (uint32_t)(next - lane));
-    const int s = 63 - __builtin_clzll(heads[k] & ((2ull << lane) - 1ull));
-    const uint32_t st = __shfl(start, s, 64);
-    if (key[k] >= 0) {
-      const uint32_t pos = st + (uint32_t)(lane - s);
-      s_fx[pos] = FX[k];
-      s_fy[pos] = FY[k];
-      s_key[pos] = (uint16_t)key[k];
-    }
-  }
-  __syncthreads();
-  int C2[kItems];
-  float FX2[kItems], FY2[kItems];
-  bool ok2[kItems];
-#pragma unroll
-  for (int k = 0; k < kItems; ++k) {
-    const uint32_t j = (uint32_t)(kItems * t + k);
-    ok2[k] = j < M;
-    C2[k] = 0; FX2[k] = 0.f; FY2[k] = 0.f;
-    if (ok2[k]) {
-      const int kk = s_key[j];
-      C2[k] = (((kk / kWinCols) + wy0) << 16) | ((kk % kWinCols) + wx0);
-      FX2[k] = s_fx[j];
-      FY2[k] = s_fy[j];
-    }
-  }
-  deposit_tmajor(a, win, wx0, wy0, C2, FX2, FY2, ok2);
-}
-
 // ---------------------------------------------------------------------------
 // FP64 mover
 // ---------------------------------------------------------------------------
@@ -693,16 +621,21 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
                                                         const float* __restrict__ E,
                                                         const float* __restrict__ B) {
   __shared__ int win[kWin];
-  __shared__ uint32_t s_hist[kWin];
+  __shared__ float2 sE[kFieldTile];
+  __shared__ float4 sB[kFieldTile];
+  __shared__ int8_t sW[kFieldTile];
   const int64_t bid = blockIdx.x;
   const int cnt_in = a.seg_in[bid];
   const int64_t base = bid * (int64_t)(BT * kItems);
   const int t = threadIdx.x;
   const int n1 = a.n1, n2 = a.n2;
   const float k1 = m.k1f;
-  // window anchor from the segment's first cell, computed by every thread
+  // window anchors from the segment's first cell, computed by every thread
   const int c0 = cnt_in > 0 ? in.cell[base] : ((1 << 16) | 1);
   const int wx0 = max((c0 & 0xFFFF) - kWinX0, 0), wy0 = max((c0 >> 16) - kWinY0, 0);
+  const int fx0 = max(min((c0 & 0xFFFF) - kTileX0, n1 + 3 - kTileCols), 0);
+  const int fy0 = max(min((c0 >> 16) - kTileY0, n2 + 3 - kTileRows), 0);
+  const bool tileB = a.flagB && !a.Bconst && !a.gridB;
   if (a.deposit)
     for (int e = t; e < kWin; e += BT) win[e] = 0;
   // thread t owns the kItems consecutive slots i0 .. i0+3 of the segment;
@@ -729,6 +662,23 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     VZ[0] = vz4.x; VZ[1] = vz4.y; VZ[2] = vz4.z; VZ[3] = vz4.w;
     Z[0] = z4.x; Z[1] = z4.y; Z[2] = z4.z; Z[3] = z4.w;
   }
+  // the field tile (nodes beyond the grid of a small domain are never read)
+  for (int e = t; e < kFieldTile; e += BT) {
+    const int gx = fx0 + e % kTileCols, gy = fy0 + e / kTileCols;
+    float2 ev = make_float2(0.f, 0.f);
+    float4 bv = make_float4(0.f, 0.f, 0.f, 0.f);
+    int8_t wv = 1;
+    if (gx <= n1 + 2 && gy <= n2 + 2) {
+      const int64_t nd = node_index(gx, gy, n1);
+      ev = *(const float2*)(E + 2 * nd);
+      if (tileB) bv = *(const float4*)(B + 4 * nd);
+      wv = a.cellwall[nd];
+    }
+    sE[e] = ev;
+    sB[e] = bv;
+    sW[e] = wv;
+  }
+  __syncthreads();
   // particles flagged dead by the MCC leave before the push
   // (part_expmover.f90:84-98); rare, so one branch per thread
   uint32_t deadm = 0;
@@ -755,9 +705,18 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     clamp_cell(ix, iy, n1, n2);
     const float px = 1.0f - fx, py = 1.0f - fy;
     const float w0 = px * py, w1 = fx * py, w2 = fx * fy, w3 = px * fy;
-    const int64_t n00 = node_index(ix, iy, n1), n01 = node_index(ix, iy + 1, n1);
-    const float2 e00 = *(const float2*)(E + 2 * n00), e10 = *(const float2*)(E + 2 * (n00 + 1));
-    const float2 e11 = *(const float2*)(E + 2 * (n01 + 1)), e01 = *(const float2*)(E + 2 * n01);
+    // the cell's four nodes from the tile when it holds them
+    const int tx = ix - fx0, ty = iy - fy0;
+    const bool in_tile = (unsigned)tx < (unsigned)(kTileCols - 1) && (unsigned)ty < (unsigned)(kTileRows - 1);
+    const int q = ty * kTileCols + tx;
+    float2 e00, e10, e11, e01;
+    if (in_tile) {
+      e00 = sE[q]; e10 = sE[q + 1]; e11 = sE[q + kTileCols + 1]; e01 = sE[q + kTileCols];
+    } else {
+      const int64_t n00 = node_index(ix, iy, n1), n01 = node_index(ix, iy + 1, n1);
+      e00 = *(const float2*)(E + 2 * n00); e10 = *(const float2*)(E + 2 * (n00 + 1));
+      e11 = *(const float2*)(E + 2 * (n01 + 1)); e01 = *(const float2*)(E + 2 * n01);
+    }
     float Ex = fmaf(w3, e01.x, fmaf(w2, e11.x, fmaf(w1, e10.x, w0 * e00.x)));
     float Ey = fmaf(w3, e01.y, fmaf(w2, e11.y, fmaf(w1, e10.y, w0 * e00.y)));
     bool boris = false;
@@ -766,6 +725,14 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
       if (a.Bconst) {
         Bx = (float)a.Bc0; By = (float)a.Bc1; Bz = (float)a.Bc2;
         boris = true;
+      } else if (tileB && in_tile) {
+        const float4 B00 = sB[q], B10 = sB[q + 1], B11 = sB[q + kTileCols + 1], B01 = sB[q + kTileCols];
+        if (B00.w > 1.0e-5f) {
+          Bx = fmaf(w3, B01.x, fmaf(w2, B11.x, fmaf(w1, B10.x, w0 * B00.x)));
+          By = fmaf(w3, B01.y, fmaf(w2, B11.y, fmaf(w1, B10.y, w0 * B00.y)));
+          Bz = fmaf(w3, B01.z, fmaf(w2, B11.z, fmaf(w1, B10.z, w0 * B00.z)));
+          boris = true;
+        }
       } else {
         int bx = ix, by = iy;
         float b0 = w0, b1 = w1, b2 = w2, b3 = w3;
@@ -820,7 +787,11 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     iy += (int)sy;
     int cx = ix, cy = iy;
     clamp_cell(cx, cy, n1, n2);
-    const bool lost = live && a.cellwall[node_index(cx, cy, n1)] != 0;  // part_expmover.f90:254-258
+    // part_expmover.f90:254-258, from the tile's wall flags when it holds the cell
+    const int wx = cx - fx0, wy = cy - fy0;
+    const bool w_tile = (unsigned)wx < (unsigned)kTileCols && (unsigned)wy < (unsigned)kTileRows;
+    const int8_t wall = w_tile ? sW[wy * kTileCols + wx] : a.cellwall[node_index(cx, cy, n1)];
+    const bool lost = live && wall != 0;
     lostm |= lost ? 1u << k : 0u;
     IXU[k] = ix; IYU[k] = iy;
     C[k] = (iy << 16) | ix;
@@ -885,9 +856,22 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
   }
   if (!a.deposit) return;
   deposit_global_runs(a, gkey, FX, FY);
-  if (4 * block_runs > kept) {
-    // fewer than four items per run on average: sort the items by cell first
-    deposit_sorted<BT>(a, win, wx0, wy0, dkey, FX, FY, s_hist);
+  if (2 * block_runs > kept) {
+    // fewer than two items per run on average (pushes have scattered the
+    // segment): every item's fixed-point weights straight to the window,
+    // where LDS atomics now rarely collide; merging runs first pays only
+    // while the particles are in cell order
+#pragma unroll
+    for (int k = 0; k < kItems; ++k) {
+      if (dkey[k] < 0) continue;
+      int q0, q1, q2, q3;
+      cic_fix(FX[k], FY[k], q0, q1, q2, q3);
+      const int kk = dkey[k];
+      atomicAdd(&win[kk], q0);
+      atomicAdd(&win[kk + 1], q1);
+      atomicAdd(&win[kk + kWinCols + 1], q2);
+      atomicAdd(&win[kk + kWinCols], q3);
+    }
   } else {
     bool ok[kItems];
 #pragma unroll
