@@ -40,17 +40,19 @@ namespace {
 constexpr int kVisitLog = 2 * kMaxLevels;     // ig = 1 .. 2ng-1
 constexpr size_t kTailLds = 160 * 1024 - 2048;  // dynamic LDS of k_tail (static arrays take < 2 KB)
 constexpr int kTailThreads = 1024;
-constexpr int kS = kSweepsPerLaunch;
-constexpr int kHalo = 2 * kS + 1;             // validity shrinks by one node per colour pass
+constexpr int kS = kSweepsPerLaunch;          // most sweeps of one launch (kRot - 1 state buffers ahead)
+// halo of a launch of at most KS sweeps: validity shrinks by one node per colour pass
+template <int KS> struct Halo { static constexpr int v = 2 * KS + 1; };
 constexpr int kTX = 64;                       // wide tiles: 64 x TileY nodes, 1024 threads
 constexpr int kSmallTX = 32;                  // small tiles: 32 x SmallTileY nodes, 512 threads
 constexpr int kSmallThreads = 512;
 constexpr int kSmallTileBelow = 64;           // a level with fewer wide tiles than this takes small ones
 constexpr int kSweepThreads = 1024;
 constexpr int kDefaultTailNodes = 5000;
-constexpr int kResidBlocks = 128;             // blocks of the FP64 finest-level residual
+constexpr int kResidBlocks = 512;             // blocks of the FP64 finest-level residual (their partial sums fit the pinned read-back)
 // byte offset of the residual partials behind the scalars, visit log and status
 constexpr size_t kPartialsOff = (64 * 8 + (kVisitLog + 1) * sizeof(int) + 7) / 8 * 8;
+static_assert(kPartialsOff - 32 * 8 + kResidBlocks * 8 <= kPinnedDoubles * 8, "read-back fits the pinned buffer");
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
@@ -238,10 +240,13 @@ struct SweepArgs {
   double eps;
 };
 
-// Sweeps k0+1 .. k0+kexec of a visit on one TX x TY tile with NT threads
-// (see the file comment).
-template <typename T, int kTX, int TY, int kSweepThreads>
-__global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec) {
+// Sweeps k0+1 .. k0+kexec (kexec <= KS) of a visit on one TX x TY tile with
+// NT threads (see the file comment).  A launch that continues a visit first
+// takes the reference's exit test for the sweeps of the visit's previous
+// launch, kchk .. k0.
+template <typename T, int kTX, int TY, int kSweepThreads, int KS>
+__global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec, int kchk) {
+  constexpr int kHalo = Halo<KS>::v;
   constexpr int RW = kTX + 2 * kHalo, RH = TY + 2 * kHalo, RN = RW * RH;
   constexpr int HW = (RW - 2) / 2;  // nodes of one colour per inner region row
   static_assert((RW & 1) == 0, "region width must be even");
@@ -260,7 +265,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
       go = 0;
     } else if (k0 > 0) {
       double r = 0.0;
-      const int ks = first_stop(A.partials, nb, k0 - kS + 1, k0, A.dig, A.eps, A.nr, &r);
+      const int ks = first_stop(A.partials, nb, kchk, k0, A.dig, A.eps, A.nr, &r);
       if (ks) {
         go = 0;
         if (bid == 0 && threadIdx.x == 0) {
@@ -1058,7 +1063,8 @@ int* d_visit_of(lepic_ctx* ctx) { return (int*)(ctx->d_scalar + 64); }
 double* d_vres_of(lepic_ctx* ctx) { return ctx->d_scalar + 32; }
 int* d_status_of(lepic_ctx* ctx) { return (int*)(ctx->d_scalar + 64) + kVisitLog; }
 
-VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, double eps, int nexec) {
+VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, double eps, int nexec,
+                   int klast0 = -1) {
   Level& L = ctx->lv[l];
   VisitRef v;
   v.ctrl = ctx->d_ctrl + l;
@@ -1066,15 +1072,39 @@ VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, do
   v.nb = L.nbx * L.nby;
   v.epoch = epoch; v.dig = dig; v.nr = nr; v.ig = ig; v.eps = eps;
   v.nexec = nexec;
-  v.klast0 = ((nexec - 1) / kS) * kS;
+  v.klast0 = klast0 >= 0 ? klast0 : ((nexec - 1) / kS) * kS;
   v.visit = d_visit_of(ctx);
   v.vres = d_vres_of(ctx);
   return v;
 }
 
-// launches of one visit of a multi-block level: sweeps k0+1 .. k_end
+// Start of the last launch of a visit whose launches cover sweeps
+// k0+1 .. k_end, the first of them `first` sweeps and the others kS.
+int last_launch_start(int k0, int k_end, int first) {
+  int k = k0, last = k0;
+  for (int size = std::min(first, kS); k < k_end; size = kS) {
+    last = k;
+    k += std::min(size, k_end - k);
+  }
+  return last;
+}
+
+template <typename T, int kTX, int TY, int NT>
+void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k, int kexec, int kchk) {
+  const dim3 g(L.nbx, L.nby);
+  if (kexec <= 2) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 2>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+  else if (kexec <= 3) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 3>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+  else hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, kS>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+}
+
+// launches of one visit of a multi-block level: sweeps k0+1 .. k_end, the
+// first launch `first` sweeps (the number the same visit took in the
+// previous V-cycle: most visits then finish in one launch of the smallest
+// halo), the others kS; each continuation first tests the previous launch's
+// sweeps and returns at once when the visit has stopped there
 template <typename T>
-int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end) {
+int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end,
+                 int first = kS) {
   Level& L = ctx->lv[l];
   SweepArgs<T> A;
   for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<T>(L, r);
@@ -1086,14 +1116,13 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.partials = L.partials;
   A.nbx = L.nbx; A.nby = L.nby;
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
-  for (int k = k0; k < k_end; k += kS) {
-    const int kexec = std::min(kS, k_end - k);
-    if (L.small_tiles)
-      hipLaunchKernelGGL((k_sweeps<T, kSmallTX, SmallTileY<T>::v, kSmallThreads>), dim3(L.nbx, L.nby),
-                         dim3(kSmallThreads), 0, ctx->stream, A, k, kexec);
-    else
-      hipLaunchKernelGGL((k_sweeps<T, kTX, TileY<T>::v, kSweepThreads>), dim3(L.nbx, L.nby), dim3(kSweepThreads),
-                         0, ctx->stream, A, k, kexec);
+  int kchk = 0;
+  for (int k = k0, size = std::min(std::max(first, 1), kS); k < k_end; size = kS) {
+    const int kexec = std::min(size, k_end - k);
+    if (L.small_tiles) launch_sweeps<T, kSmallTX, SmallTileY<T>::v, kSmallThreads>(ctx, L, A, k, kexec, kchk);
+    else launch_sweeps<T, kTX, TileY<T>::v, kSweepThreads>(ctx, L, A, k, kexec, kchk);
+    kchk = k + 1;
+    k += kexec;
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
@@ -1105,7 +1134,7 @@ int launch_restrict(lepic_ctx* ctx, int l, bool f32, int epoch, double eps) {
   Level& F = ctx->lv[l];
   Level& C = ctx->lv[l + 1];
   RestrictArgs<T> A;
-  A.vf = visit_ref(ctx, l, epoch, l - ng, kNr, l, eps, kNr);
+  A.vf = visit_ref(ctx, l, epoch, l - ng, kNr, l, eps, kNr, last_launch_start(0, kNr, F.first_dn));
   for (int r = 0; r < kRot; ++r) A.frot[r] = rot_of<T>(F, r);
   A.fb = b_of<T>(F, f32);
   A.fbc = F.bc;
@@ -1135,7 +1164,9 @@ int launch_prolong(lepic_ctx* ctx, int l, bool f32, int epoch_up, double eps) {
   A.n1 = F.n1; A.n2 = F.n2;
   for (int r = 0; r < kRot; ++r) A.crot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
   A.coarse_mb = C.mb ? 1 : 0;
-  if (C.mb) A.vc = visit_ref(ctx, l + 1, epoch_up, ng - (l + 1), kNr, 2 * ng - (l + 1), eps, kNr);
+  if (C.mb)
+    A.vc = visit_ref(ctx, l + 1, epoch_up, ng - (l + 1), kNr, 2 * ng - (l + 1), eps, kNr,
+                     last_launch_start(0, kNr, C.first_up));
   else A.vc = VisitRef{};
   dim3 g((C.n1 + 1 + 127) / 128, std::min(C.n2 + 1, kRowBlocks));
   hipLaunchKernelGGL(k_prolong<T>, g, dim3(128), 0, ctx->stream, A);
@@ -1216,20 +1247,20 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
   const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
   ctx->mg_epoch += 2;
   for (int l = 1; l < Ls; ++l) {  // down leg
-    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, kNr)) return rc;
+    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, kNr, ctx->lv[l].first_dn)) return rc;
     if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
   }
   if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
   for (int l = Ls - 1; l >= 1; --l) {  // up leg
     if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
-    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, kNr)) return rc;
+    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, kNr, ctx->lv[l].first_up)) return rc;
   }
   Level& L1 = ctx->lv[1];
   const unsigned gx = (unsigned)((L1.n1 + 1 + 255) / 256);
   if (!f32) {
     if (L1.mb) {
       Finish64Args A;
-      A.v = visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr);
+      A.v = visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr, last_launch_start(0, kNr, L1.first_up));
       for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<double>(L1, r);
       A.n1 = L1.n1; A.n2 = L1.n2;
       A.status = d_status_of(ctx);
@@ -1242,7 +1273,8 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     A.bc = ctx->bc;
     A.n1 = L1.n1; A.n2 = L1.n2;
     A.mb = L1.mb ? 1 : 0;
-    A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr) : VisitRef{};
+    A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr, last_launch_start(0, kNr, L1.first_up))
+                : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
   }
   LEPIC_HIP(hipGetLastError());
@@ -1499,6 +1531,13 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
       vres[2 * ng - 1] = rs;
     }
     const int nvis = 2 * ng - 1;
+    if (ng > 1)
+      for (int l = 1; l < ctx->Lc; ++l) {
+        Level& L = ctx->lv[l];
+        if (!L.mb) continue;
+        L.first_dn = std::min(std::max(visit[l], 1), kS);
+        L.first_up = std::min(std::max(visit[2 * ng - l], 1), kS);
+      }
     for (int ig = 1; ig <= nvis; ++ig) {
       int l = ig < ng ? ig : (ig > ng ? 2 * ng - ig : ng);
       ktot += (double)visit[ig] / std::pow(2.0, 2 * (l - 1));
