@@ -50,7 +50,7 @@ struct Level {
   void* rot[kRot] = {};
   int nbx = 0, nby = 0;
   double* partials = nullptr;
-  // sweeps of the first launch of the level's down and up visits: the
+  // sweeps of the first launch of the level's down and up visits, from the
   // number each visit took in the previous V-cycle
   int first_dn = kSweepsPerLaunch, first_up = kSweepsPerLaunch;
 };

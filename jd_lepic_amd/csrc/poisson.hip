@@ -1098,8 +1098,8 @@ void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k,
 }
 
 // launches of one visit of a multi-block level: sweeps k0+1 .. k_end, the
-// first launch `first` sweeps (the number the same visit took in the
-// previous V-cycle: most visits then finish in one launch of the smallest
+// first launch `first` sweeps (one more than the same visit took in the
+// previous V-cycle: most visits then finish in one launch of a small
 // halo), the others kS; each continuation first tests the previous launch's
 // sweeps and returns at once when the visit has stopped there
 template <typename T>
@@ -1535,8 +1535,10 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
       for (int l = 1; l < ctx->Lc; ++l) {
         Level& L = ctx->lv[l];
         if (!L.mb) continue;
-        L.first_dn = std::min(std::max(visit[l], 1), kS);
-        L.first_up = std::min(std::max(visit[2 * ng - l], 1), kS);
+        // one sweep more than last time: a spare sweep costs less than a
+        // continuation launch when the count grows by one
+        L.first_dn = std::min(std::max(visit[l] + 1, 2), kS);
+        L.first_up = std::min(std::max(visit[2 * ng - l] + 1, 2), kS);
       }
     for (int ig = 1; ig <= nvis; ++ig) {
       int l = ig < ng ? ig : (ig > ng ? 2 * ng - ig : ng);

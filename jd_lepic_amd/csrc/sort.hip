@@ -397,26 +397,68 @@ struct SegScatterArgs {
 
 template <bool kMixed>
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(SegScatterArgs A) {
+  // kSegRounds slots per thread (slot t + r*kSegThreads, so every access of a
+  // round is coalesced); the payload and run loads of all rounds are issued
+  // before any store
+  constexpr int R = kSegRounds;
   const int64_t sg = blockIdx.x;
   const int cnt = A.seg[sg];
   const int64_t base = sg * (int64_t)kChunk;
   const uint32_t rb = A.rbase[sg];
-  for (int j = threadIdx.x; j < cnt; j += kSegThreads) {
-    const int64_t i = base + j;
-    const uint32_t m = A.meta[i];
-    const int64_t o = (int64_t)A.roff[rb + (m >> kRankBits)] + (m & (kChunk - 1));
-    if (o >= A.cap) {        // cannot happen for consistent runs; never write past the buffer
-      *A.err = 1;
-      continue;
+  const int t = threadIdx.x;
+  bool v[R];
+  uint32_t m[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int j = t + r * kSegThreads;
+    v[r] = j < cnt;
+    m[r] = v[r] ? A.meta[base + j] : 0u;
+  }
+  if (kMixed) {
+    int32_t c[R];
+    float f[6][R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = base + t + r * kSegThreads;
+      c[r] = v[r] ? A.ci[i] : 0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) f[k][r] = (v[r] && (k != 2 || A.g[2])) ? A.f[k][i] : 0.f;
     }
-    if (kMixed) {
-      A.co[o] = A.ci[i];
+    int64_t o[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) o[r] = v[r] ? (int64_t)A.roff[rb + (m[r] >> kRankBits)] + (m[r] & (kChunk - 1)) : 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!v[r]) continue;
+      if (o[r] >= A.cap) {        // cannot happen for consistent runs; never write past the buffer
+        *A.err = 1;
+        continue;
+      }
+      A.co[o[r]] = c[r];
 #pragma unroll
       for (int k = 0; k < 6; ++k)
-        if (k != 2 || A.g[2]) A.g[k][o] = A.f[k][i];
-    } else {
+        if (k != 2 || A.g[2]) A.g[k][o[r]] = f[k][r];
+    }
+  } else {
+    double d[6][R];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) A.b[k][o] = A.a[k][i];
+    for (int r = 0; r < R; ++r) {
+      const int64_t i = base + t + r * kSegThreads;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) d[k][r] = v[r] ? A.a[k][i] : 0.0;
+    }
+    int64_t o[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) o[r] = v[r] ? (int64_t)A.roff[rb + (m[r] >> kRankBits)] + (m[r] & (kChunk - 1)) : 0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (!v[r]) continue;
+      if (o[r] >= A.cap) {
+        *A.err = 1;
+        continue;
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) A.b[k][o[r]] = d[k][r];
     }
   }
 }
