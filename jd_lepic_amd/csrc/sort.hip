@@ -89,10 +89,11 @@ struct SegSortArgs {
   uint32_t* nrun;             // [segment] runs of the segment
 };
 
-// y-major 0-based cell index of slot i: INT(x/h) with the clamp of
-// sorting.f90:147-151 (FP64), or the stored cell (mixed)
+// 0-based cell (ix, iy) of slot i: INT(x/h) with the clamp of
+// sorting.f90:147-151 (FP64), or the stored cell (mixed); the y-major sort
+// key is ix + n1*iy
 template <bool kMixed>
-__device__ __forceinline__ uint32_t cell_key(const SegSortArgs& A, int64_t i) {
+__device__ __forceinline__ void cell_of(const SegSortArgs& A, int64_t i, uint32_t& cx, uint32_t& cy) {
   int ix, iy;
   if (kMixed) {
     const int c = A.cell[i];
@@ -104,7 +105,8 @@ __device__ __forceinline__ uint32_t cell_key(const SegSortArgs& A, int64_t i) {
   }
   ix = ix < 0 ? 0 : (ix > A.n1 - 1 ? A.n1 - 1 : ix);
   iy = iy < 0 ? 0 : (iy > A.n2 - 1 ? A.n2 - 1 : iy);
-  return (uint32_t)ix + (uint32_t)A.n1 * (uint32_t)iy;
+  cx = (uint32_t)ix;
+  cy = (uint32_t)iy;
 }
 
 // exclusive prefix of v over the block's 256 threads in thread order; the
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
   __shared__ uint16_t s_i[2][kChunk];
   __shared__ uint32_t s_wc[4][256];
   __shared__ uint16_t s_head[kChunk + 1];
-  __shared__ uint32_t s_mm[2][4];
+  __shared__ uint32_t s_mm[4][4];
   __shared__ uint32_t s_tot[4];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int64_t sg = blockIdx.x;
@@ -147,32 +149,50 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
     if (t == 0) A.nrun[sg] = 0;
     return;
   }
-  // keys in slot order; wave w owns positions [256 w, 256 w + 256)
-  uint32_t kmin = 0xFFFFFFFFu, kmax = 0u;
+  // cells in slot order; wave w owns positions [256 w, 256 w + 256).  The
+  // segment is sorted by the compact key (iy - ymin) * dxs + (ix - xmin),
+  // dxs = xmax - xmin + 1, which orders cells as ix + n1*iy does but spans
+  // far fewer values when the segment's particles sit in a few short pieces
+  // of neighbouring rows (one 8-bit pass instead of two)
+  uint32_t X[kSegRounds], Y[kSegRounds];
+  uint32_t xmin = 0xFFFFFFFFu, xmax = 0u, ymin = 0xFFFFFFFFu, ymax = 0u;
 #pragma unroll
   for (int r = 0; r < kSegRounds; ++r) {
     const int q = w * 256 + r * 64 + lane;
+    X[r] = Y[r] = 0u;
     if (q < cnt) {
-      const uint32_t k = cell_key<kMixed>(A, base + q);
-      s_k[0][q] = k;
-      s_i[0][q] = (uint16_t)q;
-      kmin = min(kmin, k);
-      kmax = max(kmax, k);
+      cell_of<kMixed>(A, base + q, X[r], Y[r]);
+      xmin = min(xmin, X[r]); xmax = max(xmax, X[r]);
+      ymin = min(ymin, Y[r]); ymax = max(ymax, Y[r]);
     }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
-    kmin = min(kmin, (uint32_t)__shfl_xor((int)kmin, o, 64));
-    kmax = max(kmax, (uint32_t)__shfl_xor((int)kmax, o, 64));
+    xmin = min(xmin, (uint32_t)__shfl_xor((int)xmin, o, 64));
+    xmax = max(xmax, (uint32_t)__shfl_xor((int)xmax, o, 64));
+    ymin = min(ymin, (uint32_t)__shfl_xor((int)ymin, o, 64));
+    ymax = max(ymax, (uint32_t)__shfl_xor((int)ymax, o, 64));
   }
-  if (lane == 0) { s_mm[0][w] = kmin; s_mm[1][w] = kmax; }
+  if (lane == 0) { s_mm[0][w] = xmin; s_mm[1][w] = xmax; s_mm[2][w] = ymin; s_mm[3][w] = ymax; }
   __syncthreads();
-  kmin = min(min(s_mm[0][0], s_mm[0][1]), min(s_mm[0][2], s_mm[0][3]));
-  kmax = max(max(s_mm[1][0], s_mm[1][1]), max(s_mm[1][2], s_mm[1][3]));
-  const uint32_t span = kmax - kmin;
+  xmin = min(min(s_mm[0][0], s_mm[0][1]), min(s_mm[0][2], s_mm[0][3]));
+  xmax = max(max(s_mm[1][0], s_mm[1][1]), max(s_mm[1][2], s_mm[1][3]));
+  ymin = min(min(s_mm[2][0], s_mm[2][1]), min(s_mm[2][2], s_mm[2][3]));
+  ymax = max(max(s_mm[3][0], s_mm[3][1]), max(s_mm[3][2], s_mm[3][3]));
+  const uint32_t dxs = xmax - xmin + 1u;
+#pragma unroll
+  for (int r = 0; r < kSegRounds; ++r) {
+    const int q = w * 256 + r * 64 + lane;
+    if (q < cnt) {
+      s_k[0][q] = (Y[r] - ymin) * dxs + (X[r] - xmin);
+      s_i[0][q] = (uint16_t)q;
+    }
+  }
+  const uint32_t kmin = 0u, span = (ymax - ymin) * dxs + (xmax - xmin);
   int npass = 0;
   while (npass < 4 && (span >> (8 * npass)) != 0u) ++npass;
-  // stable LSD passes over the key relative to kmin
+  if (npass == 0) __syncthreads();   // block-uniform; the passes start with a barrier
+  // stable LSD passes over the compact key
   int cur = 0;
   for (int p = 0; p < npass; ++p) {
     const int shift = 8 * p;
@@ -244,7 +264,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
   }
   for (uint32_t r = t; r < nrun; r += kSegThreads) {
     const int h = s_head[r];
-    A.rkey[base + r] = s_k[cur][h];
+    const uint32_t ck = s_k[cur][h];
+    A.rkey[base + r] = (ck % dxs + xmin) + (uint32_t)A.n1 * (ck / dxs + ymin);
     A.rcnt[base + r] = (uint32_t)(s_head[r + 1] - h);
   }
   if (t == 0) A.nrun[sg] = nrun;
