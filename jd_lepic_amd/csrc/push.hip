@@ -46,7 +46,11 @@ constexpr int kWinRows = 16;
 constexpr int kWinCols = 64;
 constexpr int kWinX0 = 8;
 constexpr int kWinY0 = 7;
-constexpr int kWin = kWinRows * kWinCols;
+// rows are padded by one word: with a stride of 64 the nodes of one column
+// in different rows share an LDS bank, and the atomics of lanes in
+// neighbouring rows serialise
+constexpr int kWinStride = kWinCols + 1;
+constexpr int kWin = kWinRows * kWinStride;
 // Field tile of the mixed mover: E, B and the cell wall flags of kTileRows x
 // kTileCols nodes staged in LDS per segment, from kTileX0 columns left and
 // kTileY0 rows below the segment's first cell.  A sorted segment of 1024
@@ -132,10 +136,10 @@ __device__ __forceinline__ void deposit_cell64(const MoverCommon& a, unsigned lo
   const long long q[4] = {fix64(w[0]), fix64(w[1]), fix64(w[2]), fix64(w[3])};
   const int cx = ix - wx0, cy = iy - wy0;
   if (cx >= 0 && cx + 1 < kWinCols && cy >= 0 && cy + 1 < kWinRows) {
-    atomicAdd(&win[cy * kWinCols + cx], (unsigned long long)q[0]);
-    atomicAdd(&win[cy * kWinCols + cx + 1], (unsigned long long)q[1]);
-    atomicAdd(&win[(cy + 1) * kWinCols + cx + 1], (unsigned long long)q[2]);
-    atomicAdd(&win[(cy + 1) * kWinCols + cx], (unsigned long long)q[3]);
+    atomicAdd(&win[cy * kWinStride + cx], (unsigned long long)q[0]);
+    atomicAdd(&win[cy * kWinStride + cx + 1], (unsigned long long)q[1]);
+    atomicAdd(&win[(cy + 1) * kWinStride + cx + 1], (unsigned long long)q[2]);
+    atomicAdd(&win[(cy + 1) * kWinStride + cx], (unsigned long long)q[3]);
   } else {
     const int64_t n00 = node_index(ix, iy, a.n1), n01 = node_index(ix, iy + 1, a.n1);
     acc_add(&a.acc[n00], q[0]);
@@ -152,7 +156,7 @@ __device__ __forceinline__ void flush_window(const MoverCommon& a, const T* win,
   for (int e = threadIdx.x; e < kWin; e += blockDim.x) {
     const long long v = (long long)win[e];
     if (v != 0) {
-      const int ix = wx0 + e % kWinCols, iy = wy0 + e / kWinCols;
+      const int ix = wx0 + e % kWinStride, iy = wy0 + e / kWinStride;
       acc_add(&a.acc[node_index(ix, iy, a.n1)], v);
     }
   }
@@ -373,7 +377,7 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, i
       deposit_global(a, ix, iy, q0, q1, q2, q3);
       continue;
     }
-    const int kk = cy * kWinCols + cx;
+    const int kk = cy * kWinStride + cx;
     if (kk == prev) {
       // same cell as this thread's previous particle: add into its slot
 #pragma unroll
@@ -402,8 +406,8 @@ __device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, i
     if (tail && key[k] >= 0) {
       atomicAdd(&win[kk], v[0]);
       atomicAdd(&win[kk + 1], v[1]);
-      atomicAdd(&win[kk + kWinCols + 1], v[2]);
-      atomicAdd(&win[kk + kWinCols], v[3]);
+      atomicAdd(&win[kk + kWinStride + 1], v[2]);
+      atomicAdd(&win[kk + kWinStride], v[3]);
     }
   }
 }
@@ -824,7 +828,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
       clamp_cell(ix, iy, n1, n2);
       const int cx = ix - wx0, cy = iy - wy0;
       if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) gkey[k] = (iy << 16) | ix;
-      else dkey[k] = cy * kWinCols + cx;
+      else dkey[k] = cy * kWinStride + cx;
     }
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
@@ -869,8 +873,8 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
       const int kk = dkey[k];
       atomicAdd(&win[kk], q0);
       atomicAdd(&win[kk + 1], q1);
-      atomicAdd(&win[kk + kWinCols + 1], q2);
-      atomicAdd(&win[kk + kWinCols], q3);
+      atomicAdd(&win[kk + kWinStride + 1], q2);
+      atomicAdd(&win[kk + kWinStride], q3);
     }
   } else {
     bool ok[kItems];
