@@ -15,8 +15,9 @@ species (main.f90:689-1090 order).  The headline value is particles/s of
 push + deposit + MCC in the mixed-precision form with the learned warm start
 (FP32 push/deposit with fixed-point tile sums flushed to the FP64 grid, FP32
 MG smoothing in FP64 refinement, Poisson started from the network's guess
-when it lowers the initial residual; the network is trained on the run's own
-first solves, untimed); plain mixed and plain FP64 are measured on the same
+when it lowers the initial residual; the network is trained offline, untimed,
+on the (drhs, dphi) pairs of randomised 512 x 512 runs by
+jd_lepic_amd.warmstart); plain mixed and plain FP64 are measured on the same
 workload beside it.  Untimed beside the loop: the mixed form's densities,
 phi, E and moments against the FP64 oracle on the same state, and mixed
 against FP64 after ten steps.
@@ -85,14 +86,30 @@ class Hip:
         return nbytes
 
 
-def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_steps=40, train_iters=300):
-    """One mode of the step loop: learn_steps untimed steps, warmup untimed
-    steps, then `steps` timed steps, so every mode times the same steps of
-    the run.  With learned=True the first learn_steps solves record (drhs,
-    dphi) samples, the network is trained on them (its time reported apart),
-    and every later solve starts from the network's guess when that lowers
-    the initial residual (the library skips the network for a growing
-    back-off after a guess that gains too little)."""
+def offline_net(args):
+    """The learned warm start's network, trained offline (untimed) on
+    (drhs, dphi) pairs of 512 x 512 e-/Ar+ runs with randomised B peak and
+    density (jd_lepic_amd.warmstart); returns the sample store holding it and
+    a record of the regime."""
+    from jd_lepic_amd import warmstart as WS
+    t = time.perf_counter()
+    store, meta = WS.build_dataset(args.ws_samples, args.ws_runs, n=512, ppc=50)
+    t_data = time.perf_counter() - t
+    first, last, t_train = WS.train(store, args.ws_iters, batch=4, lr=2e-3)
+    rec = dict(samples=int(sum(m["samples"] for m in meta)), runs=len(meta), grid="512x512, 50 ppc",
+               bpeak_mT=[round(1e3 * min(m["bpeak_T"] for m in meta), 2), round(1e3 * max(m["bpeak_T"] for m in meta), 2)],
+               n0=[min(m["n0"] for m in meta), max(m["n0"] for m in meta)], dataset_s=t_data, train_iters=args.ws_iters,
+               train_s=t_train, loss_first=first, loss_last=last)
+    return store, rec
+
+
+def run_mode(W, precision, steps, warmup, hip, seed=7, net_store=None, learn_steps=40):
+    """One mode of the step loop: learn_steps + warmup untimed steps, then
+    `steps` timed steps, so every mode times the same steps of the run.  With
+    a trained network (net_store) every solve after the untimed steps starts
+    from the network's guess when that lowers the initial residual (the
+    library skips the network for a growing back-off after a guess that gains
+    too little)."""
     ctx = WL.build_context(W, precision)
     ctx.kernel_timing(True)      # mover kernels bracketed by events inside the library
     stream = C.c_void_p()
@@ -101,20 +118,15 @@ def run_mode(W, precision, steps, warmup, hip, seed=7, learned=False, learn_step
     ev = [hip.event() for _ in range(5 + 2 * nsp)]
     ev_end = ev[4 + 2 * nsp]
     acc = dict(field=0.0, sort=0.0, mcc=0.0, push=0.0, vcycles=0, push_kernel_ms=[[] for _ in range(nsp)],
-               net_used=0, net_probed=0, res0_gain=[], train_s=0.0, loss=None)
+               net_used=0, net_probed=0, res0_gain=[])
     pre = learn_steps
+    learned = net_store is not None
     if learned:
-        ctx.net_create(capacity=64, seed=seed)
-        ctx.net_recording(True)
+        ctx.net_create(capacity=0, seed=seed)
+        ctx.net_copy(net_store)
     # the first solve starts cold and converges from zero; later ones warm-start
     for it in range(1, pre + warmup + steps + 1):
         timed = it > pre + warmup
-        if learned and it == pre + 1:
-            ctx.net_recording(False)
-            hip.sync()
-            t = time.perf_counter()
-            acc["loss"] = ctx.net_train(train_iters, batch=4, lr=2e-3)
-            acc["train_s"] = time.perf_counter() - t
         if it == pre + warmup + 1:
             hip.sync()
             t0 = time.perf_counter()
@@ -418,8 +430,11 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-plain-mixed", action="store_true")
     ap.add_argument("--skip-learned", action="store_true")
-    ap.add_argument("--learn-steps", type=int, default=40)
-    ap.add_argument("--train-iters", type=int, default=300)
+    ap.add_argument("--learn-steps", type=int, default=40, help="untimed steps before the warm-up of every mode")
+    ap.add_argument("--ws-samples", type=int, default=4000,
+                    help="(drhs, dphi) samples of the warm start's offline training set")
+    ap.add_argument("--ws-runs", type=int, default=8, help="randomised 512x512 runs the samples come from")
+    ap.add_argument("--ws-iters", type=int, default=2000, help="training iterations (batches of 4)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--skip-accuracy", action="store_true",
@@ -456,18 +471,21 @@ def main():
                    ion_mover_ms=float(np.mean([np.mean(x) for x in a["push_kernel_ms"][1:]])))
         if nsp > 2:
             out["species_mover_ms"] = {sp["name"]: float(np.mean(x)) for sp, x in zip(W["species"], a["push_kernel_ms"])}
-        if a["loss"] is not None:
-            out.update(train_s=a["train_s"], train_loss_first=a["loss"][0], train_loss_last=a["loss"][1],
-                       net_probed=a["net_probed"] / K, net_start_used=a["net_used"] / K,
+        if a["net_probed"]:
+            out.update(net_probed=a["net_probed"] / K, net_start_used=a["net_used"] / K,
                        res0_gain_median=float(np.median(a["res0_gain"])) if a["res0_gain"] else None)
         return out
 
     res = {}
+    ws_rec = None
     if not args.skip_learned:
+        print(f"[bench] learned warm start: offline training on {args.ws_samples} samples of {args.ws_runs} "
+              f"randomised 512x512 runs (untimed)", file=sys.stderr, flush=True)
+        store, ws_rec = offline_net(args)
         print(f"[bench] {W['name']}: mixed precision + learned warm start, {N} particles, "
-              f"{args.learn_steps} recorded + {args.warmup}+{K} steps", file=sys.stderr, flush=True)
-        r = run_mode(W, L.MIXED, K, args.warmup, hip, learned=True, learn_steps=args.learn_steps,
-                     train_iters=args.train_iters)
+              f"{args.learn_steps}+{args.warmup}+{K} steps", file=sys.stderr, flush=True)
+        r = run_mode(W, L.MIXED, K, args.warmup, hip, net_store=store, learn_steps=args.learn_steps)
+        store.close()
         res["mixed_learned"] = (r, summary(*r))
     if not args.skip_plain_mixed:
         print(f"[bench] {W['name']}: mixed precision, {args.learn_steps}+{args.warmup}+{K} steps", file=sys.stderr,
@@ -517,6 +535,8 @@ def main():
     }
     for k, (_, sm) in res.items():
         out[k] = sm
+    if ws_rec is not None:
+        out["mixed_learned"]["offline_training"] = ws_rec
     if "fp64" in res:
         (acc_f, _, cnt_f), s_f = res["fp64"]
         kf = np.array(acc_f["push_kernel_ms"][0])

@@ -350,6 +350,12 @@ typedef struct {
 int lepic_net_create(lepic_ctx *ctx, int32_t capacity, uint64_t seed);
 int lepic_net_recording(lepic_ctx *ctx, int32_t on);
 int lepic_net_samples(lepic_ctx *ctx, int32_t *n);
+/* Offline training across runs: append src's recorded samples (oldest
+ * first) to dst's sample store (same grid, both networks created; the
+ * store keeps its newest `capacity` samples), and copy src's trained
+ * network (weights and output scale) into dst, whose grid may differ. */
+int lepic_net_copy_samples(lepic_ctx *dst, lepic_ctx *src, int32_t *copied);
+int lepic_net_copy(lepic_ctx *dst, lepic_ctx *src);
 int lepic_net_policy(lepic_ctx *ctx, float min_gain, int32_t max_backoff);
 int lepic_net_train(lepic_ctx *ctx, int32_t iters, int32_t batch, float lr, float *loss_first,
                     float *loss_last);

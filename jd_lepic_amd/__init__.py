@@ -269,6 +269,17 @@ class Context:
         self._check(self.lib.lepic_net_samples(self._ctx, C.byref(n)))
         return n.value
 
+    def net_copy_samples(self, src):
+        """Append the recorded samples of context `src` (same grid) to this
+        context's sample store; returns how many were copied."""
+        n = C.c_int32()
+        self._check(self.lib.lepic_net_copy_samples(self._ctx, src._ctx, C.byref(n)))
+        return n.value
+
+    def net_copy(self, src):
+        """This context's network becomes a copy of `src`'s trained one."""
+        self._check(self.lib.lepic_net_copy(self._ctx, src._ctx))
+
     def net_policy(self, min_gain=2.0, max_backoff=256):
         self._check(self.lib.lepic_net_policy(self._ctx, min_gain, max_backoff))
 

@@ -112,6 +112,8 @@ SIGNATURES = {
     "lepic_net_create": (C.c_int, [_ctxp, C.c_int32, C.c_uint64]),
     "lepic_net_recording": (C.c_int, [_ctxp, C.c_int32]),
     "lepic_net_samples": (C.c_int, [_ctxp, _ip]),
+    "lepic_net_copy_samples": (C.c_int, [_ctxp, _ctxp, C.POINTER(C.c_int32)]),
+    "lepic_net_copy": (C.c_int, [_ctxp, _ctxp]),
     "lepic_net_policy": (C.c_int, [_ctxp, C.c_float, C.c_int32]),
     "lepic_backup_write": (C.c_int, [C.c_char_p, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32, _ip,
                                      C.POINTER(_dp)]),

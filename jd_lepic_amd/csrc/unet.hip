@@ -1043,6 +1043,54 @@ int lepic_net_policy(lepic_ctx* ctx, float min_gain, int32_t max_backoff) {
   return 0;
 }
 
+int lepic_net_copy_samples(lepic_ctx* dst, lepic_ctx* src, int32_t* copied) {
+  lepic_ctx* ctx = dst;   // errors are reported on the receiving context
+  Net* D = need_net(dst);
+  Net* S = need_net(src);
+  if (!D || !S) return dst ? dst->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (D->Hs != S->Hs || D->Ws != S->Ws)
+    return dst->fail(LEPIC_ERR_ARG, "sample images of the two contexts differ in size (different grids)");
+  if (D->cap < 1) return dst->fail(LEPIC_ERR_STATE, "the receiving network holds no samples (capacity 0)");
+  const int64_t hw = (int64_t)D->Hs * D->Ws;
+  // the source's samples oldest first, after its stream's pending work
+  LEPIC_HIP(hipStreamSynchronize(src->stream));
+  const int n = S->count, first = S->count < S->cap ? 0 : S->head;
+  for (int q = 0; q < n; ++q) {
+    const int s = (first + q) % std::max(S->cap, 1);
+    const int d = D->head;
+    LEPIC_HIP(hipMemcpyAsync(D->s_x + (size_t)d * hw, S->s_x + (size_t)s * hw, hw * 4, hipMemcpyDeviceToDevice,
+                             dst->stream));
+    LEPIC_HIP(hipMemcpyAsync(D->s_t + (size_t)d * hw, S->s_t + (size_t)s * hw, hw * 4, hipMemcpyDeviceToDevice,
+                             dst->stream));
+    D->s_sx[d] = S->s_sx[s];
+    D->s_st[d] = S->s_st[s];
+    D->head = (D->head + 1) % D->cap;
+    D->count = std::min(D->count + 1, D->cap);
+  }
+  LEPIC_HIP(hipStreamSynchronize(dst->stream));
+  if (copied) *copied = n;
+  return 0;
+}
+
+int lepic_net_copy(lepic_ctx* dst, lepic_ctx* src) {
+  lepic_ctx* ctx = dst;   // errors are reported on the receiving context
+  Net* D = need_net(dst);
+  Net* S = need_net(src);
+  if (!D || !S) return dst ? dst->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
+  if (D->nparam != S->nparam) return dst->fail(LEPIC_ERR_ARG, "network layouts differ");
+  LEPIC_HIP(hipStreamSynchronize(src->stream));
+  LEPIC_HIP(hipMemcpyAsync(D->w, S->w, D->nparam * 4, hipMemcpyDeviceToDevice, dst->stream));
+  if (int rc = zero_async(dst, D->m, D->nparam * 4)) return rc;
+  if (int rc = zero_async(dst, D->v, D->nparam * 4)) return rc;
+  D->adam_t = 0;
+  D->ratio = S->ratio;
+  D->trained = S->trained;
+  D->backoff = D->skip_left = 0;
+  if (int rc = refresh_bf16(dst, *D)) return rc;
+  LEPIC_HIP(hipStreamSynchronize(dst->stream));
+  return 0;
+}
+
 int lepic_net_samples(lepic_ctx* ctx, int32_t* n) {
   Net* N = need_net(ctx);
   if (!N || !n) return ctx ? ctx->fail(LEPIC_ERR_STATE, "no network") : LEPIC_ERR_ARG;
