@@ -13,10 +13,9 @@
 // (global memory for particles outside it).  Deposition accumulates in an
 // LDS window of 16 rows x 64 columns of nodes around the segment's first
 // cell and flushes the window to the species' global accumulator; nodes
-// outside the window go to the accumulator directly.  While a segment's
-// particles are in cell order, equal cells are merged first within a thread
-// and in runs of adjacent lanes (segmented DPP sums); once pushes have
-// scattered them, each particle's weights go to the window directly.
+// outside the window go to the accumulator directly.  Equal cells are
+// merged first among a thread's four consecutive particles (register sums in
+// select form) and then over runs of adjacent lanes (segmented DPP sums).
 // Window and accumulator hold
 // fixed-point integers (CIC weight x 2^kFixShift64 in FP64 contexts,
 // x 2^kFixShiftMx in mixed ones): integer addition is associative, so the
@@ -221,13 +220,10 @@ __device__ inline void compact_offsets(const MoverCommon& a, int64_t bid, int cn
 // Offsets for thread-major items (thread t holds slots kItems*t + k): the
 // same order-preserving compaction as compact_offsets.  Returns the number
 // of survivors of the segment.
-// extra: a per-wave count summed over the block through the same exchange
-// (the deposition's run count), returned in *extra_total.
 template <int BT>
 __device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t bid, int cnt_in,
-                                                  const bool surv[kItems], uint32_t off[kItems],
-                                                  uint32_t extra, uint32_t* extra_total) {
-  __shared__ uint32_t s_wave[BT / 64], s_extra[BT / 64];
+                                                  const bool surv[kItems], uint32_t off[kItems]) {
+  __shared__ uint32_t s_wave[BT / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   uint32_t mine = 0;
 #pragma unroll
@@ -239,17 +235,15 @@ __device__ inline uint32_t compact_offsets_tmajor(const MoverCommon& a, int64_t 
     const uint32_t o = __shfl_up(incl, d, 64);
     if (lane >= d) incl += o;
   }
-  if (lane == 63) { s_wave[w] = incl; s_extra[w] = extra; }
+  if (lane == 63) s_wave[w] = incl;
   __syncthreads();
-  uint32_t before = incl - mine, total = 0, ex = 0;
+  uint32_t before = incl - mine, total = 0;
 #pragma unroll
   for (int v = 0; v < BT / 64; ++v) {
     const uint32_t c = s_wave[v];
     if (v < w) before += c;
     total += c;
-    ex += s_extra[v];
   }
-  *extra_total = ex;
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
     off[k] = before;
@@ -349,72 +343,84 @@ __device__ __forceinline__ void deposit_global(const MoverCommon& a, int ix, int
   acc_add(&a.acc[n01], q3);
 }
 
-// CIC deposition of thread-major items (consecutive particles per thread):
-// equal cells within a thread are summed first, then runs of equal cells
-// across adjacent lanes per item slot by a segmented DPP scan (cheaper than
-// the LDS atomics it saves even when runs are short); slots empty in the
-// whole wave are skipped.  Window-local cell indices keep the node
-// arithmetic to adds; cells whose nodes leave the window take the global
-// path.
-__device__ __forceinline__ void deposit_tmajor(const MoverCommon& a, int* win, int wx0, int wy0,
-                                               const int C[kItems], const float FX[kItems],
-                                               const float FY[kItems], const bool ok[kItems]) {
-  const int lane = threadIdx.x & 63;
-  int key[kItems];          // window-local cell, -1: nothing in this slot
-  int w[kItems][4];
-  int prev = -1;
+// Window keys of a thread's items: key[k] = window-local cell (row stride
+// kWinStride) or -1; cells whose nodes leave the window get gkey[k] = packed
+// cell (iy<<16)|ix, else -1.
+__device__ __forceinline__ void window_keys(const MoverCommon& a, int wx0, int wy0, const int C[kItems],
+                                            const bool ok[kItems], int key[kItems], int gkey[kItems]) {
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    key[k] = -1;
-    w[k][0] = w[k][1] = w[k][2] = w[k][3] = 0;
-    if (!ok[k]) continue;
     int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
     clamp_cell(ix, iy, a.n1, a.n2);
-    int q0, q1, q2, q3;
-    cic_fix(FX[k], FY[k], q0, q1, q2, q3);
     const int cx = ix - wx0, cy = iy - wy0;
-    if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) {
-      deposit_global(a, ix, iy, q0, q1, q2, q3);
-      continue;
-    }
-    const int kk = cy * kWinStride + cx;
-    if (kk == prev) {
-      // same cell as this thread's previous particle: add into its slot
-#pragma unroll
-      for (int j = k - 1; j >= 0; --j)
-        if (key[j] == kk) { w[j][0] += q0; w[j][1] += q1; w[j][2] += q2; w[j][3] += q3; break; }
-    } else {
-      key[k] = kk;
-      w[k][0] = q0; w[k][1] = q1; w[k][2] = q2; w[k][3] = q3;
-      prev = kk;
-    }
+    const bool in = (unsigned)cx < (unsigned)(kWinCols - 1) && (unsigned)cy < (unsigned)(kWinRows - 1);
+    key[k] = ok[k] && in ? cy * kWinStride + cx : -1;
+    gkey[k] = ok[k] && !in ? (iy << 16) | ix : -1;
   }
+}
+
+// CIC deposition of a thread's four consecutive items into the LDS window
+// while the segment's particles are in cell order.  Runs of equal cell
+// within the thread are summed in registers in select form (no branches per
+// item); a run that a cell change ends inside the thread is added by its
+// lane directly (rare: a few lanes per wave), and the run that ends the
+// thread is summed over runs of equal cell in adjacent lanes (segmented DPP
+// sums) and added once per run.  key[k]: window-local cell or -1.
+__device__ __forceinline__ void deposit_window_merged(int* win, const int key[kItems], const float FX[kItems],
+                                                      const float FY[kItems]) {
+  const int lane = threadIdx.x & 63;
+  int cur = -1;
+  int acc[4] = {0, 0, 0, 0};
+  int ek[kItems];          // the run that ends just before item k (k >= 1), or -1
+  int ew[kItems][4];
 #pragma unroll
   for (int k = 0; k < kItems; ++k) {
-    const uint64_t act = __ballot(key[k] >= 0);
-    if (act == 0) continue;                       // wave-uniform
-    // inactive lanes take a key no active lane has, distinct from neighbours
-    const int kk = key[k] >= 0 ? key[k] : -2 - lane;
-    const int kp = __shfl_up(kk, 1, 64);
-    const uint64_t heads = __ballot(lane == 0 || kp != kk);
-    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
-    int v[4] = {w[k][0], w[k][1], w[k][2], w[k][3]};
-    // every active lane its own run (a slot holding only the cell changes of
-    // a few threads): nothing to merge
-    if (__popcll(heads & act) != __popcll(act)) seg_sum4(v, s);
-    const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
-    if (tail && key[k] >= 0) {
-      atomicAdd(&win[kk], v[0]);
-      atomicAdd(&win[kk + 1], v[1]);
-      atomicAdd(&win[kk + kWinStride + 1], v[2]);
-      atomicAdd(&win[kk + kWinStride], v[3]);
+    int q[4];
+    cic_fix(FX[k], FY[k], q[0], q[1], q[2], q[3]);
+    const bool in = key[k] >= 0;
+    const bool start = in && key[k] != cur;
+    ek[k] = start ? cur : -1;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      ew[k][c] = acc[c];
+      const int qc = in ? q[c] : 0;
+      acc[c] = start ? qc : acc[c] + qc;
     }
+    cur = start ? key[k] : cur;
+  }
+#pragma unroll
+  for (int k = 1; k < kItems; ++k) {
+    if (__ballot(ek[k] >= 0) == 0) continue;      // wave-uniform
+    if (ek[k] >= 0) {
+      const int kk = ek[k];
+      atomicAdd(&win[kk], ew[k][0]);
+      atomicAdd(&win[kk + 1], ew[k][1]);
+      atomicAdd(&win[kk + kWinStride + 1], ew[k][2]);
+      atomicAdd(&win[kk + kWinStride], ew[k][3]);
+    }
+  }
+  const uint64_t act = __ballot(cur >= 0);
+  if (act == 0) return;                           // wave-uniform
+  // lanes without a run take a key no active lane has, distinct from neighbours
+  const int kk = cur >= 0 ? cur : -2 - lane;
+  const int kp = __shfl_up(kk, 1, 64);
+  const uint64_t heads = __ballot(lane == 0 || kp != kk);
+  if (__popcll(heads & act) != __popcll(act)) {
+    const int s = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));
+    seg_sum4(acc, s);
+  }
+  const bool tail = lane == 63 || ((heads >> (lane + 1)) & 1ull);
+  if (tail && cur >= 0) {
+    atomicAdd(&win[cur], acc[0]);
+    atomicAdd(&win[cur + 1], acc[1]);
+    atomicAdd(&win[cur + kWinStride + 1], acc[2]);
+    atomicAdd(&win[cur + kWinStride], acc[3]);
   }
 }
 
 // Items whose nodes leave the LDS window (a segment that continues on the
 // next row after the end of one, or particles far from their segment):
-// merged like deposit_tmajor -- within a thread, then runs across lanes --
+// merged within a thread, then over runs of equal cell in adjacent lanes,
 // and added to the global density per run.  gk: packed cell (iy<<16)|ix or
 // -1.  Skipped by waves that have none.
 __device__ __forceinline__ void deposit_global_runs(const MoverCommon& a, const int gk[kItems],
@@ -813,32 +819,8 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
                    0.5 * a.Nm * a.mass * ((double)VX[k] * VX[k] + (double)VY[k] * VY[k] + (double)VZ[k] * VZ[k]));
       }
   }
-  // the deposition's window-local keys (cells leaving the window go to the
-  // global density here) and the wave's number of runs of equal keys
-  int dkey[kItems], gkey[kItems];
-  uint32_t runs = 0;
-  if (a.deposit) {
-    const int lane = t & 63;
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      dkey[k] = -1;
-      gkey[k] = -1;
-      if (!surv[k]) continue;
-      int ix = C[k] & 0xFFFF, iy = C[k] >> 16;
-      clamp_cell(ix, iy, n1, n2);
-      const int cx = ix - wx0, cy = iy - wy0;
-      if (cx < 0 || cx + 1 >= kWinCols || cy < 0 || cy + 1 >= kWinRows) gkey[k] = (iy << 16) | ix;
-      else dkey[k] = cy * kWinStride + cx;
-    }
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      const int kk = dkey[k] >= 0 ? dkey[k] : -2 - lane;
-      const int kp = __shfl_up(kk, 1, 64);
-      runs += __popcll(__ballot(dkey[k] >= 0 && (lane == 0 || kp != kk)));
-    }
-  }
-  uint32_t off[kItems], block_runs = 0;
-  const uint32_t kept = compact_offsets_tmajor<BT>(a, bid, cnt_in, surv, off, runs, &block_runs);
+  uint32_t off[kItems];
+  const uint32_t kept = compact_offsets_tmajor<BT>(a, bid, cnt_in, surv, off);
   if (kept == (uint32_t)cnt_in && nv == kItems) {
     // nothing of the segment was lost: every survivor keeps its slot
     *(int4*)(out.cell + i0) = make_int4(C[0], C[1], C[2], C[3]);
@@ -859,29 +841,13 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     }
   }
   if (!a.deposit) return;
+  // the deposition at the new positions: the window's cells merged in
+  // registers and over adjacent lanes, cells leaving the window to the
+  // global accumulator
+  int dkey[kItems], gkey[kItems];
+  window_keys(a, wx0, wy0, C, surv, dkey, gkey);
   deposit_global_runs(a, gkey, FX, FY);
-  if (2 * block_runs > kept) {
-    // fewer than two items per run on average (pushes have scattered the
-    // segment): every item's fixed-point weights straight to the window,
-    // where LDS atomics now rarely collide; merging runs first pays only
-    // while the particles are in cell order
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) {
-      if (dkey[k] < 0) continue;
-      int q0, q1, q2, q3;
-      cic_fix(FX[k], FY[k], q0, q1, q2, q3);
-      const int kk = dkey[k];
-      atomicAdd(&win[kk], q0);
-      atomicAdd(&win[kk + 1], q1);
-      atomicAdd(&win[kk + kWinStride + 1], q2);
-      atomicAdd(&win[kk + kWinStride], q3);
-    }
-  } else {
-    bool ok[kItems];
-#pragma unroll
-    for (int k = 0; k < kItems; ++k) ok[k] = dkey[k] >= 0;
-    deposit_tmajor(a, win, wx0, wy0, C, FX, FY, ok);
-  }
+  deposit_window_merged(win, dkey, FX, FY);
   __syncthreads();
   flush_window(a, win, wx0, wy0);
 }
@@ -957,7 +923,10 @@ __global__ __launch_bounds__(kBlock) void k_deposit_mixed(MoverCommon a, SoaMx i
 #pragma unroll
   for (int k = 0; k < kItems; ++k) ok[k] = k < nv;
   __syncthreads();
-  deposit_tmajor(a, win, s_wx0, s_wy0, C, FX, FY, ok);
+  int key[kItems], gkey[kItems];
+  window_keys(a, s_wx0, s_wy0, C, ok, key, gkey);
+  deposit_global_runs(a, gkey, FX, FY);
+  deposit_window_merged(win, key, FX, FY);
   __syncthreads();
   flush_window(a, win, s_wx0, s_wy0);
 }
