@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "lepic_internal.h"
@@ -149,6 +150,7 @@ struct VisitRef {
   double eps;
   int nexec;               // sweeps launched in the visit
   int klast0;              // sweeps before the visit's last launch
+  int fixed;               // the visit runs exactly nexec sweeps (mixed form)
   int* visit;
   double* vres;
 };
@@ -195,6 +197,16 @@ __device__ int first_stop(const double* partials, int nb, int kf, int kt, int di
   return 0;
 }
 
+// the same over any range of sweeps still in the ring, kS at a time
+__device__ int first_stop_range(const double* partials, int nb, int kf, int kt, int dig, double eps, int nr,
+                                double* r_out) {
+  for (int k = kf; k <= kt; k += kS) {
+    const int ks = first_stop(partials, nb, k, min(k + kS - 1, kt), dig, eps, nr, r_out);
+    if (ks) return ks;
+  }
+  return 0;
+}
+
 struct Resolved {
   int ks;                  // sweeps of the visit (0: not finished)
   int buf;                 // rotating buffer holding the state after them
@@ -208,6 +220,22 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   LevelCtrl* c = V.ctrl;
   const int c0 = c->cur[(V.epoch - 1) & 1];
   Resolved out{0, -1, 0.0};
+  if (V.fixed) {
+    // exactly nexec sweeps ran; the log records where sors.f90:371-377 would
+    // have stopped among them as -k, or nexec when it would have gone on
+    // (the host's schedule for the next V-cycle)
+    double r = 0.0;
+    const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
+    out.ks = V.nexec;
+    out.buf = (c0 + V.nexec) % kRot;
+    out.res = r;
+    if (writer && threadIdx.x == 0) {
+      c->cur[V.epoch & 1] = canon ? 0 : out.buf;
+      V.visit[V.ig] = ks ? -ks : V.nexec;
+      V.vres[V.ig] = r;
+    }
+    return out;
+  }
   if (c->done_epoch == V.epoch) {
     out.ks = c->ksor;
     out.res = c->res;
@@ -237,6 +265,7 @@ struct SweepArgs {
   double* partials;
   int nbx, nby;
   int epoch, dig, nr;
+  int fixed;               // run the launch's sweeps without the exit test (mixed form)
   double eps;
 };
 
@@ -261,7 +290,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
     LevelCtrl* c = A.ctrl;
     int go = 1;
     const int c0 = c->cur[(A.epoch - 1) & 1];
-    if (__hip_atomic_load(&c->done_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == A.epoch) {
+    if (A.fixed) {
+      go = 1;
+    } else if (__hip_atomic_load(&c->done_epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == A.epoch) {
       go = 0;
     } else if (k0 > 0) {
       double r = 0.0;
@@ -1064,7 +1095,7 @@ double* d_vres_of(lepic_ctx* ctx) { return ctx->d_scalar + 32; }
 int* d_status_of(lepic_ctx* ctx) { return (int*)(ctx->d_scalar + 64) + kVisitLog; }
 
 VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, double eps, int nexec,
-                   int klast0 = -1) {
+                   int klast0 = -1, int fixed = 0) {
   Level& L = ctx->lv[l];
   VisitRef v;
   v.ctrl = ctx->d_ctrl + l;
@@ -1073,6 +1104,7 @@ VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, do
   v.epoch = epoch; v.dig = dig; v.nr = nr; v.ig = ig; v.eps = eps;
   v.nexec = nexec;
   v.klast0 = klast0 >= 0 ? klast0 : ((nexec - 1) / kS) * kS;
+  v.fixed = fixed;
   v.visit = d_visit_of(ctx);
   v.vres = d_vres_of(ctx);
   return v;
@@ -1087,6 +1119,22 @@ int last_launch_start(int k0, int k_end, int first) {
     k += std::min(size, k_end - k);
   }
   return last;
+}
+
+// How a multi-block level's visit runs.  FP64: the reference's exit test
+// (sors.f90:371-377) decides, the first launch `first` sweeps and
+// continuations up to kNr.  Mixed: the visit runs exactly the number of
+// sweeps the test gave the same visit in the previous V-cycle (the count
+// lags one V-cycle, no continuation launch tests anything).
+struct VisitPlan {
+  int k_end, first, klast0, fixed;
+};
+
+VisitPlan visit_plan(const Level& L, bool up, bool f32) {
+  const int f = up ? L.first_up : L.first_dn;
+  if (!f32) return VisitPlan{kNr, f, last_launch_start(0, kNr, f), 0};
+  const int first = std::min(f, kS);
+  return VisitPlan{f, first, last_launch_start(0, f, first), 1};
 }
 
 template <typename T, int kTX, int TY, int NT>
@@ -1104,7 +1152,7 @@ void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k,
 // sweeps and returns at once when the visit has stopped there
 template <typename T>
 int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end,
-                 int first = kS) {
+                 int first = kS, int fixed = 0) {
   Level& L = ctx->lv[l];
   SweepArgs<T> A;
   for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<T>(L, r);
@@ -1116,6 +1164,7 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.partials = L.partials;
   A.nbx = L.nbx; A.nby = L.nby;
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
+  A.fixed = fixed;
   int kchk = 0;
   for (int k = k0, size = std::min(std::max(first, 1), kS); k < k_end; size = kS) {
     const int kexec = std::min(size, k_end - k);
@@ -1134,7 +1183,8 @@ int launch_restrict(lepic_ctx* ctx, int l, bool f32, int epoch, double eps) {
   Level& F = ctx->lv[l];
   Level& C = ctx->lv[l + 1];
   RestrictArgs<T> A;
-  A.vf = visit_ref(ctx, l, epoch, l - ng, kNr, l, eps, kNr, last_launch_start(0, kNr, F.first_dn));
+  const VisitPlan pf = visit_plan(F, false, std::is_same<T, float>::value);
+  A.vf = visit_ref(ctx, l, epoch, l - ng, kNr, l, eps, pf.k_end, pf.klast0, pf.fixed);
   for (int r = 0; r < kRot; ++r) A.frot[r] = rot_of<T>(F, r);
   A.fb = b_of<T>(F, f32);
   A.fbc = F.bc;
@@ -1165,8 +1215,10 @@ int launch_prolong(lepic_ctx* ctx, int l, bool f32, int epoch_up, double eps) {
   for (int r = 0; r < kRot; ++r) A.crot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
   A.coarse_mb = C.mb ? 1 : 0;
   if (C.mb)
-    A.vc = visit_ref(ctx, l + 1, epoch_up, ng - (l + 1), kNr, 2 * ng - (l + 1), eps, kNr,
-                     last_launch_start(0, kNr, C.first_up));
+  {
+    const VisitPlan pc = visit_plan(C, true, std::is_same<T, float>::value);
+    A.vc = visit_ref(ctx, l + 1, epoch_up, ng - (l + 1), kNr, 2 * ng - (l + 1), eps, pc.k_end, pc.klast0, pc.fixed);
+  }
   else A.vc = VisitRef{};
   dim3 g((C.n1 + 1 + 127) / 128, std::min(C.n2 + 1, kRowBlocks));
   hipLaunchKernelGGL(k_prolong<T>, g, dim3(128), 0, ctx->stream, A);
@@ -1247,20 +1299,23 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
   const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
   ctx->mg_epoch += 2;
   for (int l = 1; l < Ls; ++l) {  // down leg
-    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, kNr, ctx->lv[l].first_dn)) return rc;
+    const VisitPlan pl = visit_plan(ctx->lv[l], false, f32);
+    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, pl.k_end, pl.first, pl.fixed)) return rc;
     if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
   }
   if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
   for (int l = Ls - 1; l >= 1; --l) {  // up leg
     if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
-    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, kNr, ctx->lv[l].first_up)) return rc;
+    const VisitPlan pl = visit_plan(ctx->lv[l], true, f32);
+    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, pl.k_end, pl.first, pl.fixed)) return rc;
   }
   Level& L1 = ctx->lv[1];
   const unsigned gx = (unsigned)((L1.n1 + 1 + 255) / 256);
   if (!f32) {
     if (L1.mb) {
       Finish64Args A;
-      A.v = visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr, last_launch_start(0, kNr, L1.first_up));
+      const VisitPlan p1 = visit_plan(L1, true, false);
+      A.v = visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed);
       for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<double>(L1, r);
       A.n1 = L1.n1; A.n2 = L1.n2;
       A.status = d_status_of(ctx);
@@ -1273,8 +1328,8 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     A.bc = ctx->bc;
     A.n1 = L1.n1; A.n2 = L1.n2;
     A.mb = L1.mb ? 1 : 0;
-    A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, kNr, last_launch_start(0, kNr, L1.first_up))
-                : VisitRef{};
+    const VisitPlan p1 = visit_plan(L1, true, true);
+    A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed) : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
   }
   LEPIC_HIP(hipGetLastError());
@@ -1381,6 +1436,9 @@ int poisson_setup(lepic_ctx* ctx) {
   for (int l = 1; l < ctx->Lc && l <= ng; ++l) {
     Level& L = ctx->lv[l];
     L.mb = true;
+    // the mixed form's visits start at the reference's sweep limit and adapt
+    // from the exit test's outcome; FP64 first launches start at kS
+    L.first_dn = L.first_up = mixed ? kNr : kS;
     const size_t nn = (size_t)(L.n1 + 3) * (L.n2 + 3);
     L.rot[0] = mixed ? (void*)L.uf : (void*)L.u;
     for (int r = 1; r < kRot; ++r) {
@@ -1535,10 +1593,23 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
       for (int l = 1; l < ctx->Lc; ++l) {
         Level& L = ctx->lv[l];
         if (!L.mb) continue;
-        // one sweep more than last time: a spare sweep costs less than a
-        // continuation launch when the count grows by one
-        L.first_dn = std::min(std::max(visit[l] + 1, 2), kS);
-        L.first_up = std::min(std::max(visit[2 * ng - l] + 1, 2), kS);
+        if (!mixed) {
+          // one sweep more than last time: a spare sweep costs less than a
+          // continuation launch when the count grows by one
+          L.first_dn = std::min(std::max(visit[l] + 1, 2), kS);
+          L.first_up = std::min(std::max(visit[2 * ng - l] + 1, 2), kS);
+        } else {
+          // the visit ran first_* sweeps; the log says where the exit test
+          // stopped (-k) or that it would have gone on (the count run)
+          auto next = [](int logged, int ran) {
+            return logged < 0 ? std::max(-logged, 2) : std::min(ran + 2, kNr);
+          };
+          const int dn = visit[l], up = visit[2 * ng - l];
+          visit[l] = L.first_dn;
+          visit[2 * ng - l] = L.first_up;
+          L.first_dn = next(dn, L.first_dn);
+          L.first_up = next(up, L.first_up);
+        }
       }
     for (int ig = 1; ig <= nvis; ++ig) {
       int l = ig < ng ? ig : (ig > ng ? 2 * ng - ig : ng);
