@@ -221,18 +221,21 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   const int c0 = c->cur[(V.epoch - 1) & 1];
   Resolved out{0, -1, 0.0};
   if (V.fixed) {
-    // exactly nexec sweeps ran; the log records where sors.f90:371-377 would
-    // have stopped among them as -k, or nexec when it would have gone on
-    // (the host's schedule for the next V-cycle)
-    double r = 0.0;
-    const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
+    // exactly nexec sweeps ran and the state after them is the result; only
+    // the writer evaluates sors.f90:371-377 over them, for the log: -k where
+    // the test would have stopped, or nexec when it would have gone on (the
+    // host's schedule for the next V-cycle)
     out.ks = V.nexec;
     out.buf = (c0 + V.nexec) % kRot;
-    out.res = r;
-    if (writer && threadIdx.x == 0) {
-      c->cur[V.epoch & 1] = canon ? 0 : out.buf;
-      V.visit[V.ig] = ks ? -ks : V.nexec;
-      V.vres[V.ig] = r;
+    if (writer) {
+      double r = 0.0;
+      const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
+      out.res = r;
+      if (threadIdx.x == 0) {
+        c->cur[V.epoch & 1] = canon ? 0 : out.buf;
+        V.visit[V.ig] = ks ? -ks : V.nexec;
+        V.vres[V.ig] = r;
+      }
     }
     return out;
   }
