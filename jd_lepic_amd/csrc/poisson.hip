@@ -55,6 +55,16 @@ constexpr int kResidBlocks = 512;             // blocks of the FP64 finest-level
 constexpr size_t kPartialsOff = (64 * 8 + (kVisitLog + 1) * sizeof(int) + 7) / 8 * 8;
 static_assert(kPartialsOff - 32 * 8 + kResidBlocks * 8 <= kPinnedDoubles * 8, "read-back fits the pinned buffer");
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
+// Mixed form, finest level when it is multi-block: the FP32 correction
+// V-cycle is a preconditioner inside FP64 refinement to the unchanged
+// tolerance, so its finest-level smoother is chosen for smoothing, not
+// taken from the reference: a fixed 4 down / 3 up red-black sweeps at
+// omega 1.15 (the reference's omega 1.4, iterative_solver.f90:27, damps the
+// highest frequencies less; its exit test stops these visits after 2 sweeps
+// on plasma solves and runs them to 10 near electrodes).  Measured: 1024^2
+// argon box 4 -> 3 V-cycles per solve; 1024x512 ion source 7.9 -> 7.7.
+constexpr int kFineDn = 4, kFineUp = 3;
+constexpr float kFineOmega = 1.15f;
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
 
@@ -1128,7 +1138,8 @@ int last_launch_start(int k0, int k_end, int first) {
 // (sors.f90:371-377) decides, the first launch `first` sweeps and
 // continuations up to kNr.  Mixed: the visit runs exactly the number of
 // sweeps the test gave the same visit in the previous V-cycle (the count
-// lags one V-cycle, no continuation launch tests anything).
+// lags one V-cycle, no continuation launch tests anything); the finest
+// level's visits run kFineDn / kFineUp sweeps.
 struct VisitPlan {
   int k_end, first, klast0, fixed;
 };
@@ -1145,6 +1156,7 @@ void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k,
   const dim3 g(L.nbx, L.nby);
   if (kexec <= 2) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 2>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
   else if (kexec <= 3) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 3>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+  else if (kexec <= 4) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 4>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
   else hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, kS>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
 }
 
@@ -1162,7 +1174,8 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.b = b_of<T>(L, f32);
   A.bc = L.bc;
   A.n1 = L.n1; A.n2 = L.n2;
-  A.aw = (T)L.aw; A.an = (T)L.an; A.ac = (T)L.ac; A.omega = (T)1.4;
+  A.aw = (T)L.aw; A.an = (T)L.an; A.ac = (T)L.ac;
+  A.omega = (f32 && l == 1) ? (T)kFineOmega : (T)1.4;   // iterative_solver.f90:27
   A.ctrl = ctx->d_ctrl + l;
   A.partials = L.partials;
   A.nbx = L.nbx; A.nby = L.nby;
@@ -1440,8 +1453,10 @@ int poisson_setup(lepic_ctx* ctx) {
     Level& L = ctx->lv[l];
     L.mb = true;
     // the mixed form's visits start at the reference's sweep limit and adapt
-    // from the exit test's outcome; FP64 first launches start at kS
-    L.first_dn = L.first_up = mixed ? kNr : kS;
+    // from the exit test's outcome (the finest level's are fixed); FP64
+    // first launches start at kS
+    L.first_dn = mixed ? (l == 1 ? kFineDn : kNr) : kS;
+    L.first_up = mixed ? (l == 1 ? kFineUp : kNr) : kS;
     const size_t nn = (size_t)(L.n1 + 3) * (L.n2 + 3);
     L.rot[0] = mixed ? (void*)L.uf : (void*)L.u;
     for (int r = 1; r < kRot; ++r) {
@@ -1610,8 +1625,10 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
           const int dn = visit[l], up = visit[2 * ng - l];
           visit[l] = L.first_dn;
           visit[2 * ng - l] = L.first_up;
-          L.first_dn = next(dn, L.first_dn);
-          L.first_up = next(up, L.first_up);
+          if (l > 1) {   // the finest level's counts are fixed
+            L.first_dn = next(dn, L.first_dn);
+            L.first_up = next(up, L.first_up);
+          }
         }
       }
     for (int ig = 1; ig <= nvis; ++ig) {
