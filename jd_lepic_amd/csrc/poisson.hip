@@ -102,6 +102,17 @@ __device__ __forceinline__ T getres(PU u, PB b, int ir, int jr, int n1, T aw, T 
   return urhs - (aw * uw + aw * ue + an * un + an * us + ac * uij);
 }
 
+// mg.f90:386-414: the weighted sum of the centre residual g9 and the eight
+// around it (g1 north-east, then clockwise), each counted when its flag holds
+template <typename T>
+__device__ __forceinline__ T full_weight(T g9, T g1, T g2, T g3, T g4, T g5, T g6, T g7, T g8, bool f1, bool f2,
+                                         bool f3, bool f4, bool f5, bool f6, bool f7, bool f8) {
+  const T r9 = g9;
+  const T r1 = f1 ? g1 : (T)0, r2 = f2 ? g2 : (T)0, r3 = f3 ? g3 : (T)0, r4 = f4 ? g4 : (T)0;
+  const T r5 = f5 ? g5 : (T)0, r6 = f6 ? g6 : (T)0, r7 = f7 ? g7 : (T)0, r8 = f8 ? g8 : (T)0;
+  return ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
+}
+
 // full-weighted residual of coarse node (i,j), mg.f90:386-414.  The nine
 // fine residuals are evaluated unconditionally (at a position clamped into
 // the interior when the reference skips one), so all loads issue at once;
@@ -128,10 +139,7 @@ __device__ __forceinline__ T restrict_node(PU u, PB b, PC bc, int i, int j, int 
   const T g9 = G(ir, jr), g1 = G(ir + 1, jr + 1), g2 = G(ir + 1, jr), g3 = G(ir + 1, jr - 1);
   const T g4 = G(ir, jr - 1), g5 = G(ir - 1, jr - 1), g6 = G(ir - 1, jr), g7 = G(ir - 1, jr + 1);
   const T g8 = G(ir, jr + 1);
-  const T r9 = g9;
-  const T r1 = f1 ? g1 : (T)0, r2 = f2 ? g2 : (T)0, r3 = f3 ? g3 : (T)0, r4 = f4 ? g4 : (T)0;
-  const T r5 = f5 ? g5 : (T)0, r6 = f6 ? g6 : (T)0, r7 = f7 ? g7 : (T)0, r8 = f8 ? g8 : (T)0;
-  return ((T)4.0 * r9 + (T)2.0 * (r2 + r4 + r6 + r8) + r1 + r3 + r5 + r7) / (T)16.0;
+  return full_weight<T>(g9, g1, g2, g3, g4, g5, g6, g7, g8, f1, f2, f3, f4, f5, f6, f7, f8);
 }
 
 // bilinear add of coarse node (i,j)'s correction to its four fine nodes,
@@ -231,22 +239,11 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   const int c0 = c->cur[(V.epoch - 1) & 1];
   Resolved out{0, -1, 0.0};
   if (V.fixed) {
-    // exactly nexec sweeps ran and the state after them is the result; only
-    // the writer evaluates sors.f90:371-377 over them, for the log: -k where
-    // the test would have stopped, or nexec when it would have gone on (the
-    // host's schedule for the next V-cycle)
+    // exactly nexec sweeps ran and the state after them is the result; the
+    // visit's log is written at the end of the V-cycle (k_visit_logs)
     out.ks = V.nexec;
     out.buf = (c0 + V.nexec) % kRot;
-    if (writer) {
-      double r = 0.0;
-      const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
-      out.res = r;
-      if (threadIdx.x == 0) {
-        c->cur[V.epoch & 1] = canon ? 0 : out.buf;
-        V.visit[V.ig] = ks ? -ks : V.nexec;
-        V.vres[V.ig] = r;
-      }
-    }
+    if (writer && threadIdx.x == 0) c->cur[V.epoch & 1] = canon ? 0 : out.buf;
     return out;
   }
   if (c->done_epoch == V.epoch) {
@@ -262,6 +259,30 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
     V.vres[V.ig] = out.res;
   }
   return out;
+}
+
+// Mixed form, end of a V-cycle: the log of every fixed-length visit,
+// evaluated off the V-cycle's critical path.  sors.f90:371-377 over the
+// sweeps the visit ran: -k where the test would have stopped, or nexec when
+// it would have gone on (the host's schedule for the next V-cycle).  The
+// down and up visits of a level keep their partial sums in separate rings,
+// so both are still there.  One wave per visit.
+struct VisitLogArgs {
+  int n;
+  VisitRef v[2 * kMaxLevels];
+};
+
+__global__ __launch_bounds__(256) void k_visit_logs(VisitLogArgs A) {
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int q = w; q < A.n; q += nw) {
+    const VisitRef& V = A.v[q];
+    double r = 0.0;
+    const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
+    if ((threadIdx.x & 63) == 0) {
+      V.visit[V.ig] = ks ? -ks : V.nexec;
+      V.vres[V.ig] = r;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -436,13 +457,28 @@ struct RestrictArgs {
   int cepoch;
 };
 
+// A block restricts a tile of kRcx x kRcy coarse nodes: the fine residuals
+// its nodes weight (2 kRcx + 1 by 2 kRcy + 1 positions) are evaluated once
+// each into LDS, with their bc flags, and every coarse node then combines
+// nine of them; the residual at a position is getres at the position clamped
+// into the interior, as restrict_node evaluates it.  Each block also zeroes
+// a slice of the coarse correction.
+constexpr int kRcx = 32, kRcy = 8;
+constexpr int kRrx = 2 * kRcx + 1, kRry = 2 * kRcy + 1;
+constexpr int kRestrictThreads = kRcx * kRcy;
+
 template <typename T>
-__global__ __launch_bounds__(128) void k_restrict(RestrictArgs<T> A) {
+__global__ __launch_bounds__(kRestrictThreads) void k_restrict(RestrictArgs<T> A) {
+  constexpr int kRN = kRrx * kRry;
+  constexpr int kIt = (kRN + kRestrictThreads - 1) / kRestrictThreads;
   __shared__ int s_fb, s_cb;
-  if (threadIdx.x < 64) {
+  __shared__ T sr[kRN];
+  __shared__ int8_t sf[kRN];
+  const int t = threadIdx.x;
+  if (t < 64) {
     const bool writer = blockIdx.x == 0 && blockIdx.y == 0;
     Resolved r = resolve_visit(A.vf, writer, false);
-    if (threadIdx.x == 0) {
+    if (t == 0) {
       s_fb = r.buf < 0 ? 0 : r.buf;
       s_cb = A.cctrl ? A.cctrl->cur[(A.cepoch - 1) & 1] : 0;
     }
@@ -450,17 +486,59 @@ __global__ __launch_bounds__(128) void k_restrict(RestrictArgs<T> A) {
   __syncthreads();
   const T* u = A.frot[s_fb];
   T* e2h = A.crot[s_cb];
-  const int m1 = A.n1 / 2, m2 = A.n2 / 2;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > m1 + 2) return;
-  for (int j = blockIdx.y; j <= m2 + 2; j += gridDim.y) {
-    e2h[(int64_t)j * (m1 + 3) + i] = (T)0;
-    if (i < 1 || i > m1 + 1 || j < 1 || j > m2 + 1) continue;
-    T out = (T)0;
-    if (A.bc2h[(int64_t)j * (m1 + 3) + i] <= 0)
-      out = restrict_node<T>(u, A.fb, A.fbc, i, j, A.n1, A.n2, A.aw, A.an, A.ac);
-    A.r2h[(int64_t)(j - 1) * (m1 + 1) + (i - 1)] = out;
+  const int n1 = A.n1, n2 = A.n2, m1 = n1 / 2, m2 = n2 / 2;
+  const int64_t W = n1 + 3;
+  {
+    const int64_t nc = (int64_t)(m1 + 3) * (m2 + 3);
+    const int64_t nbk = (int64_t)gridDim.x * gridDim.y, bk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    const int64_t per = (nc + nbk - 1) / nbk, q1 = min(nc, (bk + 1) * per);
+    for (int64_t q = bk * per + t; q < q1; q += kRestrictThreads) e2h[q] = (T)0;
   }
+  const int ci0 = 1 + (int)blockIdx.x * kRcx, cj0 = 1 + (int)blockIdx.y * kRcy;
+  const int fa0 = 2 * ci0 - 2, fc0 = 2 * cj0 - 2;   // fine position of region (0, 0)
+  {
+    T g[kIt];
+    int8_t fl[kIt];
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+      const int q = t + k * kRestrictThreads;
+      const int qq = q < kRN ? q : 0;
+      const int y = qq / kRrx, x = qq - y * kRrx;
+      const int a = fa0 + x, c = fc0 + y;
+      const int ac = a < 1 ? 1 : (a > n1 + 1 ? n1 + 1 : a);
+      const int cc = c < 1 ? 1 : (c > n2 + 1 ? n2 + 1 : c);
+      g[k] = getres(u, A.fb, ac, cc, n1, A.aw, A.an, A.ac);
+      fl[k] = (a <= n1 + 2 && c <= n2 + 2) ? A.fbc[(int64_t)c * W + a] : (int8_t)1;
+    }
+#pragma unroll
+    for (int k = 0; k < kIt; ++k) {
+      const int q = t + k * kRestrictThreads;
+      if (q < kRN) { sr[q] = g[k]; sf[q] = fl[k]; }
+    }
+  }
+  __syncthreads();
+  const int ti = t % kRcx, tj = t / kRcx;
+  const int i = ci0 + ti, j = cj0 + tj;
+  if (i > m1 + 1 || j > m2 + 1) return;
+  T out = (T)0;
+  if (A.bc2h[(int64_t)j * (m1 + 3) + i] <= 0) {
+    const int ir = 2 * i - 1, jr = 2 * j - 1;
+    const int x0 = 2 * ti + 1, y0 = 2 * tj + 1;      // region position of (ir, jr)
+    auto R = [&](int dx, int dy) { return sr[(y0 + dy) * kRrx + x0 + dx]; };
+    auto B = [&](int dx, int dy) { return sf[(y0 + dy) * kRrx + x0 + dx]; };
+    const bool xe = ir != n1 + 1, xw = ir != 1, yn = jr != n2 + 1, ys = jr != 1;
+    const bool f1 = xe && yn && B(1, 1) <= 0;
+    const bool f2 = xe && B(1, 0) <= 0;
+    const bool f3 = B(1, -1) <= 0 && xe && ys;
+    const bool f4 = B(0, -1) <= 0 && ys;
+    const bool f5 = B(-1, -1) <= 0 && xw && ys;
+    const bool f6 = B(-1, 0) <= 0 && xw;
+    const bool f7 = B(-1, 1) <= 0 && xw && yn;
+    const bool f8 = B(0, 1) <= 0 && yn;
+    out = full_weight<T>(R(0, 0), R(1, 1), R(1, 0), R(1, -1), R(0, -1), R(-1, -1), R(-1, 0), R(-1, 1), R(0, 1),
+                         f1, f2, f3, f4, f5, f6, f7, f8);
+  }
+  A.r2h[(int64_t)(j - 1) * (m1 + 1) + (i - 1)] = out;
 }
 
 // prolongation into a multi-block level from the level below (resolving the
@@ -1112,8 +1190,8 @@ VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, do
   Level& L = ctx->lv[l];
   VisitRef v;
   v.ctrl = ctx->d_ctrl + l;
-  v.partials = L.partials;
   v.nb = L.nbx * L.nby;
+  v.partials = L.partials + (dig > 0 ? (size_t)kRing * v.nb : 0);   // up visits: second ring
   v.epoch = epoch; v.dig = dig; v.nr = nr; v.ig = ig; v.eps = eps;
   v.nexec = nexec;
   v.klast0 = klast0 >= 0 ? klast0 : ((nexec - 1) / kS) * kS;
@@ -1177,7 +1255,7 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.aw = (T)L.aw; A.an = (T)L.an; A.ac = (T)L.ac;
   A.omega = (f32 && l == 1) ? (T)kFineOmega : (T)1.4;   // iterative_solver.f90:27
   A.ctrl = ctx->d_ctrl + l;
-  A.partials = L.partials;
+  A.partials = L.partials + (dig > 0 ? (size_t)kRing * L.nbx * L.nby : 0);   // up visits: second ring
   A.nbx = L.nbx; A.nby = L.nby;
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
   A.fixed = fixed;
@@ -1211,8 +1289,8 @@ int launch_restrict(lepic_ctx* ctx, int l, bool f32, int epoch, double eps) {
   for (int r = 0; r < kRot; ++r) A.crot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
   A.cctrl = C.mb ? ctx->d_ctrl + (l + 1) : nullptr;
   A.cepoch = epoch;
-  dim3 g((C.n1 + 3 + 127) / 128, std::min(C.n2 + 3, kRowBlocks));
-  hipLaunchKernelGGL(k_restrict<T>, g, dim3(128), 0, ctx->stream, A);
+  const dim3 g((unsigned)((C.n1 + 1 + kRcx - 1) / kRcx), (unsigned)((C.n2 + 1 + kRcy - 1) / kRcy));
+  hipLaunchKernelGGL(k_restrict<T>, g, dim3(kRestrictThreads), 0, ctx->stream, A);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -1347,6 +1425,13 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     const VisitPlan p1 = visit_plan(L1, true, true);
     A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed) : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
+    VisitLogArgs V{};
+    for (int l = 1; l < Ls; ++l) {
+      const VisitPlan pd = visit_plan(ctx->lv[l], false, true), pu = visit_plan(ctx->lv[l], true, true);
+      V.v[V.n++] = visit_ref(ctx, l, e_dn, l - ng, kNr, l, eps, pd.k_end, pd.klast0, pd.fixed);
+      V.v[V.n++] = visit_ref(ctx, l, e_up, ng - l, kNr, 2 * ng - l, eps, pu.k_end, pu.klast0, pu.fixed);
+    }
+    if (V.n) hipLaunchKernelGGL(k_visit_logs, dim3(1), dim3(256), 0, ctx->stream, V);
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
@@ -1473,7 +1558,7 @@ int poisson_setup(lepic_ctx* ctx) {
       L.nbx = (L.n1 + 1 + kSmallTX - 1) / kSmallTX;
       L.nby = (L.n2 + 1 + sty - 1) / sty;
     }
-    const size_t np = (size_t)kRing * L.nbx * L.nby;
+    const size_t np = 2 * (size_t)kRing * L.nbx * L.nby;   // down-visit ring, up-visit ring
     LEPIC_HIP(hipMalloc(&L.partials, np * 8));
     LEPIC_HIP(hipMemsetAsync(L.partials, 0, np * 8, ctx->stream));
   }
