@@ -300,6 +300,7 @@ struct SweepArgs {
   int nbx, nby;
   int epoch, dig, nr;
   int fixed;               // run the launch's sweeps without the exit test (mixed form)
+  int nores;               // no residuals (a visit whose sweep count nothing reads back): only the last state is written
   double eps;
 };
 
@@ -404,8 +405,17 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
       }
       __syncthreads();
     }
-    double rr = 0.0;
     T* uout = A.rot[(s_c0 + k0 + s) % kRot];
+    if (A.nores) {
+      if (s == kexec)
+        for (int q = threadIdx.x; q < kTX * TY; q += kSweepThreads) {
+          const int tr = q / kTX, tc = q - tr * kTX;
+          const int gi = gi0 + kHalo + tc, gj = gj0 + kHalo + tr;
+          if (gi <= n1 + 1 && gj <= n2 + 1) uout[(int64_t)gj * W + gi] = su[(kHalo + tr) * RW + kHalo + tc];
+        }
+      continue;
+    }
+    double rr = 0.0;
     for (int q0 = threadIdx.x; q0 < kTX * TY; q0 += NB * kSweepThreads) {
       int x[NB], gi[NB], gj[NB];
       int8_t m[NB];
@@ -1259,6 +1269,8 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.nbx = L.nbx; A.nby = L.nby;
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
   A.fixed = fixed;
+  // the mixed form's finest-level visits run fixed counts: their residuals feed nothing
+  A.nores = (f32 && l == 1 && fixed) ? 1 : 0;
   int kchk = 0;
   for (int k = k0, size = std::min(std::max(first, 1), kS); k < k_end; size = kS) {
     const int kexec = std::min(size, k_end - k);
@@ -1426,7 +1438,7 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed) : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
     VisitLogArgs V{};
-    for (int l = 1; l < Ls; ++l) {
+    for (int l = 2; l < Ls; ++l) {   // level 1 runs fixed counts without residuals
       const VisitPlan pd = visit_plan(ctx->lv[l], false, true), pu = visit_plan(ctx->lv[l], true, true);
       V.v[V.n++] = visit_ref(ctx, l, e_dn, l - ng, kNr, l, eps, pd.k_end, pd.klast0, pd.fixed);
       V.v[V.n++] = visit_ref(ctx, l, e_up, ng - l, kNr, 2 * ng - l, eps, pu.k_end, pu.klast0, pu.fixed);
