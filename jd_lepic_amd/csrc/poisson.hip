@@ -311,7 +311,11 @@ struct SweepArgs {
 template <typename T, int kTX, int TY, int kSweepThreads, int KS>
 __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec, int kchk) {
   constexpr int kHalo = Halo<KS>::v;
-  constexpr int RW = kTX + 2 * kHalo, RH = TY + 2 * kHalo, RN = RW * RH;
+  // a block's tile is kTX x TY nodes, or one node wider (taller) in the last
+  // column (row) of tiles when the level's node count leaves one node over
+  // (2^k + 1 nodes per side); the region has room for that and stays even
+  constexpr int kTP = kTX + 1, kTYP = TY + 1;
+  constexpr int RW = kTX + 2 + 2 * kHalo, RH = kTYP + 2 * kHalo, RN = RW * RH;
   constexpr int HW = (RW - 2) / 2;  // nodes of one colour per inner region row
   static_assert((RW & 1) == 0, "region width must be even");
   __shared__ T su[RN];
@@ -351,6 +355,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
 
   const int n1 = A.n1, n2 = A.n2;
   const int64_t W = n1 + 3;
+  const int tw = (int)blockIdx.x == A.nbx - 1 ? n1 + 1 - (int)blockIdx.x * kTX : kTX;
+  const int th = (int)blockIdx.y == A.nby - 1 ? n2 + 1 - (int)blockIdx.y * TY : TY;
   const int gi0 = 1 + (int)blockIdx.x * kTX - kHalo;  // global i of region column 0
   const int gj0 = 1 + (int)blockIdx.y * TY - kHalo;
   const T* uin = A.rot[(s_c0 + k0) % kRot];
@@ -408,25 +414,25 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
     T* uout = A.rot[(s_c0 + k0 + s) % kRot];
     if (A.nores) {
       if (s == kexec)
-        for (int q = threadIdx.x; q < kTX * TY; q += kSweepThreads) {
-          const int tr = q / kTX, tc = q - tr * kTX;
-          const int gi = gi0 + kHalo + tc, gj = gj0 + kHalo + tr;
-          if (gi <= n1 + 1 && gj <= n2 + 1) uout[(int64_t)gj * W + gi] = su[(kHalo + tr) * RW + kHalo + tc];
+        for (int q = threadIdx.x; q < kTP * kTYP; q += kSweepThreads) {
+          const int tr = q / kTP, tc = q - tr * kTP;
+          if (tc < tw && tr < th)
+            uout[(int64_t)(gj0 + kHalo + tr) * W + gi0 + kHalo + tc] = su[(kHalo + tr) * RW + kHalo + tc];
         }
       continue;
     }
     double rr = 0.0;
-    for (int q0 = threadIdx.x; q0 < kTX * TY; q0 += NB * kSweepThreads) {
+    for (int q0 = threadIdx.x; q0 < kTP * kTYP; q0 += NB * kSweepThreads) {
       int x[NB], gi[NB], gj[NB];
       int8_t m[NB];
       T c[NB], w[NB], e[NB], n[NB], so[NB], bb[NB];
 #pragma unroll
       for (int k = 0; k < NB; ++k) {
         const int q = q0 + k * kSweepThreads;
-        const int qq = q < kTX * TY ? q : 0;
-        const int tr = qq / kTX, tc = qq - tr * kTX;
+        const int qq = q < kTP * kTYP ? q : 0;
+        const int tr = qq / kTP, tc = qq - tr * kTP;
         x[k] = (kHalo + tr) * RW + kHalo + tc;
-        gi[k] = q < kTX * TY ? gi0 + kHalo + tc : n1 + 2;
+        gi[k] = (q < kTP * kTYP && tc < tw && tr < th) ? gi0 + kHalo + tc : n1 + 2;   // n1 + 2: not this tile's
         gj[k] = gj0 + kHalo + tr;
         m[k] = sbc[x[k]];
         c[k] = su[x[k]]; w[k] = su[x[k] - 1]; e[k] = su[x[k] + 1];
@@ -1501,6 +1507,14 @@ int read_back(lepic_ctx* ctx, int* visit, double* vres, double* resid_sum) {
 
 namespace lepic {
 
+// tiles of t nodes along a side of `nodes` nodes; a single node left over
+// joins the last tile (k_sweeps' tiles can be one node wider)
+static int tiles_of(int nodes, int t) {
+  int n = (nodes + t - 1) / t;
+  if (n > 1 && nodes - (n - 1) * t == 1) --n;
+  return n;
+}
+
 int poisson_setup(lepic_ctx* ctx) {
   const int ng = ctx->ng;
   const double eps0 = ctx->g.eps0;
@@ -1560,15 +1574,15 @@ int poisson_setup(lepic_ctx* ctx) {
       LEPIC_HIP(hipMalloc(&L.rot[r], nn * esz));
       LEPIC_HIP(hipMemsetAsync(L.rot[r], 0, nn * esz, ctx->stream));
     }
-    L.nbx = (L.n1 + 1 + kTX - 1) / kTX;
-    L.nby = (L.n2 + 1 + ty - 1) / ty;
+    L.nbx = tiles_of(L.n1 + 1, kTX);
+    L.nby = tiles_of(L.n2 + 1, ty);
     // few wide tiles leave most CUs idle: the level takes small tiles, whose
     // launch latency is lower (each block has less than half the nodes)
     L.small_tiles = L.nbx * L.nby < kSmallTileBelow;
     if (L.small_tiles) {
       const int sty = mixed ? SmallTileY<float>::v : SmallTileY<double>::v;
-      L.nbx = (L.n1 + 1 + kSmallTX - 1) / kSmallTX;
-      L.nby = (L.n2 + 1 + sty - 1) / sty;
+      L.nbx = tiles_of(L.n1 + 1, kSmallTX);
+      L.nby = tiles_of(L.n2 + 1, sty);
     }
     const size_t np = 2 * (size_t)kRing * L.nbx * L.nby;   // down-visit ring, up-visit ring
     LEPIC_HIP(hipMalloc(&L.partials, np * 8));
