@@ -240,7 +240,7 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   Resolved out{0, -1, 0.0};
   if (V.fixed) {
     // exactly nexec sweeps ran and the state after them is the result; the
-    // visit's log is written at the end of the V-cycle (k_visit_logs)
+    // visit's log is written after the V-cycle by spare blocks of k_resid64
     out.ks = V.nexec;
     out.buf = (c0 + V.nexec) % kRot;
     if (writer && threadIdx.x == 0) c->cur[V.epoch & 1] = canon ? 0 : out.buf;
@@ -261,27 +261,23 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   return out;
 }
 
-// Mixed form, end of a V-cycle: the log of every fixed-length visit,
-// evaluated off the V-cycle's critical path.  sors.f90:371-377 over the
-// sweeps the visit ran: -k where the test would have stopped, or nexec when
-// it would have gone on (the host's schedule for the next V-cycle).  The
-// down and up visits of a level keep their partial sums in separate rings,
-// so both are still there.  One wave per visit.
+// Mixed form, end of a V-cycle: the log of a fixed-length visit, evaluated
+// off the V-cycle's critical path (spare blocks of the next k_resid64).
+// sors.f90:371-377 over the sweeps the visit ran: -k where the test would
+// have stopped, or nexec when it would have gone on (the host's schedule for
+// the next V-cycle).  The down and up visits of a level keep their partial
+// sums in separate rings, so both are still there.  Wave 0, all lanes.
 struct VisitLogArgs {
   int n;
   VisitRef v[2 * kMaxLevels];
 };
 
-__global__ __launch_bounds__(256) void k_visit_logs(VisitLogArgs A) {
-  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  for (int q = w; q < A.n; q += nw) {
-    const VisitRef& V = A.v[q];
-    double r = 0.0;
-    const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
-    if ((threadIdx.x & 63) == 0) {
-      V.visit[V.ig] = ks ? -ks : V.nexec;
-      V.vres[V.ig] = r;
-    }
+__device__ void visit_log(const VisitRef& V) {
+  double r = 0.0;
+  const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
+  if (threadIdx.x == 0) {
+    V.visit[V.ig] = ks ? -ks : V.nexec;
+    V.vres[V.ig] = r;
   }
 }
 
@@ -657,14 +653,20 @@ __global__ __launch_bounds__(256) void k_add_corr(AddCorrArgs A) {
 
 // FP64 L1 residual of phi on the finest level (the reference's definition,
 // sors.f90:348-349 order); in mixed mode it also becomes the FP32 right-hand
-// side of the next correction equation.  One partial sum per block; blocks
-// walk rows j = blockIdx.x + 1 + m*gridDim.x.  zero_buf >= 0 also clears the
-// level-1 FP32 correction buffer the next visit starts from.
+// side of the next correction equation.  One partial sum per block; the
+// first gridDim.x - logs.n blocks walk rows j = blockIdx.x + 1 + m*nres,
+// the last logs.n write the logs of the V-cycle just run.  erot also clears
+// the level-1 FP32 correction buffer the next visit starts from.
 __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ u, const double* __restrict__ b,
                                                  const int8_t* __restrict__ bc, int n1, int n2, double aw,
                                                  double an, double ac, float* __restrict__ bf,
                                                  float* const* __restrict__ erot, const LevelCtrl* ctrl,
-                                                 int epoch_next, double* __restrict__ partials) {
+                                                 int epoch_next, double* __restrict__ partials, VisitLogArgs logs) {
+  const int nres = (int)gridDim.x - logs.n;
+  if ((int)blockIdx.x >= nres) {   // block-uniform
+    if (threadIdx.x < 64) visit_log(logs.v[blockIdx.x - nres]);
+    return;
+  }
   __shared__ double ws[4];
   __shared__ int s_zb;
   if (threadIdx.x == 0) s_zb = erot ? (ctrl ? ctrl->cur[(epoch_next - 1) & 1] : 0) : -1;
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ u, c
   float* ez = s_zb >= 0 ? erot[s_zb] : nullptr;
   const int64_t W = n1 + 3;
   double r = 0.0;
-  for (int j = blockIdx.x + 1; j <= n2 + 1; j += gridDim.x) {
+  for (int j = blockIdx.x + 1; j <= n2 + 1; j += nres) {
     for (int i = threadIdx.x + 1; i <= n1 + 1; i += blockDim.x) {
       const int64_t c = (int64_t)j * W + i;
       const int64_t q = (int64_t)(j - 1) * (n1 + 1) + (i - 1);
@@ -1406,7 +1408,7 @@ int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
 // One V-cycle (mg.f90:85-225) at precision T: levels 1..Ls-1 multi-block,
 // Ls..ng in k_tail; then the end-of-cycle kernel of the mode.
 template <typename T>
-int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
+int vcycle(lepic_ctx* ctx, bool f32, int k, double eps, VisitLogArgs* logs = nullptr) {
   const int ng = ctx->ng, Ls = ctx->Lc;
   const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
   ctx->mg_epoch += 2;
@@ -1443,13 +1445,14 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     const VisitPlan p1 = visit_plan(L1, true, true);
     A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed) : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
-    VisitLogArgs V{};
-    for (int l = 2; l < Ls; ++l) {   // level 1 runs fixed counts without residuals
+    // the visits whose logs the next k_resid64 writes; level 1 runs fixed
+    // counts without residuals
+    if (logs) logs->n = 0;
+    for (int l = 2; l < Ls && logs; ++l) {
       const VisitPlan pd = visit_plan(ctx->lv[l], false, true), pu = visit_plan(ctx->lv[l], true, true);
-      V.v[V.n++] = visit_ref(ctx, l, e_dn, l - ng, kNr, l, eps, pd.k_end, pd.klast0, pd.fixed);
-      V.v[V.n++] = visit_ref(ctx, l, e_up, ng - l, kNr, 2 * ng - l, eps, pu.k_end, pu.klast0, pu.fixed);
+      logs->v[logs->n++] = visit_ref(ctx, l, e_dn, l - ng, kNr, l, eps, pd.k_end, pd.klast0, pd.fixed);
+      logs->v[logs->n++] = visit_ref(ctx, l, e_up, ng - l, kNr, 2 * ng - l, eps, pu.k_end, pu.klast0, pu.fixed);
     }
-    if (V.n) hipLaunchKernelGGL(k_visit_logs, dim3(1), dim3(256), 0, ctx->stream, V);
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
@@ -1463,15 +1466,18 @@ double alpha_of(lepic_ctx* ctx) {
 int resid_blocks(lepic_ctx* ctx) { return std::min(ctx->g.n2 + 1, kResidBlocks); }
 
 // FP64 residual of phi; fills the FP32 right-hand side (and clears the
-// correction the next visit starts from) when fill_f32
-int launch_resid64(lepic_ctx* ctx, bool fill_f32, const double* u = nullptr) {
+// correction the next visit starts from) when fill_f32; `logs` (mixed form,
+// after a V-cycle) adds the blocks that write that V-cycle's visit logs
+int launch_resid64(lepic_ctx* ctx, bool fill_f32, const double* u = nullptr, const VisitLogArgs* logs = nullptr) {
   Level& L = ctx->lv[1];
   float* const* erot = nullptr;
   if (fill_f32) erot = (float* const*)ctx->d_erot;
-  hipLaunchKernelGGL(k_resid64, dim3(resid_blocks(ctx)), dim3(256), 0, ctx->stream, u ? u : (const double*)ctx->phi,
-                     (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2, L.aw, L.an, L.ac,
-                     fill_f32 ? L.bf : nullptr, erot, L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr,
-                     ctx->mg_epoch + 1, ctx->d_partials);
+  VisitLogArgs none{};
+  const VisitLogArgs& lg = logs ? *logs : none;
+  hipLaunchKernelGGL(k_resid64, dim3((unsigned)(resid_blocks(ctx) + lg.n)), dim3(256), 0, ctx->stream,
+                     u ? u : (const double*)ctx->phi, (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2,
+                     L.aw, L.an, L.ac, fill_f32 ? L.bf : nullptr, erot,
+                     L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr, ctx->mg_epoch + 1, ctx->d_partials, lg);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -1711,8 +1717,9 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
     } else {
       // FP32 correction V-cycle on the FP64 residual, then phi += e; the new
       // FP64 residual is both the convergence test and the next right-hand side
-      if (int rc = vcycle<float>(ctx, true, k, alpha * eps)) return rc;
-      if (int rc = launch_resid64(ctx, true)) return rc;
+      VisitLogArgs logs{};
+      if (int rc = vcycle<float>(ctx, true, k, alpha * eps, &logs)) return rc;
+      if (int rc = launch_resid64(ctx, true, nullptr, &logs)) return rc;
       double rs = 0.0;
       if (int rc = read_back(ctx, visit, vres, &rs)) return rc;
       vres[2 * ng - 1] = rs;
