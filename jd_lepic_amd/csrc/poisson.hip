@@ -65,6 +65,7 @@ constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 // argon box 4 -> 3 V-cycles per solve; 1024x512 ion source 7.9 -> 7.7.
 constexpr int kFineDn = 4, kFineUp = 3;
 constexpr float kFineOmega = 1.15f;
+constexpr int kTailFixUp = 2;                 // mixed form: up sweeps of the block-wide tail levels
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
 
@@ -720,6 +721,7 @@ struct TLevel {
 template <typename T>
 struct TailArgs {
   int Ls, ng, k;
+  int fix_dn, fix_up;      // block-wide levels: fixed down / up sweep counts (0: the exit test)
   T omega;
   double eps;
   const int8_t* bc_img;    // bc arrays of the LDS-resident levels, in their LDS layout at offset 0
@@ -855,7 +857,7 @@ __device__ __forceinline__ double tail_sum(double v, double* ws, int par) {
 
 template <typename T, int R, bool WAVE, typename PU, typename PB, typename PC>
 __device__ __forceinline__ void t_smooth_fused(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega, double eps,
-                               int dig, int nr, double* ws, int* ksor_out, double* res_out) {
+                               int dig, int nr, double* ws, int* ksor_out, double* res_out, int nfix) {
   const int W = n1 + 3, n1p = n1 + 1;
   const int nint = n1p * (n2 + 1);
   const int P = WAVE ? 64 : (int)blockDim.x;
@@ -931,6 +933,18 @@ __device__ __forceinline__ void t_smooth_fused(PU u, PB b, PC bc, int n1, int n2
     }
     return rs;
   };
+  if (nfix > 0) {  // a fixed count without residuals or exit test (mixed form)
+    for (int k = 1; k <= nfix; ++k) {
+      red_pass(false);
+      store_red();
+      tail_sync<WAVE>();
+      black_pass();
+      tail_sync<WAVE>();
+    }
+    *ksor_out = nfix;
+    *res_out = 0.0;
+    return;
+  }
   // sweep 1
   red_pass(false);
   store_red();
@@ -975,24 +989,27 @@ __device__ __forceinline__ int fused_items(int n1, int n2, int P) {
 // that runs each launch.
 template <typename T, bool WAVE, typename PU, typename PB, typename PC>
 __device__ __forceinline__ bool t_smooth_fused_any(PU u, PB b, PC bc, int n1, int n2, T aw, T an, T ac, T omega,
-                                                   double eps, int dig, int nr, double* ws, int* ks, double* res) {
+                                                   double eps, int dig, int nr, double* ws, int* ks, double* res,
+                                                   int nfix) {
   const int R = fused_items(n1, n2, WAVE ? 64 : (int)blockDim.x);
   if (R == 1) {
-    t_smooth_fused<T, 1, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res);
+    t_smooth_fused<T, 1, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res, nfix);
     return true;
   }
   if (R == 2 || R == 3) {
-    t_smooth_fused<T, 3, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res);
+    t_smooth_fused<T, 3, WAVE>(u, b, bc, n1, n2, aw, an, ac, omega, eps, dig, nr, ws, ks, res, nfix);
     return true;
   }
   return false;
 }
 
+// nfix > 0: exactly nfix sweeps, no residuals (LDS-resident levels of the
+// mixed form); 0: the reference's exit test
 template <typename T, bool WAVE = false>
 __device__ __forceinline__ void smooth_level(const TLevel<T>& L, T* u, T* b, const int8_t* bc, T omega, double eps,
-                                             int dig, int nr, double* ws, int* ks, double* res) {
+                                             int dig, int nr, double* ws, int* ks, double* res, int nfix = 0) {
   if (L.lds && t_smooth_fused_any<T, WAVE>((lds_p<T>)u, (lds_p<const T>)b, (lds_p<const int8_t>)bc, L.n1, L.n2,
-                                           L.aw, L.an, L.ac, omega, eps, dig, nr, ws, ks, res))
+                                           L.aw, L.an, L.ac, omega, eps, dig, nr, ws, ks, res, nfix))
     return;
   if constexpr (!WAVE) {
     if (L.lds)
@@ -1145,7 +1162,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
   const int tid = threadIdx.x, P = blockDim.x;
   for (int l = Ls; l < ng && l < Lw; ++l) {  // down leg: ig = l
     const TLevel<T>& F = A.lv[l];
-    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
+    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res, A.fix_dn);
     if (tid == 0) { A.visit[l] = ks; A.vres[l] = res; }
     tail_restrict<T>(F, A.lv[l + 1], sU[l], sB[l], sBC[l], sU[l + 1], sB[l + 1], sBC[l + 1], tid, P);
     __syncthreads();
@@ -1180,7 +1197,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
     const TLevel<T>& F = A.lv[l];
     tail_prolong<T>(F, A.lv[l + 1], sU[l], sBC[l], (const T*)sU[l + 1], tid, P);
     __syncthreads();
-    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
+    smooth_level<T>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res, A.fix_up);
     if (tid == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
   }
   const TLevel<T>& L = A.lv[Ls];
@@ -1344,6 +1361,11 @@ template <typename T>
 int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
   TailArgs<T> A{};
   A.Ls = ctx->Lc; A.ng = ctx->ng; A.k = k; A.omega = (T)1.4; A.eps = eps;
+  // mixed form: the block-wide tail levels (more than kWaveNodes nodes) run
+  // the counts the exit test gives them on plasma solves, without the
+  // per-sweep block sums the test needs
+  A.fix_dn = f32 ? kNr : 0;
+  A.fix_up = f32 ? kTailFixUp : 0;
   A.visit = d_visit_of(ctx);
   A.vres = d_vres_of(ctx);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
