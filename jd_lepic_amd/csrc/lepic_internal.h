@@ -50,8 +50,9 @@ struct Level {
   void* rot[kRot] = {};
   int nbx = 0, nby = 0;
   double* partials = nullptr;
-  // sweeps of the first launch of the level's down and up visits, from the
-  // number each visit took in the previous V-cycle
+  // FP64: sweeps of the first launch of the level's down and up visits, one
+  // more than each visit took in the previous V-cycle; mixed: the visits'
+  // fixed sweep counts
   int first_dn = kSweepsPerLaunch, first_up = kSweepsPerLaunch;
 };
 

@@ -55,17 +55,20 @@ constexpr int kResidBlocks = 512;             // blocks of the FP64 finest-level
 constexpr size_t kPartialsOff = (64 * 8 + (kVisitLog + 1) * sizeof(int) + 7) / 8 * 8;
 static_assert(kPartialsOff - 32 * 8 + kResidBlocks * 8 <= kPinnedDoubles * 8, "read-back fits the pinned buffer");
 constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
-// Mixed form, finest level when it is multi-block: the FP32 correction
-// V-cycle is a preconditioner inside FP64 refinement to the unchanged
-// tolerance, so its finest-level smoother is chosen for smoothing, not
-// taken from the reference: a fixed 4 down / 3 up red-black sweeps at
-// omega 1.15 (the reference's omega 1.4, iterative_solver.f90:27, damps the
-// highest frequencies less; its exit test stops these visits after 2 sweeps
-// on plasma solves and runs them to 10 near electrodes).  Measured: 1024^2
-// argon box 4 -> 3 V-cycles per solve; 1024x512 ion source 7.9 -> 7.7.
+// Mixed form: the FP32 correction V-cycle is a preconditioner inside FP64
+// refinement to the unchanged tolerance, so its schedule is chosen, not
+// taken from the reference's exit test (sors.f90:371-377).  The finest level
+// (when multi-block) runs a fixed 4 down / 3 up red-black sweeps at omega
+// 1.15: the reference's omega 1.4 (iterative_solver.f90:27) damps the
+// highest frequencies less, and its exit test stops these visits after 2
+// sweeps on plasma solves and runs them to 10 near electrodes.  The other
+// multi-block levels and the block-wide tail levels run the 10 down / 2 up
+// sweeps the test gives them on plasma solves, without the per-sweep
+// residuals and block sums the test needs; the small tail levels (one wave)
+// keep the test.  Measured: 1024^2 argon box 4 -> 3 V-cycles per solve.
 constexpr int kFineDn = 4, kFineUp = 3;
 constexpr float kFineOmega = 1.15f;
-constexpr int kTailFixUp = 2;                 // mixed form: up sweeps of the block-wide tail levels
+constexpr int kFixDn = kNr, kFixUp = 2;
 constexpr int kWaveNodes = 320;               // tail levels with at most this many interior nodes run in one wave
 constexpr int kRowBlocks = 1024;              // row-loop blocks of the grid-transfer kernels (one row each up to 1024 rows)
 
@@ -216,16 +219,6 @@ __device__ int first_stop(const double* partials, int nb, int kf, int kt, int di
   return 0;
 }
 
-// the same over any range of sweeps still in the ring, kS at a time
-__device__ int first_stop_range(const double* partials, int nb, int kf, int kt, int dig, double eps, int nr,
-                                double* r_out) {
-  for (int k = kf; k <= kt; k += kS) {
-    const int ks = first_stop(partials, nb, k, min(k + kS - 1, kt), dig, eps, nr, r_out);
-    if (ks) return ks;
-  }
-  return 0;
-}
-
 struct Resolved {
   int ks;                  // sweeps of the visit (0: not finished)
   int buf;                 // rotating buffer holding the state after them
@@ -240,8 +233,8 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
   const int c0 = c->cur[(V.epoch - 1) & 1];
   Resolved out{0, -1, 0.0};
   if (V.fixed) {
-    // exactly nexec sweeps ran and the state after them is the result; the
-    // visit's log is written after the V-cycle by spare blocks of k_resid64
+    // exactly nexec sweeps ran and the state after them is the result (the
+    // host logs the count)
     out.ks = V.nexec;
     out.buf = (c0 + V.nexec) % kRot;
     if (writer && threadIdx.x == 0) c->cur[V.epoch & 1] = canon ? 0 : out.buf;
@@ -260,26 +253,6 @@ __device__ Resolved resolve_visit(const VisitRef& V, bool writer, bool canon) {
     V.vres[V.ig] = out.res;
   }
   return out;
-}
-
-// Mixed form, end of a V-cycle: the log of a fixed-length visit, evaluated
-// off the V-cycle's critical path (spare blocks of the next k_resid64).
-// sors.f90:371-377 over the sweeps the visit ran: -k where the test would
-// have stopped, or nexec when it would have gone on (the host's schedule for
-// the next V-cycle).  The down and up visits of a level keep their partial
-// sums in separate rings, so both are still there.  Wave 0, all lanes.
-struct VisitLogArgs {
-  int n;
-  VisitRef v[2 * kMaxLevels];
-};
-
-__device__ void visit_log(const VisitRef& V) {
-  double r = 0.0;
-  const int ks = first_stop_range(V.partials, V.nb, 1, V.nexec, V.dig, V.eps, V.nr, &r);
-  if (threadIdx.x == 0) {
-    V.visit[V.ig] = ks ? -ks : V.nexec;
-    V.vres[V.ig] = r;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -654,20 +627,14 @@ __global__ __launch_bounds__(256) void k_add_corr(AddCorrArgs A) {
 
 // FP64 L1 residual of phi on the finest level (the reference's definition,
 // sors.f90:348-349 order); in mixed mode it also becomes the FP32 right-hand
-// side of the next correction equation.  One partial sum per block; the
-// first gridDim.x - logs.n blocks walk rows j = blockIdx.x + 1 + m*nres,
-// the last logs.n write the logs of the V-cycle just run.  erot also clears
-// the level-1 FP32 correction buffer the next visit starts from.
+// side of the next correction equation.  One partial sum per block; blocks
+// walk rows j = blockIdx.x + 1 + m*gridDim.x.  erot also clears the level-1
+// FP32 correction buffer the next visit starts from.
 __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ u, const double* __restrict__ b,
                                                  const int8_t* __restrict__ bc, int n1, int n2, double aw,
                                                  double an, double ac, float* __restrict__ bf,
                                                  float* const* __restrict__ erot, const LevelCtrl* ctrl,
-                                                 int epoch_next, double* __restrict__ partials, VisitLogArgs logs) {
-  const int nres = (int)gridDim.x - logs.n;
-  if ((int)blockIdx.x >= nres) {   // block-uniform
-    if (threadIdx.x < 64) visit_log(logs.v[blockIdx.x - nres]);
-    return;
-  }
+                                                 int epoch_next, double* __restrict__ partials) {
   __shared__ double ws[4];
   __shared__ int s_zb;
   if (threadIdx.x == 0) s_zb = erot ? (ctrl ? ctrl->cur[(epoch_next - 1) & 1] : 0) : -1;
@@ -675,7 +642,7 @@ __global__ __launch_bounds__(256) void k_resid64(const double* __restrict__ u, c
   float* ez = s_zb >= 0 ? erot[s_zb] : nullptr;
   const int64_t W = n1 + 3;
   double r = 0.0;
-  for (int j = blockIdx.x + 1; j <= n2 + 1; j += nres) {
+  for (int j = blockIdx.x + 1; j <= n2 + 1; j += gridDim.x) {
     for (int i = threadIdx.x + 1; i <= n1 + 1; i += blockDim.x) {
       const int64_t c = (int64_t)j * W + i;
       const int64_t q = (int64_t)(j - 1) * (n1 + 1) + (i - 1);
@@ -1226,7 +1193,7 @@ VisitRef visit_ref(lepic_ctx* ctx, int l, int epoch, int dig, int nr, int ig, do
   VisitRef v;
   v.ctrl = ctx->d_ctrl + l;
   v.nb = L.nbx * L.nby;
-  v.partials = L.partials + (dig > 0 ? (size_t)kRing * v.nb : 0);   // up visits: second ring
+  v.partials = L.partials;
   v.epoch = epoch; v.dig = dig; v.nr = nr; v.ig = ig; v.eps = eps;
   v.nexec = nexec;
   v.klast0 = klast0 >= 0 ? klast0 : ((nexec - 1) / kS) * kS;
@@ -1249,10 +1216,9 @@ int last_launch_start(int k0, int k_end, int first) {
 
 // How a multi-block level's visit runs.  FP64: the reference's exit test
 // (sors.f90:371-377) decides, the first launch `first` sweeps and
-// continuations up to kNr.  Mixed: the visit runs exactly the number of
-// sweeps the test gave the same visit in the previous V-cycle (the count
-// lags one V-cycle, no continuation launch tests anything); the finest
-// level's visits run kFineDn / kFineUp sweeps.
+// continuations up to kNr.  Mixed: the visit runs its fixed count
+// (kFineDn / kFineUp on the finest level, kFixDn / kFixUp below) and no
+// continuation launch tests anything.
 struct VisitPlan {
   int k_end, first, klast0, fixed;
 };
@@ -1290,12 +1256,12 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   A.aw = (T)L.aw; A.an = (T)L.an; A.ac = (T)L.ac;
   A.omega = (f32 && l == 1) ? (T)kFineOmega : (T)1.4;   // iterative_solver.f90:27
   A.ctrl = ctx->d_ctrl + l;
-  A.partials = L.partials + (dig > 0 ? (size_t)kRing * L.nbx * L.nby : 0);   // up visits: second ring
+  A.partials = L.partials;
   A.nbx = L.nbx; A.nby = L.nby;
   A.epoch = epoch; A.dig = dig; A.nr = nr; A.eps = eps;
   A.fixed = fixed;
-  // the mixed form's finest-level visits run fixed counts: their residuals feed nothing
-  A.nores = (f32 && l == 1 && fixed) ? 1 : 0;
+  // the mixed form's visits run fixed counts: their residuals feed nothing
+  A.nores = (f32 && fixed) ? 1 : 0;
   int kchk = 0;
   for (int k = k0, size = std::min(std::max(first, 1), kS); k < k_end; size = kS) {
     const int kexec = std::min(size, k_end - k);
@@ -1364,8 +1330,8 @@ int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
   // mixed form: the block-wide tail levels (more than kWaveNodes nodes) run
   // the counts the exit test gives them on plasma solves, without the
   // per-sweep block sums the test needs
-  A.fix_dn = f32 ? kNr : 0;
-  A.fix_up = f32 ? kTailFixUp : 0;
+  A.fix_dn = f32 ? kFixDn : 0;
+  A.fix_up = f32 ? kFixUp : 0;
   A.visit = d_visit_of(ctx);
   A.vres = d_vres_of(ctx);
   auto al = [](size_t x) { return (x + 15) & ~size_t(15); };
@@ -1430,7 +1396,7 @@ int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
 // One V-cycle (mg.f90:85-225) at precision T: levels 1..Ls-1 multi-block,
 // Ls..ng in k_tail; then the end-of-cycle kernel of the mode.
 template <typename T>
-int vcycle(lepic_ctx* ctx, bool f32, int k, double eps, VisitLogArgs* logs = nullptr) {
+int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
   const int ng = ctx->ng, Ls = ctx->Lc;
   const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
   ctx->mg_epoch += 2;
@@ -1467,14 +1433,6 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps, VisitLogArgs* logs = nul
     const VisitPlan p1 = visit_plan(L1, true, true);
     A.v = L1.mb ? visit_ref(ctx, 1, e_up, ng - 1, kNr, 2 * ng - 1, eps, p1.k_end, p1.klast0, p1.fixed) : VisitRef{};
     hipLaunchKernelGGL(k_add_corr, dim3(gx, std::min(L1.n2 + 1, kRowBlocks)), dim3(256), 0, ctx->stream, A);
-    // the visits whose logs the next k_resid64 writes; level 1 runs fixed
-    // counts without residuals
-    if (logs) logs->n = 0;
-    for (int l = 2; l < Ls && logs; ++l) {
-      const VisitPlan pd = visit_plan(ctx->lv[l], false, true), pu = visit_plan(ctx->lv[l], true, true);
-      logs->v[logs->n++] = visit_ref(ctx, l, e_dn, l - ng, kNr, l, eps, pd.k_end, pd.klast0, pd.fixed);
-      logs->v[logs->n++] = visit_ref(ctx, l, e_up, ng - l, kNr, 2 * ng - l, eps, pu.k_end, pu.klast0, pu.fixed);
-    }
   }
   LEPIC_HIP(hipGetLastError());
   return 0;
@@ -1488,18 +1446,15 @@ double alpha_of(lepic_ctx* ctx) {
 int resid_blocks(lepic_ctx* ctx) { return std::min(ctx->g.n2 + 1, kResidBlocks); }
 
 // FP64 residual of phi; fills the FP32 right-hand side (and clears the
-// correction the next visit starts from) when fill_f32; `logs` (mixed form,
-// after a V-cycle) adds the blocks that write that V-cycle's visit logs
-int launch_resid64(lepic_ctx* ctx, bool fill_f32, const double* u = nullptr, const VisitLogArgs* logs = nullptr) {
+// correction the next visit starts from) when fill_f32
+int launch_resid64(lepic_ctx* ctx, bool fill_f32, const double* u = nullptr) {
   Level& L = ctx->lv[1];
   float* const* erot = nullptr;
   if (fill_f32) erot = (float* const*)ctx->d_erot;
-  VisitLogArgs none{};
-  const VisitLogArgs& lg = logs ? *logs : none;
-  hipLaunchKernelGGL(k_resid64, dim3((unsigned)(resid_blocks(ctx) + lg.n)), dim3(256), 0, ctx->stream,
-                     u ? u : (const double*)ctx->phi, (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2,
-                     L.aw, L.an, L.ac, fill_f32 ? L.bf : nullptr, erot,
-                     L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr, ctx->mg_epoch + 1, ctx->d_partials, lg);
+  hipLaunchKernelGGL(k_resid64, dim3(resid_blocks(ctx)), dim3(256), 0, ctx->stream, u ? u : (const double*)ctx->phi,
+                     (const double*)ctx->rhs, (const int8_t*)ctx->bc, L.n1, L.n2, L.aw, L.an, L.ac,
+                     fill_f32 ? L.bf : nullptr, erot, L.mb ? (const LevelCtrl*)(ctx->d_ctrl + 1) : nullptr,
+                     ctx->mg_epoch + 1, ctx->d_partials);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
@@ -1591,11 +1546,9 @@ int poisson_setup(lepic_ctx* ctx) {
   for (int l = 1; l < ctx->Lc && l <= ng; ++l) {
     Level& L = ctx->lv[l];
     L.mb = true;
-    // the mixed form's visits start at the reference's sweep limit and adapt
-    // from the exit test's outcome (the finest level's are fixed); FP64
-    // first launches start at kS
-    L.first_dn = mixed ? (l == 1 ? kFineDn : kNr) : kS;
-    L.first_up = mixed ? (l == 1 ? kFineUp : kNr) : kS;
+    // the mixed form's visits run fixed counts; FP64 first launches start at kS
+    L.first_dn = mixed ? (l == 1 ? kFineDn : kFixDn) : kS;
+    L.first_up = mixed ? (l == 1 ? kFineUp : kFixUp) : kS;
     const size_t nn = (size_t)(L.n1 + 3) * (L.n2 + 3);
     L.rot[0] = mixed ? (void*)L.uf : (void*)L.u;
     for (int r = 1; r < kRot; ++r) {
@@ -1612,7 +1565,7 @@ int poisson_setup(lepic_ctx* ctx) {
       L.nbx = tiles_of(L.n1 + 1, kSmallTX);
       L.nby = tiles_of(L.n2 + 1, sty);
     }
-    const size_t np = 2 * (size_t)kRing * L.nbx * L.nby;   // down-visit ring, up-visit ring
+    const size_t np = (size_t)kRing * L.nbx * L.nby;
     LEPIC_HIP(hipMalloc(&L.partials, np * 8));
     LEPIC_HIP(hipMemsetAsync(L.partials, 0, np * 8, ctx->stream));
   }
@@ -1739,9 +1692,8 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
     } else {
       // FP32 correction V-cycle on the FP64 residual, then phi += e; the new
       // FP64 residual is both the convergence test and the next right-hand side
-      VisitLogArgs logs{};
-      if (int rc = vcycle<float>(ctx, true, k, alpha * eps, &logs)) return rc;
-      if (int rc = launch_resid64(ctx, true, nullptr, &logs)) return rc;
+      if (int rc = vcycle<float>(ctx, true, k, alpha * eps)) return rc;
+      if (int rc = launch_resid64(ctx, true)) return rc;
       double rs = 0.0;
       if (int rc = read_back(ctx, visit, vres, &rs)) return rc;
       vres[2 * ng - 1] = rs;
@@ -1756,19 +1708,9 @@ int poisson_solve(lepic_ctx* ctx, double eps, int ncycl, int start, lepic_solve_
           // continuation launch when the count grows by one
           L.first_dn = std::min(std::max(visit[l] + 1, 2), kS);
           L.first_up = std::min(std::max(visit[2 * ng - l] + 1, 2), kS);
-        } else {
-          // the visit ran first_* sweeps; the log says where the exit test
-          // stopped (-k) or that it would have gone on (the count run)
-          auto next = [](int logged, int ran) {
-            return logged < 0 ? std::max(-logged, 2) : std::min(ran + 2, kNr);
-          };
-          const int dn = visit[l], up = visit[2 * ng - l];
+        } else {   // the mixed form's fixed counts
           visit[l] = L.first_dn;
           visit[2 * ng - l] = L.first_up;
-          if (l > 1) {   // the finest level's counts are fixed
-            L.first_dn = next(dn, L.first_dn);
-            L.first_up = next(up, L.first_up);
-          }
         }
       }
     for (int ig = 1; ig <= nvis; ++ig) {
