@@ -198,14 +198,17 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
     const int shift = 8 * p;
     for (int e = t; e < 4 * 256; e += kSegThreads) (&s_wc[0][0])[e] = 0u;
     __syncthreads();
-    uint32_t d[kSegRounds];
+    uint32_t d[kSegRounds], pos[kSegRounds];
     bool v[kSegRounds];
 #pragma unroll
     for (int r = 0; r < kSegRounds; ++r) {
       const int q = w * 256 + r * 64 + lane;
       v[r] = q < cnt;
       d[r] = v[r] ? ((s_k[cur][q] - kmin) >> shift) & 255u : 0u;
-      wave_match<8>(v[r], d[r], s_wc[w]);
+      // the wave's items with my digit in earlier rounds (read before this
+      // round's counts are added) plus those before me in this round
+      const uint32_t before = v[r] ? s_wc[w][d[r]] : 0u;
+      pos[r] = before + wave_match<8>(v[r], d[r], s_wc[w]);
     }
     __syncthreads();
     // digit t: first position of each wave's items with that digit
@@ -220,11 +223,10 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
 #pragma unroll
     for (int r = 0; r < kSegRounds; ++r) {
       const int q = w * 256 + r * 64 + lane;
-      const uint32_t b = v[r] ? s_wc[w][d[r]] : 0u;   // read before this round's counts are added
-      const uint32_t rk = wave_match<8>(v[r], d[r], s_wc[w]);
       if (v[r]) {
-        s_k[1 - cur][b + rk] = s_k[cur][q];
-        s_i[1 - cur][b + rk] = s_i[cur][q];
+        const uint32_t dst = s_wc[w][d[r]] + pos[r];
+        s_k[1 - cur][dst] = s_k[cur][q];
+        s_i[1 - cur][dst] = s_i[cur][q];
       }
     }
     __syncthreads();
