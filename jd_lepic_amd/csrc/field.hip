@@ -4,6 +4,8 @@
 // reference's column-major layout (ix fastest, coalesced along x).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cstring>
 
 #include "lepic_internal.h"
@@ -77,18 +79,36 @@ __global__ void k_to_f32(const double* __restrict__ a, float* __restrict__ b, in
 
 // dens_red for one partition: np_red = 0 + np on nodes 1..n+1 (exact copy),
 // ghosts zero; np_mx = max over interior nodes (values are non-negative, so
-// the IEEE bit patterns order like the values).
-__global__ void k_dens_red(const double* __restrict__ np, double* __restrict__ red,
-                           const int8_t* __restrict__ bc, int n1, int n2,
-                           unsigned long long* __restrict__ mx) {
-  int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  int iy = blockIdx.y;
-  if (ix > n1 + 2) return;
-  const int64_t nd = node_index(ix, iy, n1);
-  bool in = ix >= 1 && ix <= n1 + 1 && iy >= 1 && iy <= n2 + 1;
-  double v = in ? 0.0 + np[nd] : 0.0;
-  red[nd] = v;
-  if (in && bc[nd] == -1 && v > 0.0) atomicMax(mx, (unsigned long long)__double_as_longlong(v));
+// the IEEE bit patterns order like the values).  Block b walks rows b,
+// b + gridDim.x, ...; the maximum is taken over the block first and one
+// atomic per block goes to the shared address (atomics on one address
+// serialise; a maximum is exact in any order).
+__global__ __launch_bounds__(256) void k_dens_red(const double* __restrict__ np, double* __restrict__ red,
+                                                  const int8_t* __restrict__ bc, int n1, int n2,
+                                                  unsigned long long* __restrict__ mx) {
+  __shared__ unsigned long long ws[4];
+  unsigned long long bits = 0ull;
+  for (int iy = blockIdx.x; iy <= n2 + 2; iy += gridDim.x)
+    for (int ix = threadIdx.x; ix <= n1 + 2; ix += blockDim.x) {
+      const int64_t nd = node_index(ix, iy, n1);
+      const bool in = ix >= 1 && ix <= n1 + 1 && iy >= 1 && iy <= n2 + 1;
+      const double v = in ? 0.0 + np[nd] : 0.0;
+      red[nd] = v;
+      if (in && bc[nd] == -1 && v > 0.0) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+        bits = b > bits ? b : bits;
+      }
+    }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long t = __shfl_xor(bits, o, 64);
+    bits = t > bits ? t : bits;
+  }
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) bits = ws[w] > bits ? ws[w] : bits;
+    if (bits) atomicMax(mx, bits);
+  }
 }
 
 }  // namespace
@@ -128,9 +148,9 @@ int dens_red_launch(lepic_ctx* ctx, double* np_mx_host) {
   const int n1 = ctx->g.n1, n2 = ctx->g.n2;
   unsigned long long* mx = (unsigned long long*)ctx->d_mcc_i64 + 32;  // 8 slots
   if (int rc = zero_async(ctx, mx, kMaxSpecies * 8)) return rc;
-  dim3 grid((n1 + 3 + 127) / 128, n2 + 3);
+  const dim3 grid((unsigned)std::min(n2 + 3, 256));
   for (int p = 0; p < ctx->nsp; ++p)
-    hipLaunchKernelGGL(k_dens_red, grid, dim3(128), 0, ctx->stream, ctx->sp[p].np, ctx->sp[p].np_red,
+    hipLaunchKernelGGL(k_dens_red, grid, dim3(256), 0, ctx->stream, ctx->sp[p].np, ctx->sp[p].np_red,
                        ctx->bc, n1, n2, mx + p);
   LEPIC_HIP(hipGetLastError());
   LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, mx, kMaxSpecies * 8, hipMemcpyDeviceToHost, ctx->stream));

@@ -384,8 +384,8 @@ int lepic_create(const lepic_grid_desc* g, const lepic_species_desc* sp, lepic_c
   H(hipMemset(ctx->d_error, 0, 64), "error");
   H(hipMalloc(&ctx->d_mcc_i64, 64 * 8), "mcc");
   H(hipMemset(ctx->d_mcc_i64, 0, 64 * 8), "mcc");
-  H(hipMalloc(&ctx->d_mcc_f64, 64 * 8), "mcc");
-  H(hipMemset(ctx->d_mcc_f64, 0, 64 * 8), "mcc");
+  H(hipMalloc(&ctx->d_mcc_f64, (kMccF64Head + kMaxRegions * kRegionBlocks) * 8), "mcc");
+  H(hipMemset(ctx->d_mcc_f64, 0, (kMccF64Head + kMaxRegions * kRegionBlocks) * 8), "mcc");
   H(hipHostMalloc((void**)&ctx->h_pinned, kPinnedDoubles * 8, hipHostMallocDefault), "pinned");
   ctx->bc_host.assign(ctx->nodes, 1);
   if (!rc) rc = poisson_setup(ctx);

@@ -16,6 +16,9 @@ struct Net;
 
 constexpr int kMaxSpecies = 8;
 constexpr int kMaxLevels = 12;
+constexpr int kMccF64Head = 64;       // scalar sums at the head of the MCC FP64 scratch
+constexpr int kRegionBlocks = 256;    // blocks of a region sum (collisions.f90 np_tot_rg)
+constexpr int kMaxRegions = 5;
 constexpr int kMaxReac = 64;
 constexpr int kMaxTypes = 32;
 constexpr int kBlock = 256;          // threads per block of the particle kernels
@@ -213,7 +216,8 @@ struct lepic_ctx {
   void* d_mcc_stage = nullptr;       // staged new particles
   int64_t stage_cap = 0;
   int64_t* d_mcc_i64 = nullptr;      // [64] counters
-  double* d_mcc_f64 = nullptr;       // [64] sums: [0..8] P_loss(3,ptype), [40..44] 1/np_tot_rg
+  double* d_mcc_f64 = nullptr;       // [kMccF64Head] sums: [0..8] P_loss(3,ptype), [40..44] 1/np_tot_rg;
+                                     // then kRegionBlocks block partials per region
   void* d_mcc_tab = nullptr;         // device copy of the reaction table
 
   // learned warm start (unet.hip)

@@ -607,23 +607,28 @@ __global__ void k_add_count(int64_t* counts, int b, const uint32_t* offs, const 
 // fixed-order block reduction (thread partials over a strided walk, then the
 // 16 wave sums in order), so the result is reproducible run to run; check
 // mode keeps the reference's sequential sum inside k_mcc_check.
-__global__ __launch_bounds__(1024) void k_region_sum(const double* __restrict__ np_red, int n1, int ixl,
-                                                     int ixr, int iyl, int iyr, double kk,
-                                                     double* __restrict__ out) {
-  __shared__ double ws[16];
-  const int nx = ixr - ixl + 1, ny = iyr - iyl + 1;
-  const int64_t cnt = (int64_t)nx * ny;
+// 1/np_tot_rg of a region (collisions.f90): the sum of np_red over the
+// region's nodes in two stages, each in a fixed order: block b sums rows
+// iyl + b, iyl + b + kRegionBlocks, ... (lanes along the row), then one wave
+// sums the block partials.
+__global__ __launch_bounds__(256) void k_region_part(const double* __restrict__ np_red, int n1, int ixl, int ixr,
+                                                     int iyl, int iyr, double* __restrict__ part) {
+  __shared__ double ws[4];
   double s = 0.0;
-  for (int64_t q = threadIdx.x; q < cnt; q += blockDim.x)
-    s += np_red[node_index(ixl + (int)(q % nx), iyl + (int)(q / nx), n1)];
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  for (int iy = iyl + (int)blockIdx.x; iy <= iyr; iy += gridDim.x)
+    for (int ix = ixl + (int)threadIdx.x; ix <= ixr; ix += blockDim.x) s += np_red[node_index(ix, iy, n1)];
+  s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += ws[w];
-    *out = 1.0 / (double)(int64_t)(kk * t);
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ __launch_bounds__(64) void k_region_final(const double* __restrict__ part, int np, double kk,
+                                                     double* __restrict__ out) {
+  double s = 0.0;
+  for (int q = threadIdx.x; q < np; q += 64) s += part[q];
+  s = wave_sum(s);
+  if (threadIdx.x == 0) *out = 1.0 / (double)(int64_t)(kk * s);
 }
 
 }  // namespace
@@ -762,9 +767,14 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
       // 1/np_tot_rg per region
       double* d_rg = ctx->d_mcc_f64 + 40;
       const double kk = ctx->g.h1 * ctx->g.h2 * ctx->g.zmax / R.Nm[pt];
-      for (int rg = 0; rg < R.n_rg; ++rg)
-        hipLaunchKernelGGL(k_region_sum, dim3(1), dim3(1024), 0, ctx->stream, (const double*)ctx->sp[q].np_red,
-                           ctx->g.n1, R.ixl_rg[rg], R.ixr_rg[rg], R.iyl_rg, R.iyr_rg, kk, d_rg + rg);
+      for (int rg = 0; rg < R.n_rg; ++rg) {
+        double* part = ctx->d_mcc_f64 + kMccF64Head + (size_t)rg * kRegionBlocks;
+        hipLaunchKernelGGL(k_region_part, dim3(kRegionBlocks), dim3(256), 0, ctx->stream,
+                           (const double*)ctx->sp[q].np_red, ctx->g.n1, R.ixl_rg[rg], R.ixr_rg[rg], R.iyl_rg,
+                           R.iyr_rg, part);
+        hipLaunchKernelGGL(k_region_final, dim3(1), dim3(64), 0, ctx->stream, (const double*)part, kRegionBlocks,
+                           kk, d_rg + rg);
+      }
       LEPIC_HIP(hipMemcpyAsync(ctx->h_pinned, d_rg, 5 * 8, hipMemcpyDeviceToHost, ctx->stream));
       // scratch: claim bitmap, staging
       const int64_t words = (N[q] + 31) / 32;
@@ -863,7 +873,7 @@ extern "C" {
 
 int lepic_set_reactions(lepic_ctx* ctx, const lepic_reactions* r) {
   if (!ctx || !r || r->ncol < 0 || r->ncol > kMaxReac || r->npt < 2 || r->ntype_all < 1 ||
-      r->ntype_all > kMaxTypes || r->n_rg < 0 || r->n_rg > 5)
+      r->ntype_all > kMaxTypes || r->n_rg < 0 || r->n_rg > kMaxRegions)
     return LEPIC_ERR_ARG;
   if (r->ntype_all < ctx->nsp) return ctx->fail(LEPIC_ERR_ARG, "ntype_all < nspecies");
   Reactions& R = ctx->rx;
