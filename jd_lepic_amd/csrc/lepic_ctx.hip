@@ -2,6 +2,8 @@
 // entry points of liblepic_hip.so.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -376,8 +378,8 @@ int lepic_create(const lepic_grid_desc* g, const lepic_species_desc* sp, lepic_c
   H(hipMemset(ctx->Bi, 0, 4 * 4 * 8), "Bi");
   H(hipMalloc(&ctx->d_counts, kMaxSpecies * 8), "counts");
   H(hipMemset(ctx->d_counts, 0, kMaxSpecies * 8), "counts");
-  H(hipMalloc(&ctx->d_tally, kMaxSpecies * sizeof(Tally)), "tally");
-  H(hipMemset(ctx->d_tally, 0, kMaxSpecies * sizeof(Tally)), "tally");
+  H(hipMalloc(&ctx->d_tally, kMaxSpecies * kTallyCopies * sizeof(Tally)), "tally");
+  H(hipMemset(ctx->d_tally, 0, kMaxSpecies * kTallyCopies * sizeof(Tally)), "tally");
   H(hipMalloc(&ctx->d_ticket, 8), "ticket");
   H(hipMemset(ctx->d_ticket, 0, 8), "ticket");
   H(hipMalloc(&ctx->d_error, 64), "error");
@@ -755,10 +757,18 @@ int lepic_charge_deposition(lepic_ctx* ctx, int32_t ptype) {
 
 int lepic_get_mover_tally(lepic_ctx* ctx, int32_t ptype, double* P_loss, double* p_mac, double* cnt_dead) {
   if (!ctx || !valid_ptype(ctx, ptype)) return LEPIC_ERR_ARG;
-  Tally t;
+  std::vector<Tally> c(kTallyCopies);
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
-  LEPIC_HIP(hipMemcpyAsync(&t, ctx->d_tally + (ptype - 1), sizeof(Tally), hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(hipMemcpyAsync(c.data(), ctx->d_tally + (size_t)(ptype - 1) * kTallyCopies, kTallyCopies * sizeof(Tally),
+                           hipMemcpyDeviceToHost, ctx->stream));
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  Tally t{};                         // the copies summed in copy order
+  for (const Tally& u : c) {
+    for (int k = 0; k < 4; ++k) t.P_loss[k] += u.P_loss[k];
+    t.cnt_dead += u.cnt_dead;
+    for (int k = 0; k < 2; ++k)
+      for (int gI = 0; gI < 65; ++gI) t.p_mac[k][gI] += u.p_mac[k][gI];
+  }
   if (P_loss) std::memcpy(P_loss, t.P_loss, 32);
   if (p_mac)
     for (int k = 0; k < 2; ++k)
@@ -769,7 +779,7 @@ int lepic_get_mover_tally(lepic_ctx* ctx, int32_t ptype, double* P_loss, double*
 
 int lepic_reset_tallies(lepic_ctx* ctx) {
   if (!ctx) return LEPIC_ERR_ARG;
-  if (int rc = zero_async(ctx, ctx->d_tally, kMaxSpecies * sizeof(Tally))) return rc;
+  if (int rc = zero_async(ctx, ctx->d_tally, kMaxSpecies * kTallyCopies * sizeof(Tally))) return rc;
   if (ctx->rx.d_cnt_col)
     if (int rc = zero_async(ctx, ctx->rx.d_cnt_col, 3 * (size_t)ctx->rx.ncol * std::max(ctx->rx.n_rg, 1) * 8)) return rc;
   if (int rc = zero_async(ctx, ctx->d_mcc_f64, 64 * 8)) return rc;

@@ -104,6 +104,10 @@ struct Tally {
   double cnt_dead;
   double p_mac[2][65];               // (count, energy) x igrid 0..64
 };
+// Copies of a species' tally: a mover block adds to copy blockIdx % kTallyCopies
+// (atomics on one address serialise; a step loses ~1e4 electrons at the
+// walls), and a read sums the copies in copy order.
+constexpr int kTallyCopies = 64;
 
 struct Reactions {
   bool set = false;
@@ -176,7 +180,7 @@ struct lepic_ctx {
 
   // device scalars
   int64_t* d_counts = nullptr;       // [kMaxSpecies]
-  lepic::Tally* d_tally = nullptr;   // [kMaxSpecies]
+  lepic::Tally* d_tally = nullptr;   // [kMaxSpecies][kTallyCopies]
   uint32_t* d_segoff = nullptr;      // segment offsets for packing (max nseg_cap)
   int64_t segoff_cap = 0;
   unsigned long long* d_ticket = nullptr;   // monotonic block ticket of the movers

@@ -170,7 +170,7 @@ __device__ __forceinline__ void clamp_cell(int& ix, int& iy, int n1, int n2) {
 // Lost-particle tallies, part_expmover.f90:314-332, 424-427.
 __device__ inline void tally_lost(const MoverCommon& a, double xp, double yp, int ix, int iy,
                                   double Eki) {
-  Tally* t = a.tally;
+  Tally* t = a.tally + blockIdx.x % kTallyCopies;
   atomic_add_f64(&t->P_loss[0], Eki);
   int ixw = (int)llround(xp / a.h1) + 1, iyw = (int)llround(yp / a.h2) + 1;
   clamp_cell(ixw, iyw, a.n1, a.n2);
@@ -181,8 +181,9 @@ __device__ inline void tally_lost(const MoverCommon& a, double xp, double yp, in
 }
 
 __device__ inline void tally_dead(const MoverCommon& a, double Eki) {
-  atomic_add_f64(&a.tally->P_loss[2], -Eki);
-  if (a.tagneg) atomic_add_f64(&a.tally->cnt_dead, a.NmQ);
+  Tally* t = a.tally + blockIdx.x % kTallyCopies;
+  atomic_add_f64(&t->P_loss[2], -Eki);
+  if (a.tagneg) atomic_add_f64(&t->cnt_dead, a.NmQ);
 }
 
 // Order-preserving offsets of the survivors inside the block's segment;
@@ -958,7 +959,7 @@ MoverCommon common_args(lepic_ctx* ctx, int p) {
   a.Bc0 = ctx->Bc[0]; a.Bc1 = ctx->Bc[1]; a.Bc2 = ctx->Bc[2];
   a.acc = (unsigned long long*)s.acc;
   a.k1dep = s.Nm / (ctx->g.h1 * ctx->g.h2 * ctx->g.zmax);  // part_expmover.f90:564
-  a.tally = ctx->d_tally + p;
+  a.tally = ctx->d_tally + (size_t)p * kTallyCopies;
   a.sink = s.sink;
   a.ngrid = ctx->g.ngrid;
   a.seg_in = s.seg[s.cur];
