@@ -62,10 +62,10 @@ constexpr int kNr = 10;                       // sweep limit of sors.f90:63-64
 // 1.15: the reference's omega 1.4 (iterative_solver.f90:27) damps the
 // highest frequencies less, and its exit test stops these visits after 2
 // sweeps on plasma solves and runs them to 10 near electrodes.  The other
-// multi-block levels and the block-wide tail levels run the 10 down / 2 up
-// sweeps the test gives them on plasma solves, without the per-sweep
-// residuals and block sums the test needs; the small tail levels (one wave)
-// keep the test.  Measured: 1024^2 argon box 4 -> 3 V-cycles per solve.
+// multi-block levels and the tail levels above the coarsest run the 10 down
+// / 2 up sweeps the test gives them on plasma solves, without the per-sweep
+// residuals and sums the test needs; the coarsest level keeps the test.
+// Measured: 1024^2 argon box 4 -> 3 V-cycles per solve.
 constexpr int kFineDn = 4, kFineUp = 3;
 constexpr float kFineOmega = 1.15f;
 constexpr int kFixDn = kNr, kFixUp = 2;
@@ -688,7 +688,7 @@ struct TLevel {
 template <typename T>
 struct TailArgs {
   int Ls, ng, k;
-  int fix_dn, fix_up;      // block-wide levels: fixed down / up sweep counts (0: the exit test)
+  int fix_dn, fix_up;      // levels above the coarsest: fixed down / up sweep counts (0: the exit test)
   T omega;
   double eps;
   const int8_t* bc_img;    // bc arrays of the LDS-resident levels, in their LDS layout at offset 0
@@ -1138,7 +1138,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
     if (tid < 64) {
       for (int l = Lw; l < ng; ++l) {
         const TLevel<T>& F = A.lv[l];
-        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res);
+        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, l - ng, kNr, ws, &ks, &res, A.fix_dn);
         if (tid == 0) { A.visit[l] = ks; A.vres[l] = res; }
         tail_restrict<T>(F, A.lv[l + 1], sU[l], sB[l], sBC[l], sU[l + 1], sB[l + 1], sBC[l + 1], tid, 64);
         tail_sync<true>();
@@ -1150,7 +1150,7 @@ __global__ __launch_bounds__(kTailThreads) void k_tail(TailArgs<T> A) {
         const TLevel<T>& F = A.lv[l];
         tail_prolong<T>(F, A.lv[l + 1], sU[l], sBC[l], (const T*)sU[l + 1], tid, 64);
         tail_sync<true>();
-        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res);
+        smooth_level<T, true>(F, sU[l], sB[l], sBC[l], A.omega, A.eps, ng - l, kNr, ws, &ks, &res, A.fix_up);
         if (tid == 0) { A.visit[2 * ng - l] = ks; A.vres[2 * ng - l] = res; }
       }
     }
@@ -1327,9 +1327,9 @@ template <typename T>
 int launch_tail(lepic_ctx* ctx, bool f32, int k, double eps) {
   TailArgs<T> A{};
   A.Ls = ctx->Lc; A.ng = ctx->ng; A.k = k; A.omega = (T)1.4; A.eps = eps;
-  // mixed form: the block-wide tail levels (more than kWaveNodes nodes) run
-  // the counts the exit test gives them on plasma solves, without the
-  // per-sweep block sums the test needs
+  // mixed form: the tail levels above the coarsest run the counts the exit
+  // test gives them on plasma solves, without the per-sweep sums the test
+  // needs
   A.fix_dn = f32 ? kFixDn : 0;
   A.fix_up = f32 ? kFixUp : 0;
   A.visit = d_visit_of(ctx);
