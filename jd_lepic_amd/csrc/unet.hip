@@ -17,8 +17,9 @@
 // costs nothing.
 //
 // Convolution as implicit GEMM: one wave computes 16 pixels of a row times
-// all output channels; per tap and 16-channel chunk a lane loads 4 channels
-// of one pixel (8 B) and 4 input channels of one output channel's weights.
+// all output channels, from input rows and weights staged in LDS by
+// channel chunk; per tap and 16-channel step a lane reads 4 channels of one
+// pixel (8 B) and 4 input channels of one output channel's weights.
 // The input gradient is the same kernel with the weights transposed and the
 // taps flipped; the weight gradient reduces over pixels with MFMA (pixels
 // are the K dimension) and adds FP32 partial sums atomically.
@@ -68,43 +69,87 @@ struct ConvArgs {
   int B, H, W;
 };
 
+// A block computes 64 pixels of ROWS rows (a wave per 16 pixels, all rows)
+// times all cout output channels.  Input channels are taken in chunks of
+// CC: the chunk of the ROWS + 2 input rows around the output rows (66
+// pixels, border included) and of the weights (cout x 9 taps) are staged in
+// LDS with all their loads in flight, then every MFMA reads both operands
+// from LDS, each weight fragment serving the wave's ROWS rows.  CC and ROWS
+// shrink as cout grows so the LDS stays near 64 KB and the accumulators in
+// registers; rows are padded by 4 channels against bank conflicts.
+template <int NT> struct ConvShape {
+  static constexpr int CC = NT <= 4 ? 32 : 16;
+  static constexpr int ROWS = NT <= 4 ? 4 : 2;
+  static constexpr int PAD = NT <= 8 ? 4 : 0;              // 192 output channels: no room for the padding
+};
+
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs A) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int x0 = (blockIdx.x * 4 + wv) * 16;
-  const int y = blockIdx.y, b = blockIdx.z;
-  if (x0 >= A.W) return;
+  constexpr int CC = ConvShape<NT>::CC, ROWS = ConvShape<NT>::ROWS, CP = CC + ConvShape<NT>::PAD;
+  constexpr int COUT = 16 * NT, kPx = 66, kRin = ROWS + 2;  // pixels x0-1 .. x0+64, rows y0-1 .. y0+ROWS
+  __shared__ __align__(16) bf16_t s_in[kRin * kPx * CP];
+  __shared__ __align__(16) bf16_t s_w[COUT * 9 * CP];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int x0 = blockIdx.x * 64, y0 = blockIdx.y * ROWS, b = blockIdx.z;
   const int m = lane & 15, g = lane >> 4;
-  const int x = x0 + m;
-  const bool valid = x < A.W;
-  f32x4 acc[NT];
+  f32x4 acc[ROWS][NT];
 #pragma unroll
-  for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int t = 0; t < 9; ++t) {
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    const bf16_t* row = A.in + px(b, y + dy, valid ? x + dx : -1, A.H, A.W) * A.in_C + A.in_off + 4 * g;
-    const bf16_t* wt = A.w + ((size_t)m * 9 + t) * A.cin + 4 * g;
-    for (int c0 = 0; c0 < A.cin; c0 += 16) {
-      bf16x4 a = valid ? *(const bf16x4*)(row + c0) : bf16x4{0, 0, 0, 0};
+  for (int r = 0; r < ROWS; ++r)
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const bf16x4 bw = *(const bf16x4*)(wt + (size_t)16 * j * 9 * A.cin + c0);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[j], 0, 0, 0);
+    for (int j = 0; j < NT; ++j) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < A.cin; c += CC) {
+    const int cc = min(CC, A.cin - c);                     // a multiple of 16
+    const int nv4 = cc / 4;                                // 8-byte vectors per pixel or weight row
+    __syncthreads();                                       // the previous chunk's reads are done
+    // input rows y0-1 .. y0+ROWS (below the bottom border: zero), pixels
+    // x0-1 .. x0+64 (beyond the right border: zero)
+    for (int q = t; q < kRin * kPx * nv4; q += 256) {
+      const int v = q % nv4, pq = q / nv4, xx = pq % kPx, r = pq / kPx;
+      const int gx = x0 - 1 + xx, gy = y0 - 1 + r;
+      bf16x4 val = bf16x4{0, 0, 0, 0};
+      if (gx <= A.W && gy <= A.H) val = *(const bf16x4*)(A.in + px(b, gy, gx, A.H, A.W) * A.in_C + A.in_off + c + 4 * v);
+      *(bf16x4*)(s_in + (r * kPx + xx) * CP + 4 * v) = val;
+    }
+    for (int q = t; q < COUT * 9 * nv4; q += 256) {
+      const int v = q % nv4, row = q / nv4;                // row = co * 9 + tap
+      *(bf16x4*)(s_w + row * CP + 4 * v) = *(const bf16x4*)(A.w + (size_t)row * A.cin + c + 4 * v);
+    }
+    __syncthreads();
+    for (int tap = 0; tap < 9; ++tap) {
+      const int dy = tap / 3, dx = tap % 3;                // input row r + dy for output row r, pixel x-1+dx
+      const bf16_t* arow = s_in + (dy * kPx + 16 * wv + m + dx) * CP + 4 * g;
+      const bf16_t* wrow = s_w + ((size_t)m * 9 + tap) * CP + 4 * g;
+      for (int k0 = 0; k0 < cc; k0 += 16) {
+        bf16x4 a[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) a[r] = *(const bf16x4*)(arow + r * kPx * CP + k0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const bf16x4 bw = *(const bf16x4*)(wrow + (size_t)16 * j * 9 * CP + k0);
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], bw, acc[r][j], 0, 0, 0);
+        }
       }
     }
   }
+  if (x0 + 16 * wv >= A.W) return;
 #pragma unroll
-  for (int j = 0; j < NT; ++j) {
+  for (int r = 0; r < ROWS; ++r) {
+    const int y = y0 + r;
+    if (y >= A.H) break;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int xp = x0 + 4 * g + i;
-      if (xp >= A.W) continue;
-      const int co = 16 * j + m;
-      float v = acc[j][i];
-      if (A.relu) v = fmaxf(v, 0.f);
-      const size_t p = px(b, y, xp, A.H, A.W);
-      if (A.mask && !bf_pos(A.mask[p * A.mask_C + A.mask_off + co])) v = 0.f;
-      A.out[p * A.out_C + A.out_off + co] = f2bf(v);
+    for (int j = 0; j < NT; ++j) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int xp = x0 + 16 * wv + 4 * g + i;
+        if (xp >= A.W) continue;
+        const int co = 16 * j + m;
+        float v = acc[r][j][i];
+        if (A.relu) v = fmaxf(v, 0.f);
+        const size_t p = px(b, y, xp, A.H, A.W);
+        if (A.mask && !bf_pos(A.mask[p * A.mask_C + A.mask_off + co])) v = 0.f;
+        A.out[p * A.out_C + A.out_off + co] = f2bf(v);
+      }
     }
   }
 }
@@ -587,15 +632,15 @@ int conv(lepic_ctx* ctx, Net& N, const T4& in, int in_off, int cin, const bf16_t
   A.out = out.p; A.out_C = out.C; A.out_off = out_off; A.cout = cout;
   A.mask = mask ? mask->p : nullptr; A.mask_C = mask ? mask->C : 0; A.mask_off = mask_off;
   A.relu = relu; A.B = B; A.H = H; A.W = W;
-  const dim3 grid((unsigned)((W + 63) / 64), (unsigned)H, (unsigned)B);
+  auto grid = [&](int rows) { return dim3((unsigned)((W + 63) / 64), (unsigned)((H + rows - 1) / rows), (unsigned)B); };
   switch (cout / 16) {
-    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(256), 0, ctx->stream, A); break;
-    case 12: hipLaunchKernelGGL(k_conv3x3<12>, grid, dim3(256), 0, ctx->stream, A); break;
+    case 1: hipLaunchKernelGGL(k_conv3x3<1>, grid(ConvShape<1>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid(ConvShape<2>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 3: hipLaunchKernelGGL(k_conv3x3<3>, grid(ConvShape<3>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid(ConvShape<4>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 6: hipLaunchKernelGGL(k_conv3x3<6>, grid(ConvShape<6>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid(ConvShape<8>::ROWS), dim3(256), 0, ctx->stream, A); break;
+    case 12: hipLaunchKernelGGL(k_conv3x3<12>, grid(ConvShape<12>::ROWS), dim3(256), 0, ctx->stream, A); break;
     default: return ctx->fail(LEPIC_ERR_ARG, "unsupported channel count");
   }
   LEPIC_HIP(hipGetLastError());
