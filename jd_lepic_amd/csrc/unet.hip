@@ -216,34 +216,37 @@ __global__ __launch_bounds__(256) void k_wgrad(WgradArgs A) {
 // out (Ho=H/2, Wo=W/2, C) = average of 2x2 input pixels (channels in_off..)
 __global__ void k_pool(const bf16_t* __restrict__ in, int in_C, int in_off, bf16_t* __restrict__ out, int C,
                        int B, int Ho, int Wo) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = (int64_t)B * Ho * Wo * C;
-  if (q >= n) return;
-  const int c = (int)(q % C);
-  int64_t r = q / C;
-  const int x = (int)(r % Wo);
-  r /= Wo;
-  const int y = (int)(r % Ho), b = (int)(r / Ho);
+  // a thread averages 8 channels of one output pixel; block row blockIdx.y = b * Ho + y
+  const int nv = C / 8;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= Wo * nv) return;
+  const int x = q / nv, v = q - x * nv;
+  const int y = (int)blockIdx.y % Ho, b = (int)blockIdx.y / Ho;
   const int H = 2 * Ho, W = 2 * Wo;
-  const float s = (bf2f(in[px(b, 2 * y, 2 * x, H, W) * in_C + in_off + c]) +
-                   bf2f(in[px(b, 2 * y, 2 * x + 1, H, W) * in_C + in_off + c])) +
-                  (bf2f(in[px(b, 2 * y + 1, 2 * x, H, W) * in_C + in_off + c]) +
-                   bf2f(in[px(b, 2 * y + 1, 2 * x + 1, H, W) * in_C + in_off + c]));
-  out[px(b, y, x, Ho, Wo) * C + c] = f2bf(0.25f * s);
+  bf16_t p00[8], p01[8], p10[8], p11[8], o[8];
+  *(int4*)p00 = *(const int4*)(in + px(b, 2 * y, 2 * x, H, W) * in_C + in_off + 8 * v);
+  *(int4*)p01 = *(const int4*)(in + px(b, 2 * y, 2 * x + 1, H, W) * in_C + in_off + 8 * v);
+  *(int4*)p10 = *(const int4*)(in + px(b, 2 * y + 1, 2 * x, H, W) * in_C + in_off + 8 * v);
+  *(int4*)p11 = *(const int4*)(in + px(b, 2 * y + 1, 2 * x + 1, H, W) * in_C + in_off + 8 * v);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const float s = (bf2f(p00[k]) + bf2f(p01[k])) + (bf2f(p10[k]) + bf2f(p11[k]));
+    o[k] = f2bf(0.25f * s);
+  }
+  *(int4*)(out + px(b, y, x, Ho, Wo) * C + 8 * v) = *(const int4*)o;
 }
 
-// nearest upsampling of in (H/2, W/2, C) into channels [0, C) of out (H, W, out_C)
+// nearest upsampling of in (H/2, W/2, C) into channels [0, C) of out (H, W,
+// out_C): a thread moves 8 channels (16 B) of one pixel; block row
+// blockIdx.y = b * H + y
 __global__ void k_up(const bf16_t* __restrict__ in, int C, bf16_t* __restrict__ out, int out_C, int B, int H,
                      int W) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t n = (int64_t)B * H * W * C;
-  if (q >= n) return;
-  const int c = (int)(q % C);
-  int64_t r = q / C;
-  const int x = (int)(r % W);
-  r /= W;
-  const int y = (int)(r % H), b = (int)(r / H);
-  out[px(b, y, x, H, W) * out_C + c] = in[px(b, y / 2, x / 2, H / 2, W / 2) * C + c];
+  const int nv = C / 8;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= W * nv) return;
+  const int x = q / nv, v = q - x * nv;
+  const int y = (int)blockIdx.y % H, b = (int)blockIdx.y / H;
+  *(int4*)(out + px(b, y, x, H, W) * out_C + 8 * v) = *(const int4*)(in + px(b, y / 2, x / 2, H / 2, W / 2) * C + 8 * v);
 }
 
 // gradient of the upsampling: sum of the 4 fine gradients (channels [0, C)
@@ -679,16 +682,16 @@ int wgrad_layer(lepic_ctx* ctx, Net& N, int l, const T4& x, int x_off, const T4&
 
 int pool(lepic_ctx* ctx, Net& N, const T4& in, int in_off, const T4& out, int C, int lvl_out) {
   const int Ho = N.H >> lvl_out, Wo = N.W >> lvl_out;
-  hipLaunchKernelGGL(k_pool, dim3(nblk((int64_t)N.B * Ho * Wo * C)), dim3(256), 0, ctx->stream, in.p, in.C, in_off,
-                     out.p, C, N.B, Ho, Wo);
+  hipLaunchKernelGGL(k_pool, dim3(nblk((int64_t)Wo * (C / 8)), (unsigned)(N.B * Ho)), dim3(256), 0, ctx->stream, in.p,
+                     in.C, in_off, out.p, C, N.B, Ho, Wo);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
 
 int upsample(lepic_ctx* ctx, Net& N, const T4& in, int C, const T4& out, int lvl_out) {
   const int H = N.H >> lvl_out, W = N.W >> lvl_out;
-  hipLaunchKernelGGL(k_up, dim3(nblk((int64_t)N.B * H * W * C)), dim3(256), 0, ctx->stream, in.p, C, out.p, out.C,
-                     N.B, H, W);
+  hipLaunchKernelGGL(k_up, dim3(nblk((int64_t)W * (C / 8)), (unsigned)(N.B * H)), dim3(256), 0, ctx->stream, in.p, C,
+                     out.p, out.C, N.B, H, W);
   LEPIC_HIP(hipGetLastError());
   return 0;
 }
