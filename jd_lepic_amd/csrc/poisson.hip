@@ -161,6 +161,24 @@ __device__ __forceinline__ void prolong_node(PU u, PC bc, int n1, PE e2h, int m1
   if (bc[c - 1] <= 0) u[c - 1] = u[c - 1] + (r1 + r4) / (T)2.0;
 }
 
+// The correction prolong_node adds to fine node (I, J) (1..n1+1, 1..n2+1):
+// the coarse node (I/2 + 1, J/2 + 1) and its west / south neighbours, with
+// prolong_node's expressions.
+template <typename T>
+__device__ __forceinline__ T prolong_at(const T* e2h, int m1, int I, int J) {
+  const int64_t Wc = m1 + 3;
+  const int ci = I / 2 + 1, cj = J / 2 + 1;
+  const T r1 = e2h[(int64_t)cj * Wc + ci];
+  if (I & 1) {
+    if (J & 1) return r1;
+    return (r1 + e2h[(int64_t)(cj - 1) * Wc + ci]) / (T)2.0;
+  }
+  const T r4 = e2h[(int64_t)cj * Wc + ci - 1];
+  if (J & 1) return (r1 + r4) / (T)2.0;
+  const T r2 = e2h[(int64_t)(cj - 1) * Wc + ci], r3 = e2h[(int64_t)(cj - 1) * Wc + ci - 1];
+  return (r1 + r2 + r3 + r4) / (T)4.0;
+}
+
 // ---------------------------------------------------------------------------
 // visit control of multi-block levels
 // ---------------------------------------------------------------------------
@@ -272,6 +290,12 @@ struct SweepArgs {
   int fixed;               // run the launch's sweeps without the exit test (mixed form)
   int nores;               // no residuals (a visit whose sweep count nothing reads back): only the last state is written
   double eps;
+  // mixed form, first launch of an up visit: the coarse level's correction
+  // is added to the region as it is loaded (the prolongation of
+  // mg.f90:585-611, no kernel of its own); pcrot / pvc as in k_prolong
+  int pro, pro_mb, pm1;
+  T* pcrot[kRot];
+  VisitRef pvc;
 };
 
 // Sweeps k0+1 .. k0+kexec (kexec <= KS) of a visit on one TX x TY tile with
@@ -291,13 +315,19 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
   __shared__ T su[RN];
   __shared__ T sb[RN];
   __shared__ int8_t sbc[RN];
-  __shared__ int s_go, s_c0;
+  __shared__ int s_go, s_c0, s_pb;
   __shared__ double ws[kSweepThreads / 64];
   const int nb = A.nbx * A.nby;
   const int bid = blockIdx.y * A.nbx + blockIdx.x;
+  const bool pro = A.pro && k0 == 0;
   if (threadIdx.x < 64) {
     LevelCtrl* c = A.ctrl;
     int go = 1;
+    int pb = 0;
+    if (pro && A.pro_mb) {  // the coarse level's up visit, as k_prolong resolves it
+      Resolved r = resolve_visit(A.pvc, bid == 0, false);
+      pb = r.buf < 0 ? 0 : r.buf;
+    }
     const int c0 = c->cur[(A.epoch - 1) & 1];
     if (A.fixed) {
       go = 1;
@@ -318,6 +348,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
     if (threadIdx.x == 0) {
       s_go = go;
       s_c0 = c0;
+      s_pb = pb;
     }
   }
   __syncthreads();
@@ -330,11 +361,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
   const int gi0 = 1 + (int)blockIdx.x * kTX - kHalo;  // global i of region column 0
   const int gj0 = 1 + (int)blockIdx.y * TY - kHalo;
   const T* uin = A.rot[(s_c0 + k0) % kRot];
+  const T* e2h = pro ? A.pcrot[s_pb] : nullptr;
   {
     // every load of the region is issued before any LDS store: one memory
     // latency for the whole region instead of one per loop trip
     constexpr int kIt = (RN + kSweepThreads - 1) / kSweepThreads;
-    T lu[kIt], lb[kIt];
+    T lu[kIt], lb[kIt], lp[kIt];
     int8_t lc[kIt];
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
@@ -345,10 +377,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
       lu[it] = in ? uin[(int64_t)j * W + i] : (T)0;
       lc[it] = in ? A.bc[(int64_t)j * W + i] : (int8_t)1;
       lb[it] = in ? A.b[(int64_t)(j - 1) * (n1 + 1) + (i - 1)] : (T)0;
+      lp[it] = e2h ? prolong_at<T>(e2h, A.pm1, in ? i : 1, in ? j : 1) : (T)0;
     }
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
       const int q = threadIdx.x + it * kSweepThreads;
+      if (e2h && lc[it] <= 0) lu[it] = lu[it] + lp[it];   // prolong_node: bcnd <= 0 nodes
       if (q < RN) { su[q] = lu[it]; sbc[q] = lc[it]; sb[q] = lb[it]; }
     }
   }
@@ -1246,9 +1280,25 @@ void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k,
 // sweeps and returns at once when the visit has stopped there
 template <typename T>
 int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end,
-                 int first = kS, int fixed = 0) {
+                 int first = kS, int fixed = 0, bool pro = false) {
   Level& L = ctx->lv[l];
   SweepArgs<T> A;
+  A.pro = pro ? 1 : 0;
+  A.pro_mb = 0;
+  A.pm1 = L.n1 / 2;
+  A.pvc = VisitRef{};
+  if (pro) {  // the up visit's first launch takes the prolongation from level l + 1
+    Level& C = ctx->lv[l + 1];
+    for (int r = 0; r < kRot; ++r) A.pcrot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
+    A.pro_mb = C.mb ? 1 : 0;
+    if (C.mb) {
+      const VisitPlan pc = visit_plan(C, true, f32);
+      A.pvc = visit_ref(ctx, l + 1, epoch, ctx->ng - (l + 1), kNr, 2 * ctx->ng - (l + 1), eps, pc.k_end, pc.klast0,
+                        pc.fixed);
+    }
+  } else {
+    for (int r = 0; r < kRot; ++r) A.pcrot[r] = nullptr;
+  }
   for (int r = 0; r < kRot; ++r) A.rot[r] = rot_of<T>(L, r);
   A.b = b_of<T>(L, f32);
   A.bc = L.bc;
@@ -1407,9 +1457,11 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
   }
   if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
   for (int l = Ls - 1; l >= 1; --l) {  // up leg
-    if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
+    // FP64: the prolongation kernel; mixed: folded into the up visit's first launch
+    if (!f32)
+      if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
     const VisitPlan pl = visit_plan(ctx->lv[l], true, f32);
-    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, pl.k_end, pl.first, pl.fixed)) return rc;
+    if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, pl.k_end, pl.first, pl.fixed, f32)) return rc;
   }
   Level& L1 = ctx->lv[1];
   const unsigned gx = (unsigned)((L1.n1 + 1 + 255) / 256);
