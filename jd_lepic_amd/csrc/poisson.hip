@@ -106,6 +106,13 @@ __device__ __forceinline__ T getres(PU u, PB b, int ir, int jr, int n1, T aw, T 
   return urhs - (aw * uw + aw * ue + an * un + an * us + ac * uij);
 }
 
+// getres of the node at index x of an LDS region with rows of RW (same
+// expression and order)
+template <typename T>
+__device__ __forceinline__ T lds_getres(const T* su, const T* sb, int x, int RW, T aw, T an, T ac) {
+  return sb[x] - (aw * su[x - 1] + aw * su[x + 1] + an * su[x + RW] + an * su[x - RW] + ac * su[x]);
+}
+
 // mg.f90:386-414: the weighted sum of the centre residual g9 and the eight
 // around it (g1 north-east, then clockwise), each counted when its flag holds
 template <typename T>
@@ -296,14 +303,22 @@ struct SweepArgs {
   int pro, pro_mb, pm1;
   T* pcrot[kRot];
   VisitRef pvc;
+  // mixed form, last launch of a down visit: the restriction onto the level
+  // below (k_restrict's work, from the final state in LDS; the launch's halo
+  // is one sweep wider than its sweeps need) and the fine visit's record
+  int rst;
+  T* r2h;
+  const int8_t* bc2h;
+  T* rcrot[kRot];
+  LevelCtrl* rcctrl;       // null when the coarse level runs in k_tail
 };
 
 // Sweeps k0+1 .. k0+kexec (kexec <= KS) of a visit on one TX x TY tile with
 // NT threads (see the file comment).  A launch that continues a visit first
 // takes the reference's exit test for the sweeps of the visit's previous
 // launch, kchk .. k0.
-template <typename T, int kTX, int TY, int kSweepThreads, int KS>
-__global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0, int kexec, int kchk) {
+template <typename T, int kTX, int TY, int kSweepThreads, int KS, bool RST = false>
+__global__ __launch_bounds__(kSweepThreads, RST ? 8 : 1) void k_sweeps(SweepArgs<T> A, int k0, int kexec, int kchk) {
   constexpr int kHalo = Halo<KS>::v;
   // a block's tile is kTX x TY nodes, or one node wider (taller) in the last
   // column (row) of tiles when the level's node count leaves one node over
@@ -315,7 +330,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
   __shared__ T su[RN];
   __shared__ T sb[RN];
   __shared__ int8_t sbc[RN];
-  __shared__ int s_go, s_c0, s_pb;
+  __shared__ int s_go, s_c0, s_pb, s_rb;
   __shared__ double ws[kSweepThreads / 64];
   const int nb = A.nbx * A.nby;
   const int bid = blockIdx.y * A.nbx + blockIdx.x;
@@ -349,6 +364,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
       s_go = go;
       s_c0 = c0;
       s_pb = pb;
+      s_rb = (A.rst && A.rcctrl) ? A.rcctrl->cur[(A.epoch - 1) & 1] : 0;   // k_restrict's coarse buffer
     }
   }
   __syncthreads();
@@ -455,6 +471,55 @@ __global__ __launch_bounds__(kSweepThreads) void k_sweeps(SweepArgs<T> A, int k0
     if (threadIdx.x < 64) {  // fixed-order sum of the 16 wave sums
       const double t = wave_sum(threadIdx.x < kSweepThreads / 64 ? ws[threadIdx.x] : 0.0);
       if (threadIdx.x == 0) A.partials[(size_t)((k0 + s) % kRing) * nb + bid] = t;
+    }
+  }
+  if (RST && A.rst) {  // the last colour pass ended with a barrier
+    // the fine visit's record (resolve_visit of a fixed-length visit)
+    if (bid == 0 && threadIdx.x == 0) A.ctrl->cur[A.epoch & 1] = (s_c0 + k0 + kexec) % kRot;
+    // the coarse nodes whose fine centre (2ci-1, 2cj-1) lies in this tile,
+    // plus the coarse border next to the domain's edge for the zeroing
+    const int m1 = n1 / 2;
+    const int ti0 = 1 + (int)blockIdx.x * kTX, tj0 = 1 + (int)blockIdx.y * TY;
+    const int ci0 = (ti0 + 1) / 2, ci1 = (ti0 + tw) / 2, cj0 = (tj0 + 1) / 2, cj1 = (tj0 + th) / 2;
+    const int zi0 = ci0 - (blockIdx.x == 0), zi1 = ci1 + ((int)blockIdx.x == A.nbx - 1);
+    const int zj0 = cj0 - (blockIdx.y == 0), zj1 = cj1 + ((int)blockIdx.y == A.nby - 1);
+    const int zw = zi1 - zi0 + 1, zn = zw * (zj1 - zj0 + 1);
+    T* e2h = A.rcrot[0];  // selected without a dynamic index into the argument
+#pragma unroll
+    for (int r = 1; r < kRot; ++r) e2h = s_rb == r ? A.rcrot[r] : e2h;
+    const int64_t Wc = m1 + 3;
+    for (int q = threadIdx.x; q < zn; q += kSweepThreads) {
+      const int ci = zi0 + q % zw, cj = zj0 + q / zw;
+      e2h[(int64_t)cj * Wc + ci] = (T)0;
+      if (ci < ci0 || ci > ci1 || cj < cj0 || cj > cj1) continue;
+      T out = (T)0;
+      if (A.bc2h[(int64_t)cj * Wc + ci] <= 0) {
+        const int ir = 2 * ci - 1, jr = 2 * cj - 1;
+        // region index of fine node (ir + dx, jr + dy), unclamped (bc flags)
+        // and clamped into the interior (getres, as restrict_node evaluates it)
+        const int xr = (jr - gj0) * RW + (ir - gi0);
+        const int dxe = ir + 1 <= n1 + 1 ? 1 : 0, dxw = ir - 1 >= 1 ? -1 : 0;
+        const int dyn = jr + 1 <= n2 + 1 ? RW : 0, dys = jr - 1 >= 1 ? -RW : 0;
+        const bool xe = ir != n1 + 1, xw = ir != 1, yn = jr != n2 + 1, ys = jr != 1;
+        const bool f1 = xe && yn && sbc[xr + 1 + RW] <= 0;
+        const bool f2 = xe && sbc[xr + 1] <= 0;
+        const bool f3 = sbc[xr + 1 - RW] <= 0 && xe && ys;
+        const bool f4 = sbc[xr - RW] <= 0 && ys;
+        const bool f5 = sbc[xr - 1 - RW] <= 0 && xw && ys;
+        const bool f6 = sbc[xr - 1] <= 0 && xw;
+        const bool f7 = sbc[xr - 1 + RW] <= 0 && xw && yn;
+        const bool f8 = sbc[xr + RW] <= 0 && yn;
+        out = full_weight<T>(lds_getres<T>(su, sb, xr, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxe + dyn, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxe, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxe + dys, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dys, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxw + dys, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxw, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dxw + dyn, RW, aw, an, ac),
+                             lds_getres<T>(su, sb, xr + dyn, RW, aw, an, ac), f1, f2, f3, f4, f5, f6, f7, f8);
+      }
+      A.r2h[(int64_t)(cj - 1) * (m1 + 1) + (ci - 1)] = out;
     }
   }
 }
@@ -1264,9 +1329,22 @@ VisitPlan visit_plan(const Level& L, bool up, bool f32) {
   return VisitPlan{f, first, last_launch_start(0, f, first), 1};
 }
 
+// the launch size KS (halo 2 KS + 1) is kexec, or kexec + 1 for a launch
+// that restricts (its full weighting reads the state two nodes past the tile)
 template <typename T, int kTX, int TY, int NT>
 void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k, int kexec, int kchk) {
   const dim3 g(L.nbx, L.nby);
+  if (A.rst) {
+    // mixed form only: FP64 restricts in its own kernel
+    if constexpr (sizeof(T) == 4) {
+      const int ks = kexec + 1;
+      if (ks <= 3) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 3, true>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+      else if (ks <= 4) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 4, true>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+      else if (ks <= kS) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, kS, true>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+      else hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, kS + 1, true>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
+    }
+    return;
+  }
   if (kexec <= 2) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 2>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
   else if (kexec <= 3) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 3>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
   else if (kexec <= 4) hipLaunchKernelGGL((k_sweeps<T, kTX, TY, NT, 4>), g, dim3(NT), 0, ctx->stream, A, k, kexec, kchk);
@@ -1280,9 +1358,22 @@ void launch_sweeps(lepic_ctx* ctx, const Level& L, const SweepArgs<T>& A, int k,
 // sweeps and returns at once when the visit has stopped there
 template <typename T>
 int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, double eps, int k0, int k_end,
-                 int first = kS, int fixed = 0, bool pro = false) {
+                 int first = kS, int fixed = 0, bool pro = false, bool rst = false) {
   Level& L = ctx->lv[l];
   SweepArgs<T> A;
+  A.rst = 0;
+  if (rst) {  // the down visit's last launch restricts onto level l + 1
+    Level& C = ctx->lv[l + 1];
+    A.r2h = b_of<T>(C, f32);
+    A.bc2h = C.bc;
+    for (int r = 0; r < kRot; ++r) A.rcrot[r] = C.mb ? rot_of<T>(C, r) : u_of<T>(C, f32);
+    A.rcctrl = C.mb ? ctx->d_ctrl + (l + 1) : nullptr;
+  } else {
+    A.r2h = nullptr;
+    A.bc2h = nullptr;
+    for (int r = 0; r < kRot; ++r) A.rcrot[r] = nullptr;
+    A.rcctrl = nullptr;
+  }
   A.pro = pro ? 1 : 0;
   A.pro_mb = 0;
   A.pm1 = L.n1 / 2;
@@ -1315,6 +1406,7 @@ int launch_visit(lepic_ctx* ctx, int l, bool f32, int epoch, int dig, int nr, do
   int kchk = 0;
   for (int k = k0, size = std::min(std::max(first, 1), kS); k < k_end; size = kS) {
     const int kexec = std::min(size, k_end - k);
+    A.rst = (rst && k + kexec == k_end) ? 1 : 0;
     if (L.small_tiles) launch_sweeps<T, kSmallTX, SmallTileY<T>::v, kSmallThreads>(ctx, L, A, k, kexec, kchk);
     else launch_sweeps<T, kTX, TileY<T>::v, kSweepThreads>(ctx, L, A, k, kexec, kchk);
     kchk = k + 1;
@@ -1451,9 +1543,12 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
   const int e_dn = ctx->mg_epoch + 1, e_up = ctx->mg_epoch + 2;
   ctx->mg_epoch += 2;
   for (int l = 1; l < Ls; ++l) {  // down leg
+    // FP64: the restriction kernel; mixed: folded into the visit's last launch
     const VisitPlan pl = visit_plan(ctx->lv[l], false, f32);
-    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, pl.k_end, pl.first, pl.fixed)) return rc;
-    if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
+    if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, pl.k_end, pl.first, pl.fixed, false, f32))
+      return rc;
+    if (!f32)
+      if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
   }
   if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
   for (int l = Ls - 1; l >= 1; --l) {  // up leg
