@@ -18,7 +18,13 @@
 //  * Coarse levels (at most mg_tail_nodes nodes) run in k_tail, one workgroup
 //    of 1024 threads that performs the whole lower part of the V-cycle (down
 //    sweeps, restrictions, the coarsest solve, prolongations and up sweeps),
-//    with the levels that fit in 64 KB of LDS held there.
+//    with the levels that fit in its LDS (up to 158 KB) held there.
+//  * Grid transfers between multi-block levels: FP64 runs them as kernels
+//    between the visits (k_restrict, LDS-tiled; k_prolong).  In mixed mode a
+//    down visit's last sweep launch computes the restriction of its tile from
+//    the region still in LDS (one node of extra halo), and an up visit's
+//    first launch adds the prolonged coarse correction to its region as it
+//    loads it.
 // FP64 mode computes the reference's expressions in their order (no
 // contraction), so a sweep is bit-identical to sors.f90; only the order of
 // the L1 residual sum differs.  Mixed mode runs the same V-cycle in FP32 on
