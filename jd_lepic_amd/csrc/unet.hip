@@ -73,15 +73,50 @@ struct ConvArgs {
 // times all cout output channels.  Input channels are taken in chunks of
 // CC: the chunk of the ROWS + 2 input rows around the output rows (66
 // pixels, border included) and of the weights (cout x 9 taps) are staged in
-// LDS with all their loads in flight, then every MFMA reads both operands
-// from LDS, each weight fragment serving the wave's ROWS rows.  CC and ROWS
-// shrink as cout grows so the LDS stays near 64 KB and the accumulators in
-// registers; rows are padded by 4 channels against bank conflicts.
+// LDS, each thread issuing its 16-byte loads in batches of kStageBatch
+// before any store, then every MFMA reads both operands from LDS, each
+// weight fragment serving the wave's ROWS rows.  The weights are the MFMA's
+// A operand and the pixels its B operand, so a lane ends with 4 consecutive
+// output channels of one pixel and writes them as one 8-byte vector.  CC
+// and ROWS shrink as cout grows so the LDS stays near 64 KB and the
+// accumulators in registers; rows are padded by 4 channels against bank
+// conflicts (8-byte aligned rows: the 16-byte loads are stored as halves).
 template <int NT> struct ConvShape {
   static constexpr int CC = NT <= 4 ? 32 : 16;
   static constexpr int ROWS = NT <= 4 ? 4 : 2;
   static constexpr int PAD = NT <= 8 ? 4 : 0;              // 192 output channels: no room for the padding
 };
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kStageBatch = 8;   // 16-byte loads per thread in flight at once while staging
+
+// Copies n 16-byte vectors into LDS, vector q from src(q) (null: zeros) to
+// dst(q) as two 8-byte halves, by the block's 256 threads; each thread
+// issues up to kStageBatch loads before its stores.
+template <typename Src, typename Dst>
+__device__ __forceinline__ void stage_lds(int n, Src src, Dst dst) {
+  for (int q0 = 0; q0 < n; q0 += 256 * kStageBatch) {
+    bf16x8 v[kStageBatch];
+#pragma unroll
+    for (int k = 0; k < kStageBatch; ++k) {
+      const int q = q0 + (int)threadIdx.x + 256 * k;
+      v[k] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (q < n) {
+        const bf16_t* p = src(q);
+        if (p) v[k] = *(const bf16x8*)p;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kStageBatch; ++k) {
+      const int q = q0 + (int)threadIdx.x + 256 * k;
+      if (q < n) {
+        bf16_t* d = dst(q);
+        *(bf16x4*)d = bf16x4{v[k][0], v[k][1], v[k][2], v[k][3]};
+        *(bf16x4*)(d + 4) = bf16x4{v[k][4], v[k][5], v[k][6], v[k][7]};
+      }
+    }
+  }
+}
 
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs A) {
@@ -99,21 +134,22 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs A) {
     for (int j = 0; j < NT; ++j) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int c = 0; c < A.cin; c += CC) {
     const int cc = min(CC, A.cin - c);                     // a multiple of 16
-    const int nv4 = cc / 4;                                // 8-byte vectors per pixel or weight row
+    const int lv = cc == 32 ? 2 : 1;                       // log2 of the 16-byte vectors per pixel or weight row
+    const int nv = 1 << lv;
     __syncthreads();                                       // the previous chunk's reads are done
-    // input rows y0-1 .. y0+ROWS (below the bottom border: zero), pixels
-    // x0-1 .. x0+64 (beyond the right border: zero)
-    for (int q = t; q < kRin * kPx * nv4; q += 256) {
-      const int v = q % nv4, pq = q / nv4, xx = pq % kPx, r = pq / kPx;
-      const int gx = x0 - 1 + xx, gy = y0 - 1 + r;
-      bf16x4 val = bf16x4{0, 0, 0, 0};
-      if (gx <= A.W && gy <= A.H) val = *(const bf16x4*)(A.in + px(b, gy, gx, A.H, A.W) * A.in_C + A.in_off + c + 4 * v);
-      *(bf16x4*)(s_in + (r * kPx + xx) * CP + 4 * v) = val;
-    }
-    for (int q = t; q < COUT * 9 * nv4; q += 256) {
-      const int v = q % nv4, row = q / nv4;                // row = co * 9 + tap
-      *(bf16x4*)(s_w + row * CP + 4 * v) = *(const bf16x4*)(A.w + (size_t)row * A.cin + c + 4 * v);
-    }
+    // input rows y0-1 .. y0+ROWS, pixels x0-1 .. x0+64 (the tensors' zero
+    // border covers -1; beyond the bottom or right border: zero)
+    stage_lds(kRin * kPx * nv,
+              [&](int q) -> const bf16_t* {
+                const int v = q & (nv - 1), pq = q >> lv, xx = pq % kPx, r = pq / kPx;
+                const int gx = x0 - 1 + xx, gy = y0 - 1 + r;
+                return gx <= A.W && gy <= A.H ? A.in + px(b, gy, gx, A.H, A.W) * A.in_C + A.in_off + c + 8 * v
+                                              : nullptr;
+              },
+              [&](int q) { return s_in + (q >> lv) * CP + 8 * (q & (nv - 1)); });
+    stage_lds(COUT * 9 * nv,
+              [&](int q) -> const bf16_t* { return A.w + (size_t)(q >> lv) * A.cin + c + 8 * (q & (nv - 1)); },
+              [&](int q) { return s_w + (q >> lv) * CP + 8 * (q & (nv - 1)); });
     __syncthreads();
     for (int tap = 0; tap < 9; ++tap) {
       const int dy = tap / 3, dx = tap % 3;                // input row r + dy for output row r, pixel x-1+dx
@@ -127,29 +163,33 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs A) {
         for (int j = 0; j < NT; ++j) {
           const bf16x4 bw = *(const bf16x4*)(wrow + (size_t)16 * j * 9 * CP + k0);
 #pragma unroll
-          for (int r = 0; r < ROWS; ++r) acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r], bw, acc[r][j], 0, 0, 0);
+          for (int r = 0; r < ROWS; ++r) acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw, a[r], acc[r][j], 0, 0, 0);
         }
       }
     }
   }
-  if (x0 + 16 * wv >= A.W) return;
+  // lane (m, g) holds output channels 16 j + 4 g .. + 3 of pixel x0 + 16 wv + m
+  const int xp = x0 + 16 * wv + m;
+  if (xp >= A.W) return;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const int y = y0 + r;
     if (y >= A.H) break;
+    const size_t p = px(b, y, xp, A.H, A.W);
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
+      const int co = 16 * j + 4 * g;
+      bf16x4 mk = bf16x4{1, 1, 1, 1};
+      if (A.mask) mk = *(const bf16x4*)(A.mask + p * A.mask_C + A.mask_off + co);
+      bf16x4 o;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int xp = x0 + 16 * wv + 4 * g + i;
-        if (xp >= A.W) continue;
-        const int co = 16 * j + m;
         float v = acc[r][j][i];
         if (A.relu) v = fmaxf(v, 0.f);
-        const size_t p = px(b, y, xp, A.H, A.W);
-        if (A.mask && !bf_pos(A.mask[p * A.mask_C + A.mask_off + co])) v = 0.f;
-        A.out[p * A.out_C + A.out_off + co] = f2bf(v);
+        if (!bf_pos((bf16_t)mk[i])) v = 0.f;
+        o[i] = (short)f2bf(v);
       }
+      *(bf16x4*)(A.out + p * A.out_C + A.out_off + co) = o;
     }
   }
 }
