@@ -1553,13 +1553,13 @@ int vcycle(lepic_ctx* ctx, bool f32, int k, double eps) {
     const VisitPlan pl = visit_plan(ctx->lv[l], false, f32);
     if (int rc = launch_visit<T>(ctx, l, f32, e_dn, l - ng, kNr, eps, 0, pl.k_end, pl.first, pl.fixed, false, f32))
       return rc;
-    if (!f32)
+    if constexpr (sizeof(T) == 8)
       if (int rc = launch_restrict<T>(ctx, l, f32, e_dn, eps)) return rc;
   }
   if (int rc = launch_tail<T>(ctx, f32, k, eps)) return rc;
   for (int l = Ls - 1; l >= 1; --l) {  // up leg
     // FP64: the prolongation kernel; mixed: folded into the up visit's first launch
-    if (!f32)
+    if constexpr (sizeof(T) == 8)
       if (int rc = launch_prolong<T>(ctx, l, f32, e_up, eps)) return rc;
     const VisitPlan pl = visit_plan(ctx->lv[l], true, f32);
     if (int rc = launch_visit<T>(ctx, l, f32, e_up, ng - l, kNr, eps, 0, pl.k_end, pl.first, pl.fixed, f32)) return rc;
