@@ -134,7 +134,7 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tot,
 }
 
 template <bool kMixed>
-__global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
+__global__ __launch_bounds__(kSegThreads, 8) void k_seg_sort(SegSortArgs A) {
   __shared__ uint32_t s_k[2][kChunk];
   __shared__ uint16_t s_i[2][kChunk];
   __shared__ uint32_t s_wc[4][256];
@@ -191,6 +191,50 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_sort(SegSortArgs A) {
   const uint32_t kmin = 0u, span = (ymax - ymin) * dxs + (xmax - xmin);
   int npass = 0;
   while (npass < 4 && (span >> (8 * npass)) != 0u) ++npass;
+  if (npass <= 1) {  // block-uniform
+    // The compact key is a single 8-bit digit: its counts give every slot's
+    // place directly.  The slot's run is the number of distinct keys below
+    // its own and its rank the number of equal keys before it (the wave's
+    // earlier rounds, earlier waves, then earlier lanes): the stable sort's
+    // runs without moving the keys.
+    for (int e = t; e < 4 * 256; e += kSegThreads) (&s_wc[0][0])[e] = 0u;
+    __syncthreads();
+    uint32_t d[kSegRounds], pos[kSegRounds];
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+      const int q = w * 256 + r * 64 + lane;
+      const bool v = q < cnt;
+      d[r] = v ? (Y[r] - ymin) * dxs + (X[r] - xmin) : 0u;
+      const uint32_t before = v ? s_wc[w][d[r]] : 0u;
+      pos[r] = before + wave_match<8>(v, d[r], s_wc[w]);
+    }
+    __syncthreads();
+    // key t: its count and its run index
+    const uint32_t c0 = s_wc[0][t], c1 = s_wc[1][t], c2 = s_wc[2][t], c3 = s_wc[3][t];
+    const uint32_t ck = c0 + c1 + c2 + c3;
+    uint32_t nrun;
+    const uint32_t run = block_excl_scan(ck > 0u ? 1u : 0u, s_tot, &nrun);
+    // the earlier waves' counts of key t, then the run index, in LDS
+    s_wc[1][t] = c0;
+    s_wc[2][t] = c0 + c1;
+    s_wc[3][t] = c0 + c1 + c2;
+    s_wc[0][t] = run;
+    if (ck > 0u) {
+      A.rkey[base + run] = (t % dxs + xmin) + (uint32_t)A.n1 * (t / dxs + ymin);
+      A.rcnt[base + run] = ck;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSegRounds; ++r) {
+      const int q = w * 256 + r * 64 + lane;
+      if (q < cnt) {
+        const uint32_t rank = (w > 0 ? s_wc[w][d[r]] : 0u) + pos[r];
+        A.meta[base + q] = (s_wc[0][d[r]] << kRankBits) | rank;
+      }
+    }
+    if (t == 0) A.nrun[sg] = nrun;
+    return;
+  }
   if (npass == 0) __syncthreads();   // block-uniform; the passes start with a barrier
   // stable LSD passes over the compact key
   int cur = 0;

@@ -213,6 +213,33 @@ def test_sort_digit_passes(n1, n2, precision):
 
 
 @pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
+def test_sort_of_nearly_sorted_particles(precision):
+    """Particles sorted once and then moved by up to a cell and a half, as
+    between two sorts of a run: each segment's cells then span fewer than
+    256 compact keys, the segment sort's single-digit path; the first three
+    segments' particles share one cell (a key span of zero)."""
+    dom = S.box_domain(64, 40, Bmode=None)
+    parts = S.particles(dom, 100000, 1.0, seed=41, edge_frac=0.0)
+    sp = [L.Species(E_CHG, E_MASS, 1.0, 100000)]
+    with make_ctx(dom, sp, precision=precision) as ctx:
+        ctx.upload(1, parts)
+        ctx.part_sorting(1)
+        moved = ctx.download(1).copy()
+        rng = np.random.default_rng(43)
+        L1, L2 = dom.n1 * dom.h1, dom.n2 * dom.h2
+        moved[:, 0] = np.clip(moved[:, 0] + rng.uniform(-1.5, 1.5, len(moved)) * dom.h1, 0.1 * dom.h1, L1 - 0.1 * dom.h1)
+        moved[:, 1] = np.clip(moved[:, 1] + rng.uniform(-1.5, 1.5, len(moved)) * dom.h2, 0.1 * dom.h2, L2 - 0.1 * dom.h2)
+        moved[:3072, 0] = 10.5 * dom.h1
+        moved[:3072, 1] = 7.5 * dom.h2
+        ctx.upload(1, moved)
+        before = ctx.download(1)
+        pl = ctx.part_sorting(1, want_plist=True)
+        got = ctx.download(1)
+    ref, pref = O.part_sorting(64, 40, (dom.h1, dom.h2), before)
+    assert np.array_equal(got, ref) and np.array_equal(pl, pref)
+
+
+@pytest.mark.parametrize("precision", [L.FP64, L.MIXED])
 def test_sort_of_unpacked_segments(precision):
     """The mover leaves partly filled segments; the sort reads them as they
     are and must give what sorting the packed species gives (and, in FP64,
