@@ -35,7 +35,7 @@ typedef struct lepic_ctx lepic_ctx;
 enum {
   LEPIC_OK = 0,
   LEPIC_ERR_ARG = -1,          /* bad argument or size */
-  LEPIC_ERR_UNSUPPORTED = -2,  /* option outside this build's hot path */
+  LEPIC_ERR_UNSUPPORTED = -2,  /* option outside the library's supported hot path */
   LEPIC_ERR_HIP = -3,          /* HIP runtime error */
   LEPIC_ERR_CAPACITY = -4,     /* particle capacity exceeded (nmax, collisions.f90:187-191) */
   LEPIC_ERR_DIVERGED = -5,     /* MG divergence (iterative_solver.f90:39-51) */

@@ -481,7 +481,7 @@ int lepic_set_boundary(lepic_ctx* ctx, const int32_t* bcnd, const double* phi) {
   for (int64_t i = 0; i < ctx->nodes; ++i) {
     int v = bcnd[i];
     if (v == 0 || v == -2)
-      return ctx->fail(LEPIC_ERR_UNSUPPORTED, "periodic (0) and Neumann (-2) nodes are outside this build");
+      return ctx->fail(LEPIC_ERR_UNSUPPORTED, "periodic (0) and Neumann (-2) nodes are not supported by this library");
     if (v != -1 && (v < 1 || v > ctx->g.ngrid))
       return ctx->fail(LEPIC_ERR_ARG, "bcnd values must be -1 or a wall label 1..ngrid");
   }

@@ -654,7 +654,7 @@ int build_species(lepic_ctx* ctx, SpeciesMcc* out) {
       const int tt = R.col_info[(size_t)(c - 1) * 10 + 1 + n_re];  // col_info(c, ind_nby+n_re)
       S.reac[icol - 1] = c;
       S.ttype[icol - 1] = tt;
-      if (R.charge[tt] != 0.0) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "charged (DSMC) collision targets are outside this build");
+      if (R.charge[tt] != 0.0) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "charged (DSMC) collision targets are not supported by this library");
       nu = nu + R.np_mx[tt] * R.sigv_mx[(size_t)pt * R.ncol + (c - 1)];  // collisions.f90:103
     }
     S.nu_max = std::min(nu, R.nu_uplim[pt]);  // line 106
@@ -689,7 +689,7 @@ namespace lepic {
 int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lepic_mcc_info* info) {
   Reactions& R = ctx->rx;
   if (!R.set) return ctx->fail(LEPIC_ERR_STATE, "lepic_set_reactions not called");
-  if (R.flag_ionrep != 0) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "flag_ionrep >= 1 is outside this build");
+  if (R.flag_ionrep != 0) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "flag_ionrep >= 1 is not supported by this library");
   SpeciesMcc S[kMaxSpecies];
   if (int rc = build_species(ctx, S)) return rc;
   MccTab* dtab = (MccTab*)ctx->d_mcc_tab;
