@@ -158,10 +158,13 @@ def test_poisson_fp64_warm_start_sequence(tail):
             assert np.abs(got - ref_phi).max() <= 1e-12 * np.abs(ref_phi).max(), step
 
 
+# 192 x 96: levels of 193 x 97 ... nodes, whose last tile of a row or column
+# takes the side's odd node, and (tail=100) small-tile levels whose grid
+# transfers run inside the sweep launches
 @pytest.mark.parametrize("tail", [0, 100])
-def test_poisson_mixed_within_1e6_of_fp64(tail):
-    n, ng = 128, 7
-    dom, phi, b = poisson_case(n, n, 31)
+@pytest.mark.parametrize("n1,n2,ng", [(128, 128, 7), (192, 96, 6)])
+def test_poisson_mixed_within_1e6_of_fp64(n1, n2, ng, tail):
+    dom, phi, b = poisson_case(n1, n2, 31)
     with make_ctx(dom, [L.Species(E_CHG, E_MASS, 1.0, 16)], ng=ng, precision=L.MIXED, phi=phi,
                   tail=tail) as ctx:
         ctx.set_grid(L.GRID_RHS, b)
@@ -172,10 +175,10 @@ def test_poisson_mixed_within_1e6_of_fp64(tail):
         ctx.set_grid(L.GRID_RHS, b2)
         info2 = ctx.iterative_solver(eps=1e-10, start=L.START_PREV)
         got2 = ctx.get_grid(L.GRID_PHI)
-    ref = O.iterative_solver(phi, b, dom.bcnd, (dom.h1, dom.h2), n, n, ng, eps=1e-10)
+    ref = O.iterative_solver(phi, b, dom.bcnd, (dom.h1, dom.h2), n1, n2, ng, eps=1e-10)
     assert info["res"] <= 1e-10
     assert np.abs(got - ref["phi"]).max() <= 1e-6 * np.abs(ref["phi"]).max()
-    ref2 = O.iterative_solver(ref["phi"], b2, dom.bcnd, (dom.h1, dom.h2), n, n, ng, eps=1e-10)
+    ref2 = O.iterative_solver(ref["phi"], b2, dom.bcnd, (dom.h1, dom.h2), n1, n2, ng, eps=1e-10)
     assert info2["res"] <= 1e-10
     assert np.abs(got2 - ref2["phi"]).max() <= 1e-6 * np.abs(ref2["phi"]).max()
 
