@@ -682,6 +682,7 @@ int lepic_set_flag_dead(lepic_ctx* ctx, int32_t ptype, const int8_t* flags, int6
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
   LEPIC_HIP(hipMemcpyAsync(ctx->sp[ptype - 1].flag_dead, flags, np, hipMemcpyHostToDevice, ctx->stream));
   LEPIC_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->sp[ptype - 1].dead_maybe = true;
   return 0;
 }
 

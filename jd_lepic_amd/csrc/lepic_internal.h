@@ -91,6 +91,9 @@ struct Species {
   int32_t* cell[2] = {};             // mixed: (iy<<16)|ix, 1-based cell
   float* f[2][6] = {};               // mixed: fx fy z vx vy vz
   int8_t* flag_dead = nullptr;
+  // some flag_dead entry may be set: collisions or lepic_set_flag_dead ran
+  // since the last push (a push consumes and clears every set flag)
+  bool dead_maybe = true;
   double* np = nullptr;              // deposited density (nodes)
   int64_t* acc = nullptr;            // fixed-point deposition accumulator (nodes), scaled into np
   double* np_red = nullptr;

@@ -692,6 +692,7 @@ int mcc_run(lepic_ctx* ctx, int it, int mode, uint64_t seed, int32_t* iseed, lep
   if (R.flag_ionrep != 0) return ctx->fail(LEPIC_ERR_UNSUPPORTED, "flag_ionrep >= 1 is not supported by this library");
   SpeciesMcc S[kMaxSpecies];
   if (int rc = build_species(ctx, S)) return rc;
+  for (int p = 0; p < ctx->nsp; ++p) ctx->sp[p].dead_maybe = true;   // collisions flag their reactants
   MccTab* dtab = (MccTab*)ctx->d_mcc_tab;
   if (int rc = upload_tab(ctx, &dtab)) return rc;
   ctx->d_mcc_tab = dtab;

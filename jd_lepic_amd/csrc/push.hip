@@ -89,6 +89,7 @@ struct MoverCommon {
   int nB1, nB2;
   double hB1, hB2, Bc0, Bc1, Bc2;
   int deposit;
+  int check_dead;               // 0: no flag_dead entry is set, the push skips reading them
   unsigned long long* acc;      // fixed-point density accumulator (int64 per node)
   double k1dep;                 // Nm/(h1*h2*zmax)
   Tally* tally;
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(BT) void k_mover64(MoverCommon a, Soa64 in, Soa64W 
     if (k * BT + t >= cnt_in) continue;
     double xp = in.x[i], yp = in.y[i], zp = in.z[i];
     double vx = in.vx[i], vy = in.vy[i], vz = in.vz[i];
-    if (flag_dead[i] == 1) {  // part_expmover.f90:84-98
+    if (a.check_dead && flag_dead[i] == 1) {  // part_expmover.f90:84-98
       flag_dead[i] = 0;
       tally_dead(a, 0.5 * a.Nm * a.mass * (vx * vx + vy * vy + vz * vz));
       continue;
@@ -664,7 +665,7 @@ __global__ __launch_bounds__(BT) void k_mover_mixed(MoverCommon a, MixedParams m
     const float4 vx4 = *(const float4*)(in.vx + i0), vy4 = *(const float4*)(in.vy + i0);
     const float4 vz4 = *(const float4*)(in.vz + i0);
     const float4 z4 = in.z ? *(const float4*)(in.z + i0) : make_float4(0.f, 0.f, 0.f, 0.f);
-    dead4 = *(const uint32_t*)(flag_dead + i0);
+    dead4 = a.check_dead ? *(const uint32_t*)(flag_dead + i0) : 0u;
     C[0] = c4.x; C[1] = c4.y; C[2] = c4.z; C[3] = c4.w;
     FX[0] = fx4.x; FX[1] = fx4.y; FX[2] = fx4.z; FX[3] = fx4.w;
     FY[0] = fy4.x; FY[1] = fy4.y; FY[2] = fy4.z; FY[3] = fy4.w;
@@ -989,6 +990,7 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   Species& s = ctx->sp[p];
   MoverCommon a = common_args(ctx, p);
   a.deposit = deposit ? 1 : 0;
+  a.check_dead = s.dead_maybe ? 1 : 0;
   const bool mixed = ctx->g.precision == LEPIC_MIXED;
   const int64_t nb = std::max<int64_t>(nseg_of(s), 1);  // one block per segment
   if (deposit)
@@ -1022,6 +1024,7 @@ int mover_launch(lepic_ctx* ctx, int p, int deposit) {
   }
   s.cur = o;
   s.packed = false;
+  s.dead_maybe = false;   // the push cleared every flag it consumed
   if (deposit)
     if (int rc = np_from_acc(ctx, p)) return rc;
   return 0;
