@@ -235,8 +235,8 @@ __global__ __launch_bounds__(kSegThreads, 8) void k_seg_sort(SegSortArgs A) {
     if (t == 0) A.nrun[sg] = nrun;
     return;
   }
-  if (npass == 0) __syncthreads();   // block-uniform; the passes start with a barrier
-  // stable LSD passes over the compact key
+  // stable LSD passes over the compact key (two or more digits; each pass
+  // starts with a barrier, which also orders the key stores above)
   int cur = 0;
   for (int p = 0; p < npass; ++p) {
     const int shift = 8 * p;
